@@ -1,0 +1,228 @@
+"""CPU oracle for the ring-array FDTD hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker. The product
+package ``soundwavesimulation_amd`` never imports it.
+
+Two restatements of the reference scheme live here:
+
+* ``fdtd2d_oracle.c`` (float64, OpenMP), loaded through ctypes from
+  ``oracle/_build/liboracle.so`` (built by ``oracle/Makefile``);
+* a NumPy float64 restatement (:func:`np_step`, :func:`np_forward`) for tiny
+  grids, used to cross-check the C code.
+
+Both follow ``fdtd2d/torch_backend.py:477-598`` (batched torch path),
+``fdtd2d/solver.py:52-124`` and ``fdtd2d/wave.py:109-129`` (NumPy twin) and
+``invert_ring.py:603-701`` (residual, adjoint, imaging).
+
+How each function is pinned (see tests/test_oracle.py):
+  forward traces / gradient / misfit -- golden vectors written by the
+      reference's own float64 torch path (tests/golden/*.npz);
+  Laplacian -- exact on quadratics, zero on linear fields;
+  interior leapfrog -- discrete plane-wave dispersion relation;
+  Mur edge -- exact one-step outflow at c dt = h (k = 0);
+  gather / scatter -- dot-product test <P u, r> = <u, P^T r> and
+      exactness on bilinear fields;
+  adjoint time stepping -- dot-product test against the forward operator on
+      a problem whose waves never reach the boundary.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force=False):
+    """Compile the C oracle with ``make`` (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        lib.oracle_forward.restype = ctypes.c_int
+        lib.oracle_gradient.restype = ctypes.c_int
+        lib.oracle_step.restype = ctypes.c_int
+        lib.oracle_n_threads.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def set_threads(n):
+    load().oracle_set_threads(int(n))
+
+
+def n_threads():
+    return int(load().oracle_n_threads())
+
+
+def laplacian(f, dx, dy=None):
+    f = _f64(f)
+    out = np.empty_like(f)
+    load().oracle_laplacian(f.shape[0], f.shape[1], ctypes.c_double(dx),
+                            ctypes.c_double(dx if dy is None else dy), _d(f), _d(out))
+    return out
+
+
+def step(c, dx, dy, dt, u_old, u, adjoint=False, point=None, dense=None):
+    """One oracle step for one shot; ``point = (row, col, value)``."""
+    c = _f64(c)
+    u_old = _f64(u_old)
+    u = _f64(u)
+    out = np.empty_like(u)
+    row, col, value = (-1, -1, 0.0) if point is None else point
+    dense_p = None
+    if dense is not None:
+        dense = _f64(dense)
+        dense_p = _d(dense)
+    rc = load().oracle_step(c.shape[0], c.shape[1], ctypes.c_double(dx), ctypes.c_double(dy),
+                            ctypes.c_double(dt), _d(c), _d(u_old), _d(u), _d(out),
+                            int(bool(adjoint)), int(row), int(col), ctypes.c_double(value),
+                            dense_p)
+    if rc:
+        raise MemoryError("oracle_step allocation failed")
+    return out
+
+
+def forward(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0, weights,
+            keep_wave=False):
+    """Forward shots. Returns traces (S, R, N) and, optionally, wave (N, S, ny, nx)."""
+    c = _f64(c)
+    ny, nx = c.shape
+    rows, cols = _i32(src_rows), _i32(src_cols)
+    vals = np.zeros(int(n_steps))
+    sv = np.asarray(src_values, dtype=float).ravel()
+    vals[:min(sv.size, vals.size)] = sv[:vals.size]
+    r0, c0, w = _i32(row0), _i32(col0), _f64(weights)
+    S, R = rows.size, r0.size
+    traces = np.empty((S, R, int(n_steps)))
+    wave = np.empty((int(n_steps), S, ny, nx)) if keep_wave else None
+    rc = load().oracle_forward(ny, nx, ctypes.c_double(dx), ctypes.c_double(dy),
+                               ctypes.c_double(dt), _d(c), S, _i(rows), _i(cols), _d(vals),
+                               int(n_steps), R, _i(r0), _i(c0), _d(w), _d(traces),
+                               _d(wave) if keep_wave else None)
+    if rc:
+        raise MemoryError("oracle_forward allocation failed")
+    return traces, wave
+
+
+def gradient(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0, weights,
+             observed, tx, store_every=1):
+    """Misfit J, gradient w.r.t. c^2 (ny, nx) and predicted traces (S, R, N)."""
+    c = _f64(c)
+    ny, nx = c.shape
+    rows, cols = _i32(src_rows), _i32(src_cols)
+    vals = np.zeros(int(n_steps))
+    sv = np.asarray(src_values, dtype=float).ravel()
+    vals[:min(sv.size, vals.size)] = sv[:vals.size]
+    r0, c0, w = _i32(row0), _i32(col0), _f64(weights)
+    obs = _f64(observed)
+    txs = _i32(tx)
+    S, R = rows.size, r0.size
+    if obs.shape != (S, R, int(n_steps)):
+        raise ValueError("observed must have shape (shots, receivers, steps)")
+    J = ctypes.c_double(0.0)
+    g = np.empty((ny, nx))
+    traces = np.empty((S, R, int(n_steps)))
+    rc = load().oracle_gradient(ny, nx, ctypes.c_double(dx), ctypes.c_double(dy),
+                                ctypes.c_double(dt), _d(c), S, _i(rows), _i(cols), _d(vals),
+                                int(n_steps), R, _i(r0), _i(c0), _d(w), _d(obs), _i(txs),
+                                int(store_every), ctypes.byref(J), _d(g), _d(traces))
+    if rc:
+        raise MemoryError("oracle_gradient allocation failed")
+    return float(J.value), g, traces
+
+
+# ---------------------------------------------------------------------------
+# NumPy float64 restatement (tiny grids only)
+# ---------------------------------------------------------------------------
+
+def np_laplacian(f, dx, dy):
+    """Zero-padded 5-point Laplacian (conv2d with padding=1)."""
+    p = np.pad(np.asarray(f, dtype=float), 1)
+    return ((p[1:-1, :-2] + p[1:-1, 2:]) / dx**2 + (p[:-2, 1:-1] + p[2:, 1:-1]) / dy**2
+            - 2.0 * (1.0 / dx**2 + 1.0 / dy**2) * p[1:-1, 1:-1])
+
+
+def np_mur(c, dx, dy, dt, u, un):
+    """Mur edges then averaged corners, in place on ``un`` (solver.py:90-124)."""
+    k = lambda cv, h: (cv * dt - h) / (cv * dt + h)  # noqa: E731
+    un[1:-1, 0] = u[1:-1, 1] + k(c[1:-1, 0], dx) * (un[1:-1, 1] - u[1:-1, 0])
+    un[1:-1, -1] = u[1:-1, -2] + k(c[1:-1, -1], dx) * (un[1:-1, -2] - u[1:-1, -1])
+    un[0, 1:-1] = u[1, 1:-1] + k(c[0, 1:-1], dy) * (un[1, 1:-1] - u[0, 1:-1])
+    un[-1, 1:-1] = u[-2, 1:-1] + k(c[-1, 1:-1], dy) * (un[-2, 1:-1] - u[-1, 1:-1])
+    vals = []
+    for r, cc, ri, ci in ((0, 0, 1, 1), (0, -1, 1, -2), (-1, 0, -2, 1), (-1, -1, -2, -2)):
+        cv = c[r, cc]
+        fx = u[r, ci] + k(cv, dx) * (un[r, ci] - u[r, cc])
+        fy = u[ri, cc] + k(cv, dy) * (un[ri, cc] - u[r, cc])
+        vals.append(((r, cc), 0.5 * (fx + fy)))
+    for (r, cc), v in vals:
+        un[r, cc] = v
+    return un
+
+
+def np_step(c, dx, dy, dt, u_old, u, adjoint=False, point=None, dense=None):
+    c = np.asarray(c, dtype=float)
+    c2 = c * c
+    if adjoint:
+        acc = dt * dt * np_laplacian(c2 * u, dx, dy)
+    else:
+        acc = dt * dt * c2 * np_laplacian(u, dx, dy)
+    un = 2.0 * u - u_old + acc
+    if point is not None:
+        un[point[0], point[1]] += point[2]
+    if dense is not None:
+        un = un + dense
+    return np_mur(c, dx, dy, dt, u, un)
+
+
+def np_forward(c, dx, dy, dt, row, col, src_values, n_steps, row0, col0, weights):
+    """Single-shot forward traces (R, N) by the NumPy restatement."""
+    ny, nx = np.shape(c)
+    uo = np.zeros((ny, nx))
+    u = np.zeros((ny, nx))
+    w = np.asarray(weights, dtype=float)
+    r0 = np.asarray(row0)
+    c0 = np.asarray(col0)
+    traces = np.zeros((r0.size, int(n_steps)))
+    sv = np.asarray(src_values, dtype=float)
+    for t in range(int(n_steps)):
+        val = sv[t] if t < sv.size else 0.0
+        un = np_step(c, dx, dy, dt, uo, u, point=(row, col, val))
+        traces[:, t] = (un[r0, c0] * w[0] + un[r0, c0 + 1] * w[1]
+                        + un[r0 + 1, c0] * w[2] + un[r0 + 1, c0 + 1] * w[3])
+        uo, u = u, un
+    return traces
