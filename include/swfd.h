@@ -1,0 +1,210 @@
+/*
+ * swfd -- MI355X (gfx950) kernels for the 2D ring-array FDTD hot path.
+ *
+ * C ABI of libswfd.so. Every pointer is a device pointer unless stated
+ * otherwise; every field is float32, row-major with x (columns) fastest;
+ * batched fields are (n_shots, ny, nx). Streams are hipStream_t passed as
+ * void* (0 = the null stream). Functions return SWFD_OK or an error code;
+ * swfd_last_error() gives the message of the last failure on the calling
+ * thread. The library allocates no device memory: callers own every buffer.
+ *
+ * The reference (scjoshi/SoundWaveSimulation) has no native layer: its hot
+ * path is a sequence of PyTorch ops. Each entry point below names the
+ * reference function(s) whose op sequence it replaces; the Python host layer
+ * (soundwavesimulation_amd.torch_backend) keeps the reference call surface
+ * and calls these entry points through ctypes.
+ */
+#ifndef SWFD_H
+#define SWFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWFD_OK 0
+#define SWFD_EINVAL 1 /* bad argument (shape, null pointer, range) */
+#define SWFD_EHIP 2   /* HIP runtime error */
+
+/* Grid and scheme constants, all as the float32 values the kernels use.
+ * Laplacian taps: inv_dx2 = 1/dx^2, inv_dy2 = 1/dy^2, center = -2 (inv_dx2 +
+ * inv_dy2), each rounded from double (torch_backend.py:122-132); dt2 is dt*dt
+ * rounded from double (torch_backend.py:242). */
+typedef struct swfd_grid {
+    int32_t ny, nx; /* rows (y) and columns (x); both >= 3 */
+    float inv_dx2, inv_dy2, center;
+    float dt2;
+} swfd_grid;
+
+/* Medium coefficients produced by swfd_prepare_medium: c^2, dt^2 c^2, and the
+ * Mur coefficients k_left[ny], k_right[ny], k_bottom[nx], k_top[nx],
+ * corner kx[4], corner ky[4] (corners (0,0), (0,-1), (-1,0), (-1,-1)). */
+typedef struct swfd_medium {
+    const float *c2;    /* (ny, nx) */
+    const float *c2dt2; /* (ny, nx) */
+    const float *mur;   /* 2 ny + 2 nx + 8 */
+} swfd_medium;
+
+/* Bilinear receivers (array.py:60-76) and, for adjoint injection, the
+ * scatter table: cell_map[y*nx+x] = entry id or -1; entry e covers pairs
+ * [entry_start[e], entry_start[e+1]) of (pair_rec, pair_w), summed in that
+ * order; entry_cell[e] = y*nx+x. The table may be absent (cell_map = NULL)
+ * when no adjoint injection or scatter is requested. */
+typedef struct swfd_receivers {
+    int32_t n;               /* R */
+    const int32_t *row0;     /* (R) footprint rows row0, row0+1 */
+    const int32_t *col0;     /* (R) footprint cols col0, col0+1 */
+    const float *weights;    /* (4, R): w00, w10, w01, w11 */
+    int32_t n_entries;       /* E */
+    const int32_t *cell_map; /* (ny*nx) */
+    const int32_t *entry_start; /* (E+1) */
+    const int32_t *entry_cell;  /* (E) */
+    const int32_t *pair_rec;    /* (P) */
+    const float *pair_w;        /* (P) */
+} swfd_receivers;
+
+/* One time step for n_shots shots (torch_backend.py:177-233 single shot,
+ * :493-598 batched): interior leapfrog, sources, Mur edges then corners.
+ * adjoint = 0: u' = 2u - u_old + dt^2 c^2 Lap(u); adjoint = 1: Lap(c^2 u).
+ * Sources, all optional and added before the boundary update:
+ *   point: u'[s, src_rows[s], src_cols[s]] += src_values[src_index];
+ *   dense: u' += dense[s];
+ *   residual (S, R): bilinear scatter of the residual (needs the table).
+ * Imaging (optional, image_field != NULL):
+ *   grad_out = grad_in + image_scale * sum_s u'[s] * Lap(image_field[s]).
+ * samples (optional): gather of u' after the boundary, (S, R).
+ * u_next must not alias u_old or u; u_next_copy (optional) receives u' too. */
+typedef struct swfd_step_args {
+    int32_t n_shots;
+    int32_t adjoint;
+    const float *u_old, *u;
+    float *u_next, *u_next_copy;
+    const int32_t *src_rows, *src_cols;
+    const float *src_values;
+    int32_t src_index;
+    const float *dense;
+    const float *residual;
+    const float *image_field;
+    const float *grad_in;
+    float *grad_out;
+    float image_scale;
+    float *samples;
+} swfd_step_args;
+
+/* A run of forward steps [step_begin, step_begin + n_steps) with point
+ * sources (simulate_shot torch_backend.py:443-474, simulate_shots_batch
+ * :736-770, forward_store(_batch) invert_ring.py:395-455).
+ *   init_old/init_u: state before step_begin (read only).
+ *   work0..2: scratch fields; final_old/final_u (optional) receive the state
+ *   after the last step (they may be work0/work1 or the init buffers).
+ *   src_values: per global step, length >= step_begin + n_steps.
+ *   samples (optional): (N_total, S, R), row n = gather after step n.
+ *   hist (optional): output of step n is stored in slot (n - hist_first) /
+ *   hist_every when n >= hist_first and aligned; hist_slots bounds the slot.
+ *   ckpt (optional): the state before step j*ckpt_every (j >= 1) is stored as
+ *   ckpt slots 2(j-1) (older) and 2(j-1)+1; ckpt_slots bounds the slot. */
+typedef struct swfd_forward_args {
+    int32_t n_shots;
+    int32_t step_begin, n_steps;
+    const float *init_old, *init_u;
+    float *work0, *work1, *work2;
+    float *final_old, *final_u;
+    const int32_t *src_rows, *src_cols;
+    const float *src_values;
+    float *samples;
+    float *hist;
+    int32_t hist_first, hist_every, hist_slots;
+    float *ckpt;
+    int32_t ckpt_every, ckpt_slots;
+} swfd_forward_args;
+
+/* A run of adjoint steps n = step_hi - 1 down to step_lo with residual
+ * injection and imaging (TorchLeastSquaresFWI._adjoint_accumulate_batch,
+ * invert_ring.py:683-701; gradient.py:124-141).
+ *   residual: (N_total, S, R), row n drives step n.
+ *   hist: forward field after step j lives in slot (j - hist_first) /
+ *   hist_every; step n images with the field after step n-1 when n >= 1,
+ *   n-1 >= hist_first and (n-1-hist_first) % hist_every == 0, adding
+ *   image_scale * sum_s lam'[s] * Lap(field) into grad (in place). */
+typedef struct swfd_adjoint_args {
+    int32_t n_shots;
+    int32_t step_hi, step_lo;
+    const float *init_old, *init_u;
+    float *work0, *work1, *work2;
+    float *final_old, *final_u;
+    const float *residual;
+    const float *hist;
+    int32_t hist_first, hist_every, hist_slots;
+    float image_scale;
+    float *grad;
+} swfd_adjoint_args;
+
+/* Thread-local message of the last failure. */
+const char *swfd_last_error(void);
+
+/* Library version string. */
+const char *swfd_version(void);
+
+/* Floats in the medium-coefficient buffer for an (ny, nx) grid. */
+size_t swfd_medium_floats(int32_t ny, int32_t nx);
+
+/* c (ny, nx) -> coef = [c^2 | dt2 c^2 | Mur coefficients]
+ * (TorchScalarWave2D.set_c torch_backend.py:158-168, _refresh_boundaries
+ * :264-275). dt, dx, dy are rounded to float32 as the reference's tensor ops
+ * do. */
+int swfd_prepare_medium(const swfd_grid *grid, const float *c, double dt, double dx,
+                        double dy, float *coef, void *stream);
+
+/* Fill a swfd_medium view of a coefficient buffer (host-only, no launch). */
+void swfd_medium_view(const swfd_grid *grid, const float *coef, swfd_medium *out);
+
+/* One step; see swfd_step_args (TorchFDTD2D._step torch_backend.py:300-329,
+ * TorchFDTD2DBatch.step/.step_record/.adjoint_step_residual/
+ * .adjoint_step_image/.adjoint_step :665-733). */
+int swfd_step(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+              const swfd_step_args *args, void *stream);
+
+/* Bilinear gather (S, ny, nx) -> (S, R) (TorchRingSampler.record/.record_batch
+ * torch_backend.py:368-387, _gather_receivers :540-548). */
+int swfd_gather(const swfd_grid *grid, const swfd_receivers *rec, int32_t n_shots,
+                const float *fields, float *samples, void *stream);
+
+/* Bilinear scatter (S, R) -> (S, ny, nx), output overwritten
+ * (TorchRingSampler.record_adjoint(_batch) torch_backend.py:389-440). */
+int swfd_scatter(const swfd_grid *grid, const swfd_receivers *rec, int32_t n_shots,
+                 const float *residual, float *fields, void *stream);
+
+/* Zero-padded 5-point Laplacian of n fields (TorchScalarWave2D.laplacian
+ * torch_backend.py:170-174). */
+int swfd_laplacian(const swfd_grid *grid, int32_t n_fields, const float *fields, float *out,
+                   void *stream);
+
+/* Multi-step forward run; see swfd_forward_args. */
+int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+                 const swfd_forward_args *args, void *stream);
+
+/* Multi-step adjoint run; see swfd_adjoint_args. */
+int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+                 const swfd_adjoint_args *args, void *stream);
+
+/* Workspace floats needed by swfd_residual. */
+size_t swfd_residual_workspace_floats(int32_t n_steps, int32_t n_shots, int32_t n_rec);
+
+/* Residual and misfit (TorchLeastSquaresFWI._residual_batch and
+ * 0.5 * sum(residual^2), invert_ring.py:603-609,630,651):
+ *   residual[n, s, r] = samples[n, s, r] - observed[obs_index[s], r, n],
+ *   set to 0 where r == mute[s] (mute[s] < 0: no muting);
+ *   *misfit (device float) = 0.5 * sum residual^2, summed in a fixed order.
+ * observed element (e, r, n) is at observed[e*obs_stride_e + r*obs_stride_r + n]. */
+int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                  const float *observed, int64_t obs_stride_e, int64_t obs_stride_r,
+                  const int32_t *obs_index, const int32_t *mute, float *residual,
+                  float *workspace, float *misfit, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWFD_H */

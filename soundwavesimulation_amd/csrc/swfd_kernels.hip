@@ -1,0 +1,1167 @@
+// swfd_kernels.hip -- CDNA4 (gfx950) kernels and the C ABI of libswfd.so.
+//
+// One fused kernel advances every shot of a batch by one leapfrog step:
+// interior update, point / dense / bilinear-residual sources, Mur edges and
+// corners, optional imaging condition, and (in spare workgroups) the bilinear
+// receiver gather of the previous step's field. Fields are float32 (S, ny, nx)
+// row-major, x fastest.
+//
+// Work decomposition: a workgroup owns a tile of 64 columns (one wavefront
+// wide, so every row segment is a coalesced 256-byte access) by waves*RB rows
+// and loops over a group of shots, keeping the medium coefficients of its
+// cells in registers across the shot loop. Each lane walks a column strip of
+// RB rows and reuses the vertical neighbours it already loaded. Tiles are
+// numbered so that all shot groups of a tile, and neighbouring tiles, land on
+// the same XCD (workgroup i runs on XCD i % 8), which keeps each XCD's L2
+// serving the coefficients and the halo rows its tiles share.
+//
+// Boundary cells (outermost ring) are few; their lanes recompute the
+// neighbouring interior values they depend on straight from the read-only
+// inputs, so no workgroup ever waits for another and the output buffer never
+// aliases an input.
+//
+// Arithmetic: the interior update uses fused multiply-adds in one fixed
+// order shared by the main path and the boundary recomputation (so both see
+// bit-identical interior values); the Mur update, the gather and the residual
+// use separately rounded products and sums in the reference's op order.
+// The library is compiled with -ffp-contract=off so that only the explicit
+// __fmaf_rn calls fuse.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "swfd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define SWFD_HIP_CHECK(expr)                                                            \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SWFD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kLanes = 64;
+constexpr int kXcds = 8;
+constexpr int kGatherThreads = 256;
+
+// All kernel inputs of one step. Passed by value (kernarg segment).
+struct StepParams {
+    int ny, nx, n_shots;
+    float idx2, idy2, cen, dt2;
+    const float *c2, *c2dt2, *mur;
+    const float *f_old, *f_cur;
+    float *f_out, *f_out2;
+    // point source
+    const int *src_rows, *src_cols;
+    const float *src_values;
+    int src_index;
+    // dense source (S, ny, nx)
+    const float *dense;
+    // bilinear residual injection
+    const float *resid;
+    const int *cell_map, *entry_start, *pair_rec;
+    const float *pair_w;
+    int n_rec;
+    // imaging
+    const float *img, *g_in;
+    float *g_out;
+    float img_scale;
+    // gather of gat_field (S, ny, nx) into gat_out (S, R)
+    const float *gat_field;
+    float *gat_out;
+    const int *rec_row0, *rec_col0;
+    const float *rec_w;
+    // launch geometry
+    int tiles_x, n_groups, shots_per_group, waves;
+    int n_work, chunk, n_stencil_blocks, n_edge_blocks;
+};
+
+__device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
+                                      float idx2, float idy2, float cen)
+{
+    return __fmaf_rn(cen, c, __fmaf_rn(idy2, s + n, idx2 * (w + e)));
+}
+
+// Sources added to the interior value of cell (y, x) of shot s, in the order
+// point, dense, residual scatter.
+__device__ __forceinline__ float add_sources(const StepParams &p, int s, size_t soff,
+                                             size_t ci, int y, int x, float v)
+{
+    if (p.src_rows != nullptr && y == p.src_rows[s] && x == p.src_cols[s])
+        v += p.src_values[p.src_index];
+    if (p.dense != nullptr) v += p.dense[soff + ci];
+    if (p.resid != nullptr) {
+        const int e = p.cell_map[ci];
+        if (e >= 0) {
+            const float *res = p.resid + (size_t)s * p.n_rec;
+            float acc = 0.0f;
+            for (int k = p.entry_start[e]; k < p.entry_start[e + 1]; ++k)
+                acc += res[p.pair_rec[k]] * p.pair_w[k];
+            v += acc;
+        }
+    }
+    return v;
+}
+
+// Interior update of one cell read straight from global memory (used for the
+// neighbours of boundary cells). Same arithmetic as the main path.
+template <bool ADJ>
+__device__ float interior_value(const StepParams &p, int s, size_t soff, int y, int x)
+{
+    const int W = p.nx, H = p.ny;
+    const float *u = p.f_cur + soff;
+    const size_t i = (size_t)y * W + x;
+    const float uc = u[i];
+    const float uw = x > 0 ? u[i - 1] : 0.0f;
+    const float ue = x < W - 1 ? u[i + 1] : 0.0f;
+    const float us = y > 0 ? u[i - W] : 0.0f;
+    const float un = y < H - 1 ? u[i + W] : 0.0f;
+    const float uo = p.f_old[soff + i];
+    float v;
+    if (!ADJ) {
+        v = __fmaf_rn(p.c2dt2[i], lap5(uw, ue, us, un, uc, p.idx2, p.idy2, p.cen),
+                      2.0f * uc - uo);
+    } else {
+        const float *c2 = p.c2;
+        const float pw = x > 0 ? c2[i - 1] * uw : 0.0f;
+        const float pe = x < W - 1 ? c2[i + 1] * ue : 0.0f;
+        const float ps = y > 0 ? c2[i - W] * us : 0.0f;
+        const float pn = y < H - 1 ? c2[i + W] * un : 0.0f;
+        v = __fmaf_rn(p.dt2, lap5(pw, pe, ps, pn, c2[i] * uc, p.idx2, p.idy2, p.cen),
+                      2.0f * uc - uo);
+    }
+    return add_sources(p, s, soff, i, y, x, v);
+}
+
+// Mur update of an edge cell that is not a corner (solver.py:99-109).
+template <bool ADJ>
+__device__ float edge_value(const StepParams &p, int s, size_t soff, int y, int x)
+{
+    const int W = p.nx, H = p.ny;
+    const float *u = p.f_cur + soff;
+    if (x == 0) {
+        const float k = p.mur[y];
+        return u[(size_t)y * W + 1] +
+               k * (interior_value<ADJ>(p, s, soff, y, 1) - u[(size_t)y * W]);
+    }
+    if (x == W - 1) {
+        const float k = p.mur[H + y];
+        return u[(size_t)y * W + W - 2] +
+               k * (interior_value<ADJ>(p, s, soff, y, W - 2) - u[(size_t)y * W + W - 1]);
+    }
+    if (y == 0) {
+        const float k = p.mur[2 * H + x];
+        return u[(size_t)W + x] + k * (interior_value<ADJ>(p, s, soff, 1, x) - u[x]);
+    }
+    const float k = p.mur[2 * H + W + x];
+    return u[(size_t)(H - 2) * W + x] +
+           k * (interior_value<ADJ>(p, s, soff, H - 2, x) - u[(size_t)(H - 1) * W + x]);
+}
+
+// Mur update of any boundary cell; corners average an x- and a y-update that
+// read the updated edge cells next to them (solver.py:111-124).
+template <bool ADJ>
+__device__ float boundary_value(const StepParams &p, int s, size_t soff, int y, int x)
+{
+    const int W = p.nx, H = p.ny;
+    const bool ex = (x == 0) || (x == W - 1);
+    const bool ey = (y == 0) || (y == H - 1);
+    if (!(ex && ey)) return edge_value<ADJ>(p, s, soff, y, x);
+    const float *u = p.f_cur + soff;
+    const int ci = (y == 0 ? 0 : 2) + (x == 0 ? 0 : 1);
+    const int r_in = (y == 0) ? 1 : H - 2;
+    const int c_in = (x == 0) ? 1 : W - 2;
+    const float uc = u[(size_t)y * W + x];
+    const float *kx = p.mur + 2 * H + 2 * W;
+    const float *ky = kx + 4;
+    const float from_x =
+        u[(size_t)y * W + c_in] + kx[ci] * (edge_value<ADJ>(p, s, soff, y, c_in) - uc);
+    const float from_y =
+        u[(size_t)r_in * W + x] + ky[ci] * (edge_value<ADJ>(p, s, soff, r_in, x) - uc);
+    return 0.5f * (from_x + from_y);
+}
+
+// Bilinear gather of (s, r) pairs, in the reference's product/sum order.
+__device__ __forceinline__ void gather_pairs(const StepParams &p, int t)
+{
+    const int R = p.n_rec;
+    if (t >= p.n_shots * R) return;
+    const int s = t / R;
+    const int r = t - s * R;
+    const int W = p.nx;
+    const float *f = p.gat_field + (size_t)s * ((size_t)p.ny * W);
+    const size_t i = (size_t)p.rec_row0[r] * W + p.rec_col0[r];
+    const float *w = p.rec_w;
+    float acc = f[i] * w[r];
+    acc = acc + f[i + 1] * w[R + r];
+    acc = acc + f[i + W] * w[2 * R + r];
+    acc = acc + f[i + W + 1] * w[3 * R + r];
+    p.gat_out[t] = acc;
+}
+
+// Boundary ring of one (shot, cell) item: item t covers shot t / NB and the
+// (t % NB)-th boundary cell, NB = 2 (nx + ny) - 4, ordered bottom row, top
+// row, left column, right column. Kept out of line and called before any
+// stencil state is live, so it does not raise the stencil's register budget.
+template <bool ADJ>
+__device__ __noinline__ void edge_work(const StepParams &p, int t)
+{
+    const int W = p.nx, H = p.ny;
+    const int nb = 2 * (W + H) - 4;
+    if (t >= nb * p.n_shots) return;
+    const int s = t / nb;
+    int i = t - s * nb;
+    int y, x;
+    if (i < W) {
+        y = 0;
+        x = i;
+    } else if (i < 2 * W) {
+        y = H - 1;
+        x = i - W;
+    } else if (i < 2 * W + H - 2) {
+        x = 0;
+        y = 1 + (i - 2 * W);
+    } else {
+        x = W - 1;
+        y = 1 + (i - 2 * W - (H - 2));
+    }
+    const size_t soff = (size_t)s * H * W;
+    const float v = boundary_value<ADJ>(p, s, soff, y, x);
+    const size_t ci = soff + (size_t)y * W + x;
+    p.f_out[ci] = v;
+    if (p.f_out2 != nullptr) p.f_out2[ci] = v;
+}
+
+// Feature flags of a specialised step kernel; the host derives them from the
+// non-null pointers of StepParams, so the generic boundary path (which tests
+// the pointers at run time) always agrees with the specialised main path.
+enum : unsigned {
+    kPoint = 1u,
+    kDense = 2u,
+    kResid = 4u,
+    kImage = 8u,
+    kOut2 = 16u,
+    kGather = 32u,
+};
+
+template <bool ADJ, int RB, unsigned F>
+__global__ void __launch_bounds__(256) step_kernel(StepParams p)
+{
+    if ((int)blockIdx.x < p.n_edge_blocks) {
+        edge_work<ADJ>(p, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+        return;
+    }
+    const int b = blockIdx.x - p.n_edge_blocks;
+    if constexpr ((F & kGather) != 0) {
+        if (b >= p.n_stencil_blocks) {
+            gather_pairs(p, (b - p.n_stencil_blocks) * (int)blockDim.x + (int)threadIdx.x);
+            return;
+        }
+    }
+    // XCD-aware numbering: consecutive work items stay on one XCD.
+    const int w = (b % kXcds) * p.chunk + b / kXcds;
+    if (w >= p.n_work) return;
+    const int tile = w / p.n_groups;
+    const int grp = w - tile * p.n_groups;
+    const int ty = tile / p.tiles_x;
+    const int tx = tile - ty * p.tiles_x;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int W = p.nx, H = p.ny;
+    const int x = tx * kLanes + lane;
+    const int y0 = (ty * p.waves + wave) * RB;
+    if (x >= W || y0 >= H) return;
+    const size_t plane = (size_t)H * W;
+
+    // Medium coefficients of this strip, kept across the shot loop.
+    float coef[RB];
+    float c2col[RB + 2], c2w[RB], c2e[RB];
+    int cmap[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        const int yy = min(y0 + i, H - 1);
+        const size_t ci = (size_t)yy * W + x;
+        if constexpr (!ADJ) {
+            coef[i] = p.c2dt2[ci];
+        } else {
+            c2w[i] = x > 0 ? p.c2[ci - 1] : 0.0f;
+            c2e[i] = x < W - 1 ? p.c2[ci + 1] : 0.0f;
+        }
+        if constexpr ((F & kResid) != 0) cmap[i] = p.cell_map[ci];
+    }
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int i = 0; i < RB + 2; ++i) {
+            const int yy = y0 - 1 + i;
+            c2col[i] = (yy >= 0 && yy < H) ? p.c2[(size_t)yy * W + x] : 0.0f;
+        }
+    }
+    float gacc[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) gacc[i] = 0.0f;
+
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const float *u = p.f_cur + soff;
+        const float *uo = p.f_old + soff;
+        float col[RB + 2];
+#pragma unroll
+        for (int i = 0; i < RB + 2; ++i) {
+            const int yy = y0 - 1 + i;
+            col[i] = (yy >= 0 && yy < H) ? u[(size_t)yy * W + x] : 0.0f;
+        }
+        int sr = -1, sc = -1;
+        if constexpr ((F & kPoint) != 0) {
+            sr = p.src_rows[s];
+            sc = p.src_cols[s];
+        }
+        float v[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            v[i] = 0.0f;
+            if (yy < H) {
+                const size_t ci = (size_t)yy * W + x;
+                const float uw = x > 0 ? u[ci - 1] : 0.0f;
+                const float ue = x < W - 1 ? u[ci + 1] : 0.0f;
+                const float old = uo[ci];
+                float val;
+                if constexpr (!ADJ) {
+                    val = __fmaf_rn(coef[i],
+                                    lap5(uw, ue, col[i], col[i + 2], col[i + 1], p.idx2,
+                                         p.idy2, p.cen),
+                                    2.0f * col[i + 1] - old);
+                } else {
+                    const float pw = c2w[i] * uw;
+                    const float pe = c2e[i] * ue;
+                    const float ps = c2col[i] * col[i];
+                    const float pn = c2col[i + 2] * col[i + 2];
+                    val = __fmaf_rn(p.dt2,
+                                    lap5(pw, pe, ps, pn, c2col[i + 1] * col[i + 1], p.idx2,
+                                         p.idy2, p.cen),
+                                    2.0f * col[i + 1] - old);
+                }
+                if constexpr ((F & kPoint) != 0) {
+                    if (yy == sr && x == sc) val += p.src_values[p.src_index];
+                }
+                if constexpr ((F & kDense) != 0) val += p.dense[soff + ci];
+                if constexpr ((F & kResid) != 0) {
+                    if (cmap[i] >= 0) {
+                        const float *res = p.resid + (size_t)s * p.n_rec;
+                        float acc = 0.0f;
+                        for (int k = p.entry_start[cmap[i]]; k < p.entry_start[cmap[i] + 1];
+                             ++k)
+                            acc += res[p.pair_rec[k]] * p.pair_w[k];
+                        val += acc;
+                    }
+                }
+                v[i] = val;
+            }
+        }
+        float *out = p.f_out + soff;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            const bool boundary = (x == 0) || (x == W - 1) || (yy == 0) || (yy == H - 1);
+            if (yy < H && !boundary) {
+                out[(size_t)yy * W + x] = v[i];
+                if constexpr ((F & kOut2) != 0) p.f_out2[soff + (size_t)yy * W + x] = v[i];
+            }
+        }
+        // Imaging of the previous (later-in-time) adjoint step: its field is
+        // this step's input u, already Mur-updated, held in col[].
+        if constexpr (ADJ && (F & kImage) != 0) {
+            const float *f = p.img + soff;
+            float fc[RB + 2];
+#pragma unroll
+            for (int i = 0; i < RB + 2; ++i) {
+                const int yy = y0 - 1 + i;
+                fc[i] = (yy >= 0 && yy < H) ? f[(size_t)yy * W + x] : 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const int yy = y0 + i;
+                if (yy < H) {
+                    const size_t ci = (size_t)yy * W + x;
+                    const float fw = x > 0 ? f[ci - 1] : 0.0f;
+                    const float fe = x < W - 1 ? f[ci + 1] : 0.0f;
+                    const float lf =
+                        lap5(fw, fe, fc[i], fc[i + 2], fc[i + 1], p.idx2, p.idy2, p.cen);
+                    gacc[i] = __fmaf_rn(col[i + 1], lf, gacc[i]);
+                }
+            }
+        }
+    }
+    if constexpr (ADJ && (F & kImage) != 0) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            if (yy < H) {
+                const size_t ci = (size_t)yy * W + x;
+                p.g_out[ci] = p.g_in[ci] + p.img_scale * gacc[i];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kGatherThreads) gather_kernel(StepParams p)
+{
+    gather_pairs(p, blockIdx.x * (int)blockDim.x + (int)threadIdx.x);
+}
+
+// Stand-alone imaging condition: g_out = g_in + img_scale * sum_s f_cur[s] *
+// Lap(img[s]), with the same per-cell accumulation order as the fused path.
+__global__ void __launch_bounds__(256) image_kernel(StepParams p)
+{
+    const int W = p.nx, H = p.ny;
+    const size_t plane = (size_t)H * W;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane) return;
+    const int y = (int)(i / W);
+    const int x = (int)(i - (size_t)y * W);
+    float acc = 0.0f;
+    for (int s = 0; s < p.n_shots; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const float *f = p.img + soff;
+        const float fw = x > 0 ? f[i - 1] : 0.0f;
+        const float fe = x < W - 1 ? f[i + 1] : 0.0f;
+        const float fs = y > 0 ? f[i - W] : 0.0f;
+        const float fn = y < H - 1 ? f[i + W] : 0.0f;
+        const float lf = lap5(fw, fe, fs, fn, f[i], p.idx2, p.idy2, p.cen);
+        acc = __fmaf_rn(p.f_cur[soff + i], lf, acc);
+    }
+    p.g_out[i] = p.g_in[i] + p.img_scale * acc;
+}
+
+__global__ void prepare_medium_kernel(int ny, int nx, const float *c, float dt, float dx,
+                                      float dy, float dt2, float *c2, float *c2dt2,
+                                      float *mur)
+{
+    const size_t n = (size_t)ny * nx;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float cv = c[i];
+        const float sq = cv * cv;
+        c2[i] = sq;
+        c2dt2[i] = dt2 * sq;
+    }
+    if (i < (size_t)ny) {
+        const float cl = c[i * nx];
+        const float cr = c[i * nx + nx - 1];
+        mur[i] = (cl * dt - dx) / (cl * dt + dx);
+        mur[ny + i] = (cr * dt - dx) / (cr * dt + dx);
+    }
+    if (i < (size_t)nx) {
+        const float cb = c[i];
+        const float ct = c[(size_t)(ny - 1) * nx + i];
+        mur[2 * ny + i] = (cb * dt - dy) / (cb * dt + dy);
+        mur[2 * ny + nx + i] = (ct * dt - dy) / (ct * dt + dy);
+    }
+    if (i < 4) {
+        const size_t r = (i < 2) ? 0 : (size_t)(ny - 1);
+        const size_t cc = (i % 2 == 0) ? 0 : (size_t)(nx - 1);
+        const float cv = c[r * nx + cc];
+        mur[2 * ny + 2 * nx + i] = (cv * dt - dx) / (cv * dt + dx);
+        mur[2 * ny + 2 * nx + 4 + i] = (cv * dt - dy) / (cv * dt + dy);
+    }
+}
+
+__global__ void fill_kernel(float *p, size_t n, float v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void scatter_kernel(int n_shots, size_t plane, int n_entries, int n_rec,
+                               const float *resid, const int *entry_cell,
+                               const int *entry_start, const int *pair_rec,
+                               const float *pair_w, float *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_shots * n_entries) return;
+    const int s = t / n_entries;
+    const int e = t - s * n_entries;
+    const float *res = resid + (size_t)s * n_rec;
+    float acc = 0.0f;
+    for (int k = entry_start[e]; k < entry_start[e + 1]; ++k)
+        acc += res[pair_rec[k]] * pair_w[k];
+    out[(size_t)s * plane + entry_cell[e]] = acc;
+}
+
+__global__ void laplacian_kernel(int ny, int nx, size_t total, float idx2, float idy2,
+                                 float cen, const float *f, float *out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const size_t plane = (size_t)ny * nx;
+    const size_t within = i % plane;
+    const int y = (int)(within / nx);
+    const int x = (int)(within - (size_t)y * nx);
+    const float uw = x > 0 ? f[i - 1] : 0.0f;
+    const float ue = x < nx - 1 ? f[i + 1] : 0.0f;
+    const float us = y > 0 ? f[i - nx] : 0.0f;
+    const float un = y < ny - 1 ? f[i + nx] : 0.0f;
+    out[i] = lap5(uw, ue, us, un, f[i], idx2, idy2, cen);
+}
+
+// residual[n, s, r] = samples[n, s, r] - observed[obs_index[s], r, n] (muted
+// channel zeroed) through a 64x64 LDS transpose tile; per-block partial sums
+// of residual^2 in a fixed tree order.
+constexpr int kResTile = 64;
+__global__ void __launch_bounds__(256) residual_kernel(int N, int S, int R,
+                                                       const float *samples,
+                                                       const float *obs, long long se,
+                                                       long long sr, const int *obs_index,
+                                                       const int *mute, float *resid,
+                                                       float *partials)
+{
+    __shared__ float tile[kResTile][kResTile + 1];
+    __shared__ float red[256];
+    const int tiles_r = (R + kResTile - 1) / kResTile;
+    const int tiles_n = (N + kResTile - 1) / kResTile;
+    int b = blockIdx.x;
+    const int s = b / (tiles_r * tiles_n);
+    b -= s * tiles_r * tiles_n;
+    const int tn = b / tiles_r;
+    const int tr = b - tn * tiles_r;
+    const int r0 = tr * kResTile, n0 = tn * kResTile;
+    const int tx = threadIdx.x & 63, tyy = threadIdx.x >> 6;
+    const float *o = obs + (long long)obs_index[s] * se;
+    for (int rl = tyy; rl < kResTile; rl += 4) {
+        const int r = r0 + rl, n = n0 + tx;
+        tile[rl][tx] = (r < R && n < N) ? o[(long long)r * sr + n] : 0.0f;
+    }
+    __syncthreads();
+    const int m = mute[s];
+    float local = 0.0f;
+    for (int nl = tyy; nl < kResTile; nl += 4) {
+        const int n = n0 + nl, r = r0 + tx;
+        if (n < N && r < R) {
+            const size_t idx = ((size_t)n * S + s) * R + r;
+            float d = samples[idx] - tile[tx][nl];
+            if (r == m) d = 0.0f;
+            resid[idx] = d;
+            local += d * d;
+        }
+    }
+    red[threadIdx.x] = local;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// Compensated (Kahan) sum of the partials by one workgroup: each lane sums a
+// fixed stride, then lane 0 adds the lane sums in order.
+__global__ void __launch_bounds__(256) misfit_kernel(int n_partials, const float *partials,
+                                                     float *misfit)
+{
+    __shared__ float lane_sum[256];
+    __shared__ float lane_err[256];
+    float sum = 0.0f, comp = 0.0f;
+    for (int i = threadIdx.x; i < n_partials; i += 256) {
+        const float yv = partials[i] - comp;
+        const float t = sum + yv;
+        comp = (t - sum) - yv;
+        sum = t;
+    }
+    lane_sum[threadIdx.x] = sum;
+    lane_err[threadIdx.x] = comp;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float total = 0.0f, c = 0.0f;
+        for (int i = 0; i < 256; ++i) {
+            const float yv = (lane_sum[i] - lane_err[i]) - c;
+            const float t = total + yv;
+            c = (t - total) - yv;
+            total = t;
+        }
+        misfit[0] = 0.5f * total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct LaunchShape {
+    int rb, waves, tiles_x, tiles_y, n_groups, shots_per_group;
+};
+
+int g_cu_count = 0;
+
+int cu_count()
+{
+    if (g_cu_count == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            g_cu_count = prop.multiProcessorCount;
+        else
+            g_cu_count = 256;
+    }
+    return g_cu_count;
+}
+
+// Tile shape and shot grouping for an (ny, nx, S) step. With imaging every
+// workgroup owns all shots of its tile (the shot sum stays in registers, in a
+// fixed order), so parallelism comes from tiles alone; otherwise shots are
+// split into groups until the launch holds about 16 waves per CU.
+LaunchShape choose_shape(int ny, int nx, int n_shots, bool image)
+{
+    LaunchShape sh{};
+    sh.tiles_x = (nx + kLanes - 1) / kLanes;
+    const long target_waves = 16L * cu_count();
+    const int rbs[3] = {4, 4, 2};
+    const int wvs[3] = {4, 2, 1};
+    int pick = 0;
+    if (image) {
+        for (pick = 0; pick < 2; ++pick) {
+            const int rows = rbs[pick] * wvs[pick];
+            const long tiles = (long)sh.tiles_x * ((ny + rows - 1) / rows);
+            if (tiles * wvs[pick] >= target_waves) break;
+        }
+    }
+    sh.rb = rbs[pick];
+    sh.waves = wvs[pick];
+    const int rows = sh.rb * sh.waves;
+    sh.tiles_y = (ny + rows - 1) / rows;
+    const long tiles = (long)sh.tiles_x * sh.tiles_y;
+    int groups = 1;
+    if (!image) {
+        const long want = (target_waves + tiles * sh.waves - 1) / (tiles * sh.waves);
+        groups = (int)std::min<long>(n_shots, std::max<long>(1, want));
+    }
+    sh.shots_per_group = (n_shots + groups - 1) / groups;
+    sh.n_groups = (n_shots + sh.shots_per_group - 1) / sh.shots_per_group;
+    return sh;
+}
+
+void fill_geometry(StepParams &p, const LaunchShape &sh)
+{
+    p.tiles_x = sh.tiles_x;
+    p.n_groups = sh.n_groups;
+    p.shots_per_group = sh.shots_per_group;
+    p.waves = sh.waves;
+    p.n_work = sh.tiles_x * sh.tiles_y * sh.n_groups;
+    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    p.n_stencil_blocks = p.chunk * kXcds;
+}
+
+void base_params(StepParams &p, const swfd_grid *g, const swfd_medium *m, int n_shots)
+{
+    std::memset(&p, 0, sizeof p);
+    p.ny = g->ny;
+    p.nx = g->nx;
+    p.n_shots = n_shots;
+    p.idx2 = g->inv_dx2;
+    p.idy2 = g->inv_dy2;
+    p.cen = g->center;
+    p.dt2 = g->dt2;
+    if (m != nullptr) {
+        p.c2 = m->c2;
+        p.c2dt2 = m->c2dt2;
+        p.mur = m->mur;
+    }
+}
+
+void attach_receivers(StepParams &p, const swfd_receivers *rec)
+{
+    if (rec == nullptr) return;
+    p.n_rec = rec->n;
+    p.rec_row0 = rec->row0;
+    p.rec_col0 = rec->col0;
+    p.rec_w = rec->weights;
+    p.cell_map = rec->cell_map;
+    p.entry_start = rec->entry_start;
+    p.pair_rec = rec->pair_rec;
+    p.pair_w = rec->pair_w;
+}
+
+unsigned step_flags(const StepParams &p, bool gather)
+{
+    unsigned f = 0;
+    if (p.src_rows != nullptr) f |= kPoint;
+    if (p.dense != nullptr) f |= kDense;
+    if (p.resid != nullptr) f |= kResid;
+    if (p.img != nullptr) f |= kImage;
+    if (p.f_out2 != nullptr) f |= kOut2;
+    if (gather) f |= kGather;
+    return f;
+}
+
+template <bool ADJ, int RB>
+bool launch_variant(unsigned f, dim3 grid, dim3 block, hipStream_t st, const StepParams &p)
+{
+#define SWFD_CASE(FL)                                                               \
+    case (FL):                                                                      \
+        hipLaunchKernelGGL((step_kernel<ADJ, RB, (FL)>), grid, block, 0, st, p);  \
+        return true;
+    if constexpr (!ADJ) {
+        switch (f) {
+            SWFD_CASE(0u)
+            SWFD_CASE(kPoint)
+            SWFD_CASE(kPoint | kGather)
+            SWFD_CASE(kPoint | kOut2)
+            SWFD_CASE(kPoint | kOut2 | kGather)
+            SWFD_CASE(kDense)
+            SWFD_CASE(kPoint | kDense)
+            default:
+                return false;
+        }
+    } else {
+        switch (f) {
+            SWFD_CASE(0u)
+            SWFD_CASE(kResid)
+            SWFD_CASE(kResid | kImage)
+            SWFD_CASE(kDense)
+            SWFD_CASE(kDense | kImage)
+            SWFD_CASE(kImage)
+            SWFD_CASE(kPoint)
+            SWFD_CASE(kPoint | kDense)
+            default:
+                return false;
+        }
+    }
+#undef SWFD_CASE
+}
+
+int launch_step(StepParams &p, bool adjoint, hipStream_t stream, int gather_pairs_count)
+{
+    const bool image = adjoint && p.img != nullptr;
+    const LaunchShape sh = choose_shape(p.ny, p.nx, p.n_shots, image);
+    fill_geometry(p, sh);
+    int gather_blocks = gather_pairs_count > 0 ? (gather_pairs_count + 255) / 256 : 0;
+    const dim3 block(kLanes * sh.waves);
+    if (gather_blocks > 0 && block.x != 256) {
+        // gather workgroups assume 256 lanes: use a separate launch instead
+        hipLaunchKernelGGL(gather_kernel, dim3(gather_blocks), dim3(256), 0, stream, p);
+        gather_blocks = 0;
+    }
+    const long edge_items = (long)(2 * (p.nx + p.ny) - 4) * p.n_shots;
+    p.n_edge_blocks = (int)((edge_items + block.x - 1) / block.x);
+    const dim3 grid(p.n_edge_blocks + p.n_stencil_blocks + gather_blocks);
+    const unsigned f = step_flags(p, gather_blocks > 0);
+    bool ok;
+    if (adjoint)
+        ok = sh.rb == 4 ? launch_variant<true, 4>(f, grid, block, stream, p)
+                        : launch_variant<true, 2>(f, grid, block, stream, p);
+    else
+        ok = sh.rb == 4 ? launch_variant<false, 4>(f, grid, block, stream, p)
+                        : launch_variant<false, 2>(f, grid, block, stream, p);
+    if (!ok) return fail(SWFD_EINVAL, "unsupported combination of step sources / outputs");
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+// Imaging of the field `lam` (S, ny, nx) against `img` into g_out.
+int launch_image(const swfd_grid *grid, int n_shots, const float *lam, const float *img,
+                 const float *g_in, float *g_out, float scale, hipStream_t stream)
+{
+    StepParams q;
+    base_params(q, grid, nullptr, n_shots);
+    q.f_cur = lam;
+    q.img = img;
+    q.g_in = g_in;
+    q.g_out = g_out;
+    q.img_scale = scale;
+    const size_t plane = (size_t)grid->ny * grid->nx;
+    hipLaunchKernelGGL(image_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0,
+                       stream, q);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+int check_grid(const swfd_grid *g)
+{
+    if (g == nullptr) return fail(SWFD_EINVAL, "grid is NULL");
+    if (g->ny < 3 || g->nx < 3)
+        return fail(SWFD_EINVAL, "grid must be at least 3 x 3 (got " + std::to_string(g->ny) +
+                                     " x " + std::to_string(g->nx) + ")");
+    return SWFD_OK;
+}
+
+int check_medium(const swfd_medium *m)
+{
+    if (m == nullptr || m->c2 == nullptr || m->c2dt2 == nullptr || m->mur == nullptr)
+        return fail(SWFD_EINVAL, "medium coefficients are missing");
+    return SWFD_OK;
+}
+
+int check_receivers(const swfd_receivers *r, bool need_table)
+{
+    if (r == nullptr || r->n < 1 || r->row0 == nullptr || r->col0 == nullptr ||
+        r->weights == nullptr)
+        return fail(SWFD_EINVAL, "receivers are missing");
+    if (need_table && (r->cell_map == nullptr || r->entry_start == nullptr ||
+                       r->pair_rec == nullptr || r->pair_w == nullptr ||
+                       r->entry_cell == nullptr))
+        return fail(SWFD_EINVAL, "receiver scatter table is missing");
+    return SWFD_OK;
+}
+
+hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int copy_field(float *dst, const float *src, size_t n, hipStream_t stream)
+{
+    if (dst == src) return SWFD_OK;
+    SWFD_HIP_CHECK(
+        hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    return SWFD_OK;
+}
+
+// Move the state (p_old, p_cur) into (dst_old, dst_cur) using `spare` as a
+// temporary when the two destinations hold each other's data.
+int settle_state(const float *p_old, const float *p_cur, float *dst_old, float *dst_cur,
+                 float *const work[3], size_t n, hipStream_t stream)
+{
+    int rc;
+    if (dst_old == nullptr && dst_cur == nullptr) return SWFD_OK;
+    if (dst_old == p_old && dst_cur == p_cur) return SWFD_OK;
+    if (dst_old != nullptr && dst_cur != nullptr && dst_old == p_cur && dst_cur == p_old) {
+        float *spare = nullptr;
+        for (int i = 0; i < 3; ++i)
+            if (work[i] != dst_old && work[i] != dst_cur) spare = work[i];
+        if ((rc = copy_field(spare, p_cur, n, stream))) return rc;
+        if ((rc = copy_field(dst_old, p_old, n, stream))) return rc;
+        return copy_field(dst_cur, spare, n, stream);
+    }
+    if (dst_old != nullptr && dst_old == p_cur) {
+        // dst_old currently holds the newest state: move it first
+        if (dst_cur != nullptr && (rc = copy_field(dst_cur, p_cur, n, stream))) return rc;
+        return copy_field(dst_old, p_old, n, stream);
+    }
+    if (dst_old != nullptr && (rc = copy_field(dst_old, p_old, n, stream))) return rc;
+    if (dst_cur != nullptr && (rc = copy_field(dst_cur, p_cur, n, stream))) return rc;
+    return SWFD_OK;
+}
+
+float *free_work(float *const work[3], const float *a, const float *b)
+{
+    for (int i = 0; i < 3; ++i)
+        if (work[i] != a && work[i] != b) return work[i];
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *swfd_last_error(void) { return g_last_error.c_str(); }
+
+const char *swfd_version(void) { return "swfd 0.1 gfx950"; }
+
+size_t swfd_medium_floats(int32_t ny, int32_t nx)
+{
+    return 2 * (size_t)ny * nx + 2 * (size_t)ny + 2 * (size_t)nx + 8;
+}
+
+void swfd_medium_view(const swfd_grid *grid, const float *coef, swfd_medium *out)
+{
+    const size_t n = (size_t)grid->ny * grid->nx;
+    out->c2 = coef;
+    out->c2dt2 = coef + n;
+    out->mur = coef + 2 * n;
+}
+
+int swfd_prepare_medium(const swfd_grid *grid, const float *c, double dt, double dx,
+                        double dy, float *coef, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid))) return rc;
+    if (c == nullptr || coef == nullptr) return fail(SWFD_EINVAL, "c / coef is NULL");
+    const size_t n = (size_t)grid->ny * grid->nx;
+    const size_t work = std::max<size_t>(n, (size_t)std::max(grid->ny, grid->nx));
+    const unsigned blocks = (unsigned)((work + 255) / 256);
+    hipLaunchKernelGGL(prepare_medium_kernel, dim3(blocks), dim3(256), 0, as_stream(stream),
+                       grid->ny, grid->nx, c, (float)dt, (float)dx, (float)dy, grid->dt2,
+                       coef, coef + n, coef + 2 * n);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+int swfd_step(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+              const swfd_step_args *a, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
+    if (a == nullptr) return fail(SWFD_EINVAL, "step args are NULL");
+    if (a->n_shots < 1) return fail(SWFD_EINVAL, "n_shots must be positive");
+    if (a->u_old == nullptr || a->u == nullptr || a->u_next == nullptr)
+        return fail(SWFD_EINVAL, "field pointers are NULL");
+    if (a->u_next == a->u_old || a->u_next == a->u)
+        return fail(SWFD_EINVAL, "u_next must not alias u_old or u");
+    if (a->src_rows != nullptr && (a->src_cols == nullptr || a->src_values == nullptr))
+        return fail(SWFD_EINVAL, "point source needs rows, cols and values");
+    if (a->residual != nullptr && (rc = check_receivers(rec, true))) return rc;
+    if (a->samples != nullptr && (rc = check_receivers(rec, false))) return rc;
+    if (a->image_field != nullptr && (a->grad_in == nullptr || a->grad_out == nullptr))
+        return fail(SWFD_EINVAL, "imaging needs grad_in and grad_out");
+    if (a->image_field != nullptr && !a->adjoint)
+        return fail(SWFD_EINVAL, "imaging is an adjoint-step option");
+    StepParams p;
+    base_params(p, grid, medium, a->n_shots);
+    attach_receivers(p, a->residual != nullptr ? rec : nullptr);
+    p.f_old = a->u_old;
+    p.f_cur = a->u;
+    p.f_out = a->u_next;
+    p.f_out2 = a->u_next_copy;
+    p.src_rows = a->src_rows;
+    p.src_cols = a->src_cols;
+    p.src_values = a->src_values;
+    p.src_index = a->src_index;
+    p.dense = a->dense;
+    p.resid = a->residual;
+    if ((rc = launch_step(p, a->adjoint != 0, as_stream(stream), 0))) return rc;
+    if (a->image_field != nullptr &&
+        (rc = launch_image(grid, a->n_shots, a->u_next, a->image_field, a->grad_in,
+                           a->grad_out, a->image_scale, as_stream(stream))))
+        return rc;
+    if (a->samples != nullptr)
+        return swfd_gather(grid, rec, a->n_shots, a->u_next, a->samples, stream);
+    return SWFD_OK;
+}
+
+int swfd_gather(const swfd_grid *grid, const swfd_receivers *rec, int32_t n_shots,
+                const float *fields, float *samples, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_receivers(rec, false))) return rc;
+    if (fields == nullptr || samples == nullptr || n_shots < 1)
+        return fail(SWFD_EINVAL, "gather: bad fields / samples / n_shots");
+    StepParams p;
+    base_params(p, grid, nullptr, n_shots);
+    attach_receivers(p, rec);
+    p.gat_field = fields;
+    p.gat_out = samples;
+    const int pairs = n_shots * rec->n;
+    hipLaunchKernelGGL(gather_kernel, dim3((pairs + 255) / 256), dim3(256), 0,
+                       as_stream(stream), p);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+int swfd_scatter(const swfd_grid *grid, const swfd_receivers *rec, int32_t n_shots,
+                 const float *residual, float *fields, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_receivers(rec, true))) return rc;
+    if (residual == nullptr || fields == nullptr || n_shots < 1)
+        return fail(SWFD_EINVAL, "scatter: bad residual / fields / n_shots");
+    const size_t plane = (size_t)grid->ny * grid->nx;
+    const size_t total = plane * n_shots;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), fields, total, 0.0f);
+    SWFD_HIP_CHECK(hipGetLastError());
+    const int items = n_shots * rec->n_entries;
+    if (items > 0) {
+        hipLaunchKernelGGL(scatter_kernel, dim3((items + 255) / 256), dim3(256), 0,
+                           as_stream(stream), n_shots, plane, rec->n_entries, rec->n,
+                           residual, rec->entry_cell, rec->entry_start, rec->pair_rec,
+                           rec->pair_w, fields);
+        SWFD_HIP_CHECK(hipGetLastError());
+    }
+    return SWFD_OK;
+}
+
+int swfd_laplacian(const swfd_grid *grid, int32_t n_fields, const float *fields, float *out,
+                   void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid))) return rc;
+    if (fields == nullptr || out == nullptr || n_fields < 1 || fields == out)
+        return fail(SWFD_EINVAL, "laplacian: bad fields / out / count");
+    const size_t total = (size_t)grid->ny * grid->nx * n_fields;
+    hipLaunchKernelGGL(laplacian_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), grid->ny, grid->nx, total, grid->inv_dx2,
+                       grid->inv_dy2, grid->center, fields, out);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+                 const swfd_forward_args *a, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
+    if (a == nullptr) return fail(SWFD_EINVAL, "forward args are NULL");
+    if (a->n_shots < 1 || a->n_steps < 0 || a->step_begin < 0)
+        return fail(SWFD_EINVAL, "forward: bad n_shots / n_steps / step_begin");
+    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
+        a->work1 == nullptr || a->work2 == nullptr)
+        return fail(SWFD_EINVAL, "forward: state or work buffers are NULL");
+    if (a->src_rows == nullptr || a->src_cols == nullptr || a->src_values == nullptr)
+        return fail(SWFD_EINVAL, "forward: point sources are required");
+    if (a->samples != nullptr && (rc = check_receivers(rec, false))) return rc;
+    if (a->hist != nullptr && (a->hist_every < 1 || a->hist_slots < 1))
+        return fail(SWFD_EINVAL, "forward: hist_every and hist_slots must be positive");
+    if (a->ckpt != nullptr && (a->ckpt_every < 2 || a->ckpt_slots < 2))
+        return fail(SWFD_EINVAL, "forward: ckpt_every must be >= 2 and ckpt_slots >= 2");
+    hipStream_t st = as_stream(stream);
+    const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
+    const size_t sr = (size_t)a->n_shots * (a->samples != nullptr ? rec->n : 0);
+    float *const work[3] = {a->work0, a->work1, a->work2};
+    const float *p_old = a->init_old;
+    const float *p_cur = a->init_u;
+    for (int k = 0; k < a->n_steps; ++k) {
+        const int n = a->step_begin + k;
+        float *out = nullptr;
+        if (a->hist != nullptr && n >= a->hist_first && (n - a->hist_first) % a->hist_every == 0) {
+            const int slot = (n - a->hist_first) / a->hist_every;
+            if (slot < a->hist_slots) out = a->hist + (size_t)slot * field;
+        }
+        if (out == nullptr) out = free_work(work, p_old, p_cur);
+        float *out2 = nullptr;
+        if (a->ckpt != nullptr) {
+            const int K = a->ckpt_every;
+            const int j2 = n + 2, j1 = n + 1;
+            int slot = -1;
+            if (j2 % K == 0 && j2 / K >= 1) slot = 2 * (j2 / K - 1);
+            else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
+            if (slot >= 0 && slot < a->ckpt_slots) out2 = a->ckpt + (size_t)slot * field;
+        }
+        StepParams p;
+        base_params(p, grid, medium, a->n_shots);
+        attach_receivers(p, a->samples != nullptr ? rec : nullptr);
+        p.f_old = p_old;
+        p.f_cur = p_cur;
+        p.f_out = out;
+        p.f_out2 = out2;
+        p.src_rows = a->src_rows;
+        p.src_cols = a->src_cols;
+        p.src_values = a->src_values;
+        p.src_index = n;
+        int gather_count = 0;
+        if (a->samples != nullptr && k > 0) {
+            p.gat_field = p_cur;
+            p.gat_out = a->samples + (size_t)(n - 1) * sr;
+            gather_count = a->n_shots * rec->n;
+        }
+        if ((rc = launch_step(p, false, st, gather_count))) return rc;
+        p_old = p_cur;
+        p_cur = out;
+    }
+    if (a->samples != nullptr && a->n_steps > 0) {
+        const int last = a->step_begin + a->n_steps - 1;
+        if ((rc = swfd_gather(grid, rec, a->n_shots, p_cur, a->samples + (size_t)last * sr,
+                              stream)))
+            return rc;
+    }
+    return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+}
+
+int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
+                 const swfd_adjoint_args *a, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
+    if (a == nullptr) return fail(SWFD_EINVAL, "adjoint args are NULL");
+    if (a->n_shots < 1 || a->step_lo < 0 || a->step_hi < a->step_lo)
+        return fail(SWFD_EINVAL, "adjoint: bad n_shots / step range");
+    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
+        a->work1 == nullptr || a->work2 == nullptr)
+        return fail(SWFD_EINVAL, "adjoint: state or work buffers are NULL");
+    if (a->residual == nullptr) return fail(SWFD_EINVAL, "adjoint: residual is NULL");
+    if ((rc = check_receivers(rec, true))) return rc;
+    if (a->hist != nullptr && (a->grad == nullptr || a->hist_every < 1 || a->hist_slots < 1))
+        return fail(SWFD_EINVAL, "adjoint: imaging needs grad, hist_every and hist_slots");
+    hipStream_t st = as_stream(stream);
+    const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
+    const size_t sr = (size_t)a->n_shots * rec->n;
+    float *const work[3] = {a->work0, a->work1, a->work2};
+    const float *p_old = a->init_old;
+    const float *p_cur = a->init_u;
+    // Step m images lam' (after step m) against the forward field after step
+    // m-1. The kernel of step n performs the imaging of step n+1, whose lam' is
+    // its input u; the imaging of step_lo follows the loop.
+    auto image_slot = [&](int m) -> long {
+        if (a->hist == nullptr || m < 1) return -1;
+        const int j = m - 1;
+        if (j < a->hist_first || (j - a->hist_first) % a->hist_every != 0) return -1;
+        return (j - a->hist_first) / a->hist_every;
+    };
+    for (int n = a->step_hi - 1; n >= a->step_lo; --n) {
+        float *out = free_work(work, p_old, p_cur);
+        if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+        StepParams p;
+        base_params(p, grid, medium, a->n_shots);
+        attach_receivers(p, rec);
+        p.f_old = p_old;
+        p.f_cur = p_cur;
+        p.f_out = out;
+        p.resid = a->residual + (size_t)n * sr;
+        if (n < a->step_hi - 1) {
+            const long slot = image_slot(n + 1);
+            if (slot >= a->hist_slots)
+                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+            if (slot >= 0) {
+                p.img = a->hist + (size_t)slot * field;
+                p.g_in = a->grad;
+                p.g_out = a->grad;
+                p.img_scale = a->image_scale;
+            }
+        }
+        if ((rc = launch_step(p, true, st, 0))) return rc;
+        p_old = p_cur;
+        p_cur = out;
+    }
+    if (a->step_hi > a->step_lo) {
+        const long slot = image_slot(a->step_lo);
+        if (slot >= a->hist_slots) return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+        if (slot >= 0 &&
+            (rc = launch_image(grid, a->n_shots, p_cur, a->hist + (size_t)slot * field, a->grad,
+                               a->grad, a->image_scale, st)))
+            return rc;
+    }
+    return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+}
+
+size_t swfd_residual_workspace_floats(int32_t n_steps, int32_t n_shots, int32_t n_rec)
+{
+    const size_t tiles_r = (n_rec + kResTile - 1) / kResTile;
+    const size_t tiles_n = (n_steps + kResTile - 1) / kResTile;
+    return tiles_r * tiles_n * (size_t)n_shots;
+}
+
+int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                  const float *observed, int64_t obs_stride_e, int64_t obs_stride_r,
+                  const int32_t *obs_index, const int32_t *mute, float *residual,
+                  float *workspace, float *misfit, void *stream)
+{
+    if (n_steps < 1 || n_shots < 1 || n_rec < 1)
+        return fail(SWFD_EINVAL, "residual: sizes must be positive");
+    if (samples == nullptr || observed == nullptr || obs_index == nullptr ||
+        mute == nullptr || residual == nullptr || workspace == nullptr || misfit == nullptr)
+        return fail(SWFD_EINVAL, "residual: NULL pointer");
+    const size_t blocks = swfd_residual_workspace_floats(n_steps, n_shots, n_rec);
+    if (blocks > 0x7fffffff) return fail(SWFD_EINVAL, "residual: too many tiles");
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n_steps,
+                       n_shots, n_rec, samples, observed, (long long)obs_stride_e,
+                       (long long)obs_stride_r, obs_index, mute, residual, workspace);
+    SWFD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(misfit_kernel, dim3(1), dim3(256), 0, st, (int)blocks, workspace,
+                       misfit);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,429 @@
+"""Least-squares FWI with the adjoint-state gradient on the device.
+
+Drop-in counterpart of the reference ``invert_ring.py:395-701``
+(``forward_store``, ``forward_store_batch``, ``TorchLeastSquaresFWI``) on the
+native kernels of :mod:`soundwavesimulation_amd.torch_backend`.
+
+The exact gradient (``store_every=1``, full-precision history, the
+reference's default) is computed by :class:`CheckpointedGradient`: the
+forward run keeps only the state pair before every ``checkpoint_every``-th
+step; the reverse pass re-runs each segment from its checkpoint into a
+segment history and runs the adjoint over it. Recomputation reproduces the
+forward fields bit for bit, so the result equals the store-everything
+gradient while holding ``2 N/K + K + 1`` fields instead of ``N``.
+The reference's approximate options (``store_every > 1``, reduced-precision
+``wave_dtype``) keep its stored-history algorithm.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .. import _native
+from ..torch_backend import (TorchFDTD2DBatch, _source_values_tensor, adjoint_run,
+                             auto_batch_size, forward_run, simulate_shot)
+from .medium import c_from_m, m_from_c
+from .misfit import tikhonov, total_variation
+
+_F32 = torch.float32
+
+
+def stored_steps(n_steps, store_every):
+    """Number of wavefield snapshots kept when storing every k-th step."""
+    return -(-int(n_steps) // int(store_every))
+
+
+def segment_plan(n_steps, checkpoint_every):
+    """``(n_segments, n_checkpoint_slots, n_history_slots)`` of the checkpointed gradient."""
+    K = int(checkpoint_every)
+    n_seg = -(-int(n_steps) // K)
+    return n_seg, 2 * (n_seg - 1), K + 1
+
+
+def default_checkpoint_every(n_steps):
+    """Segment length minimising the resident fields ``2 N/K + K``: about sqrt(2N)."""
+    return max(2, int(round(math.sqrt(2.0 * max(int(n_steps), 1)))))
+
+
+def gradient_bytes_per_shot(n_steps, grid_shape, n_receivers, store_every=1, wave_dtype=None,
+                            dtype=_F32, checkpoint_every=None):
+    """Device memory one gradient shot needs.
+
+    ``store_every == 1`` with full-precision history: checkpointed plan,
+    ``(6 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
+    Otherwise the reference's stored-history estimate (invert_ring.py:463-473).
+    """
+    cells = grid_shape[0] * grid_shape[1]
+    if int(store_every) == 1 and wave_dtype in (None, _F32):
+        K = checkpoint_every or default_checkpoint_every(n_steps)
+        _, n_ckpt, n_hist = segment_plan(n_steps, K)
+        return (6 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
+    wave_bytes = torch.empty((), dtype=wave_dtype or _F32).element_size()
+    return (stored_steps(n_steps, store_every) * cells * wave_bytes + 12 * cells * 4
+            + 4 * n_steps * n_receivers * 4)
+
+
+def observed_bytes_per_shot(n_steps, grid_shape, n_receivers, dtype=_F32):
+    cells = grid_shape[0] * grid_shape[1]
+    return 12 * cells * 4 + 3 * n_steps * n_receivers * 4
+
+
+def _i32(values, device):
+    return torch.as_tensor(np.asarray(values, dtype=np.int32)).to(device)
+
+
+def residual_and_misfit(samples, observed, obs_index, mute):
+    """``residual (N, S, R)`` and device scalar ``0.5 * sum(residual^2)``.
+
+    ``samples`` (N, S, R); ``observed`` (N_el, R, N) (any strides with the
+    time axis contiguous); ``obs_index`` / ``mute`` int32 (S,) device tensors.
+    """
+    N, S, R = samples.shape
+    if observed.stride(-1) != 1:
+        observed = observed.contiguous()
+    if observed.shape[1] != R or observed.shape[2] != N:
+        raise ValueError("observed must have shape (elements, receivers, steps).")
+    dev = samples.device
+    residual = torch.empty_like(samples)
+    work = torch.empty(int(_native.lib().swfd_residual_workspace_floats(N, S, R)) + 1,
+                       device=dev, dtype=_F32)
+    misfit = torch.empty(1, device=dev, dtype=_F32)
+    _native.check(_native.lib().swfd_residual(
+        N, S, R, _native.ptr(samples), _native.ptr(observed), observed.stride(0),
+        observed.stride(1), _native.ptr(obs_index), _native.ptr(mute), _native.ptr(residual),
+        _native.ptr(work), _native.ptr(misfit), _native.stream_of(dev)))
+    return residual, misfit
+
+
+class CheckpointedGradient:
+    """Exact forward + adjoint gradient with checkpointing and recomputation.
+
+    Holds the device buffers for one batch size. ``forward`` runs all steps,
+    records samples and keeps checkpoints; ``backward`` takes the residual
+    (N, S, R) and accumulates ``dt^2 sum_n lam^n * Lap(u^(n-1))`` into a
+    gradient with respect to c^2.
+    """
+
+    def __init__(self, solver, sampler, n_shots, n_steps, checkpoint_every=None):
+        self.solver = solver
+        self.sampler = sampler
+        self.n_shots = int(n_shots)
+        self.n_steps = int(n_steps)
+        self.K = int(checkpoint_every or default_checkpoint_every(n_steps))
+        if self.K < 2:
+            raise ValueError("checkpoint_every must be at least 2.")
+        self.n_seg, n_ckpt, n_hist = segment_plan(self.n_steps, self.K)
+        dev = solver.model.device
+        shape = (self.n_shots, *solver.model.shape)
+        self.work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
+        self.lam = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
+        self.ckpt = (torch.empty((max(n_ckpt, 2), *shape), device=dev, dtype=_F32)
+                     if n_ckpt > 0 else None)
+        self.hist = torch.empty((n_hist, *shape), device=dev, dtype=_F32)
+        self.samples = torch.empty((self.n_steps, self.n_shots, sampler.n_elements),
+                                   device=dev, dtype=_F32)
+        self.rows = None
+        self.cols = None
+        self.values = None
+
+    def forward(self, rows_i32, cols_i32, values):
+        """All forward steps; samples land in ``self.samples`` (N, S, R)."""
+        self.rows, self.cols, self.values = rows_i32, cols_i32, values
+        forward_run(self.solver, self.n_shots, rows_i32, cols_i32, values, self.n_steps,
+                    samples=self.samples, sampler=self.sampler, work=self.work,
+                    ckpt=self.ckpt, ckpt_every=self.K if self.ckpt is not None else 0,
+                    final=False)
+        return self.samples
+
+    def backward(self, residual_nsr, grad, image_scale):
+        """Accumulate the imaging condition of every step into ``grad`` (ny, nx)."""
+        K, N = self.K, self.n_steps
+        for t in self.lam[:2]:
+            t.zero_()
+        state = (self.lam[0], self.lam[1])
+        for j in range(self.n_seg - 1, -1, -1):
+            n0 = j * K
+            n1 = min(n0 + K, N)
+            if j == 0:
+                self.hist[0].zero_()
+                self.hist[1].zero_()
+            else:
+                self.hist[0].copy_(self.ckpt[2 * (j - 1)])
+                self.hist[1].copy_(self.ckpt[2 * (j - 1) + 1])
+            n_re = n1 - 1 - n0
+            if n_re > 0:
+                forward_run(self.solver, self.n_shots, self.rows, self.cols, self.values, n_re,
+                            init=(self.hist[0], self.hist[1]), hist=self.hist,
+                            hist_first=n0 - 2, hist_every=1, step_begin=n0, work=self.work,
+                            final=False)
+            adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
+                        state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
+                        hist_every=1, image_scale=image_scale, grad=grad)
+        return grad
+
+
+def forward_store(solver, sampler, array, pulse, tx, n_steps):
+    """Single shot with the full wavefield history (invert_ring.py:395-419)."""
+    n_steps = int(n_steps)
+    row, col = array.inject_rows_cols(tx)
+    solver.reset()
+    solver.set_source(row, col)
+    dev = solver.model.device
+    values = _source_values_tensor(pulse, n_steps, dev)
+    rc = _i32([[row], [col]], dev)
+    wave = torch.empty((n_steps, 1, *solver.model.shape), device=dev, dtype=_F32)
+    samples = torch.empty((n_steps, 1, sampler.n_elements), device=dev, dtype=_F32)
+    u_old, u = forward_run(solver, 1, rc[0], rc[1], values, n_steps, samples=samples,
+                           sampler=sampler, hist=wave, hist_first=0, hist_every=1)
+    solver.u_old, solver.u = u_old[0].clone(), u[0].clone()
+    return samples[:, 0, :].transpose(0, 1).contiguous(), wave[:, 0]
+
+
+def forward_store_batch(solver, sampler, array, source_values, sources, n_steps, batched,
+                        store_every=1, wave_dtype=None):
+    """Batched forward keeping every ``store_every``-th field (invert_ring.py:422-455).
+
+    Returns traces (S, R, N) and the stored wavefields (stored, S, ny, nx).
+    Reduced-precision ``wave_dtype`` storage is reproduced by rounding the
+    float32 history through that dtype.
+    """
+    n_steps = int(n_steps)
+    rows_cols = [array.inject_rows_cols(int(tx)) for tx in sources]
+    rows = np.asarray([r for r, _ in rows_cols], dtype=np.int64)
+    cols = np.asarray([c for _, c in rows_cols], dtype=np.int64)
+    batched.set_sources(rows, cols)
+    dev = solver.model.device
+    values = _source_values_tensor(None, n_steps, dev, source_values)
+    S = len(sources)
+    samples = torch.empty((n_steps, S, sampler.n_elements), device=dev, dtype=_F32)
+    wave = torch.empty((stored_steps(n_steps, store_every), S, *solver.model.shape),
+                       device=dev, dtype=_F32)
+    u_old, u = forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, n_steps,
+                           samples=samples, sampler=sampler, hist=wave, hist_first=0,
+                           hist_every=int(store_every))
+    batched.u_old, batched.u = u_old, u
+    if wave_dtype not in (None, _F32):
+        wave = wave.to(wave_dtype).to(_F32)
+    return samples.permute(1, 2, 0).contiguous(), wave
+
+
+class TorchLeastSquaresFWI:
+    """Least-squares FWI on the device (invert_ring.py:483-701), same constructor."""
+
+    def __init__(self, solver, sampler, array, pulse, n_steps, n_shots, observed, mask,
+                 alpha=0.0, tv=0.0, tv_eps=1.0e-9, c_min=1400.0, c_max=2000.0,
+                 c_background=1500.0, shot_seed=0, parallel_shots=16, batched_solvers=None,
+                 store_every=1, wave_dtype=None, checkpoint_every=None):
+        self.solver = solver
+        self.model = solver.model
+        self.sampler = sampler
+        self.array = array
+        dev = self.model.device
+        self.pulse = torch.as_tensor(pulse).to(device=dev, dtype=_F32)
+        self.n_steps = int(n_steps)
+        self.source_values = _source_values_tensor(self.pulse, self.n_steps, dev)
+        self.n_shots = int(n_shots)
+        if self.n_shots < 1 or self.n_shots > array.n_elements:
+            raise ValueError("n_shots must be between 1 and the number of elements.")
+        if not isinstance(observed, torch.Tensor):
+            observed = torch.stack([torch.as_tensor(o) for o in observed])
+        if observed.shape[0] != array.n_elements:
+            raise ValueError("Observed data must contain one gather per transmitter.")
+        self.observed = observed.to(device=dev, dtype=_F32).contiguous()
+        self.store_every = int(store_every)
+        if self.store_every < 1:
+            raise ValueError("store_every must be positive.")
+        self.wave_dtype = _F32 if wave_dtype is None else wave_dtype
+        self.checkpoint_every = (int(checkpoint_every) if checkpoint_every
+                                 else default_checkpoint_every(self.n_steps))
+        if parallel_shots == "auto":
+            parallel_shots = auto_batch_size(
+                dev, self.n_shots,
+                gradient_bytes_per_shot(self.n_steps, self.model.shape, sampler.n_elements,
+                                        self.store_every, self.wave_dtype, _F32,
+                                        self.checkpoint_every))
+        self.parallel_shots = int(parallel_shots)
+        if self.parallel_shots < 1:
+            raise ValueError("parallel_shots must be positive.")
+        self.parallel_shots = min(self.parallel_shots, self.n_shots)
+        self._batched_solvers = {} if batched_solvers is None else batched_solvers
+        self._engines = {}
+        self._shot_rng = np.random.default_rng(shot_seed)
+        self.active_sources = None
+        self.source_history = []
+        self.mask = np.asarray(mask, dtype=bool)
+        self.alpha = float(alpha)
+        self.tv = float(tv)
+        self.tv_eps = float(tv_eps)
+        self.c_min = float(c_min)
+        self.c_max = float(c_max)
+        self.c_background = float(c_background)
+
+    # -- medium ----------------------------------------------------------------
+    def project(self, m):
+        m = np.asarray(m, dtype=float)
+        projected = np.clip(m, 1.0 / self.c_max**2, 1.0 / self.c_min**2).copy()
+        projected[~self.mask] = m_from_c(self.c_background)
+        return projected
+
+    def apply_medium(self, m):
+        m = self.project(m)
+        self.solver.set_c(c_from_m(m))
+        return m
+
+    # -- shots -----------------------------------------------------------------
+    @staticmethod
+    def _residual(predicted, observed, tx):
+        residual = (predicted - observed).clone()
+        residual[tx].zero_()
+        return residual
+
+    def _sample_sources(self):
+        if self.n_shots == self.array.n_elements:
+            selected = np.arange(self.array.n_elements, dtype=int)
+        else:
+            selected = np.sort(self._shot_rng.choice(self.array.n_elements, size=self.n_shots,
+                                                     replace=False))
+        self.active_sources = selected
+        self.source_history.append(selected.copy())
+        return selected
+
+    def _source_batches(self, sources):
+        for start in range(0, len(sources), self.parallel_shots):
+            yield sources[start:start + self.parallel_shots]
+
+    def _batched_solver(self, batch_size):
+        batched = self._batched_solvers.get(batch_size)
+        if batched is None:
+            batched = TorchFDTD2DBatch(self.solver, batch_size)
+            self._batched_solvers[batch_size] = batched
+        return batched
+
+    def _engine(self, batch_size):
+        engine = self._engines.get(batch_size)
+        if engine is None:
+            engine = CheckpointedGradient(self.solver, self.sampler, batch_size, self.n_steps,
+                                          self.checkpoint_every)
+            self._engines[batch_size] = engine
+        return engine
+
+    def _mute_indices(self, shot_batch):
+        """Receiver channel muted per shot: the transmitter (ring geometry)."""
+        return shot_batch
+
+    @staticmethod
+    def _residual_batch(predicted, observed, sources):
+        residual = predicted - observed
+        batches = torch.arange(len(sources), device=predicted.device)
+        transmitters = torch.as_tensor(np.asarray(sources), device=predicted.device)
+        residual[batches, transmitters] = 0.0
+        return residual
+
+    def _rows_cols(self, shot_batch):
+        rc = [self.array.inject_rows_cols(int(tx)) for tx in shot_batch]
+        dev = self.model.device
+        return (_i32([r for r, _ in rc], dev), _i32([c for _, c in rc], dev))
+
+    def _exact(self):
+        return self.store_every == 1 and self.wave_dtype == _F32
+
+    # -- objective ---------------------------------------------------------------
+    def misfit(self, m):
+        m = self.apply_medium(m)
+        sources = self.active_sources
+        if sources is None:
+            sources = self._sample_sources()
+        dev = self.model.device
+        total = torch.zeros(1, device=dev, dtype=_F32)
+        for shot_batch in self._source_batches(sources):
+            S = len(shot_batch)
+            rows, cols = self._rows_cols(shot_batch)
+            samples = torch.empty((self.n_steps, S, self.sampler.n_elements), device=dev,
+                                  dtype=_F32)
+            forward_run(self.solver, S, rows, cols, self.source_values, self.n_steps,
+                        samples=samples, sampler=self.sampler, final=False)
+            _, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
+                                          _i32(self._mute_indices(shot_batch), dev))
+            total = total + part
+        regularization, _ = self.regularization(m)
+        return float(total.item()) + regularization
+
+    def misfit_and_grad(self, m):
+        m = self.apply_medium(m)
+        sources = self._sample_sources()
+        dev = self.model.device
+        total = torch.zeros(1, device=dev, dtype=_F32)
+        g_c2 = torch.zeros(self.model.shape, device=dev, dtype=_F32)
+        for shot_batch in self._source_batches(sources):
+            if self._exact():
+                part = self._batch_gradient_checkpointed(shot_batch, g_c2)
+            else:
+                part = self._batch_gradient_stored(shot_batch, g_c2)
+            total = total + part
+        value = float(total.item())
+        regularization, g_regularization = self.regularization(m)
+        gradient = -g_c2.cpu().numpy().astype(float) / (m * m)
+        gradient += g_regularization
+        gradient[~self.mask] = 0.0
+        return value + regularization, gradient
+
+    def _batch_gradient_checkpointed(self, shot_batch, g_c2):
+        dev = self.model.device
+        engine = self._engine(len(shot_batch))
+        rows, cols = self._rows_cols(shot_batch)
+        samples = engine.forward(rows, cols, self.source_values)
+        residual, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
+                                             _i32(self._mute_indices(shot_batch), dev))
+        engine.backward(residual, g_c2, self.solver.dt2)
+        return part
+
+    def _batch_gradient_stored(self, shot_batch, g_c2):
+        dev = self.model.device
+        batched = self._batched_solver(len(shot_batch))
+        predicted, wave = forward_store_batch(self.solver, self.sampler, self.array,
+                                              self.source_values, shot_batch, self.n_steps,
+                                              batched, self.store_every, self.wave_dtype)
+        samples = predicted.permute(2, 0, 1).contiguous()
+        residual, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
+                                             _i32(self._mute_indices(shot_batch), dev))
+        self._adjoint_accumulate_batch(batched, wave, residual, g_c2, residual_layout="nsr")
+        return part
+
+    def regularization(self, m):
+        value, gradient = tikhonov(m, self.mask, self.alpha)
+        tv_value, tv_gradient = total_variation(m, self.mask, self.tv, self.tv_eps)
+        return value + tv_value, gradient + tv_gradient
+
+    def _adjoint_accumulate(self, wave, residual, g_c2):
+        """Unbatched adjoint of one shot through the single-step API (invert_ring.py:668-681)."""
+        self.solver.reset()
+        dt2 = self.solver.dt2
+        for n in range(self.n_steps - 1, -1, -1):
+            source = self.sampler.record_adjoint(residual[:, n])
+            adjoint_field = self.solver.inject_field_and_step(source, adjoint=True)
+            if n == 0:
+                continue
+            laplacian = self.model.laplacian(wave[n - 1])
+            g_c2.add_(adjoint_field * laplacian * dt2)
+
+    def _adjoint_accumulate_batch(self, batched, wave, residual, g_c2, residual_layout="srn"):
+        """Adjoint over a stored history (invert_ring.py:683-701); adds into ``g_c2``.
+
+        ``residual`` is (S, R, N) as the reference passes it, or (N, S, R) with
+        ``residual_layout="nsr"``.
+        """
+        S = batched.n_shots
+        residual_nsr = (residual.permute(2, 0, 1).contiguous() if residual_layout == "srn"
+                        else residual.contiguous())
+        dev = self.model.device
+        shape = (S, *self.model.shape)
+        lam = [torch.zeros(shape, device=dev, dtype=_F32) for _ in range(3)]
+        adjoint_run(self.solver, self.sampler, S, residual_nsr, self.n_steps, 0,
+                    state=(lam[0], lam[1]), work=lam, hist=wave.contiguous(), hist_first=0,
+                    hist_every=self.store_every, image_scale=self.store_every * batched.dt2,
+                    grad=g_c2)
+        batched.u_old, batched.u = lam[0], lam[1]
+        return g_c2
