@@ -127,7 +127,9 @@ typedef struct swfd_forward_args {
  *   hist: forward field after step j lives in slot (j - hist_first) /
  *   hist_every; step n images with the field after step n-1 when n >= 1,
  *   n-1 >= hist_first and (n-1-hist_first) % hist_every == 0, adding
- *   image_scale * sum_s lam'[s] * Lap(field) into grad (in place). */
+ *   image_scale * sum_s lam'[s] * Lap(field) into grad (in place). Shot groups
+ *   accumulate their partial images in `workspace` across the run, summed into
+ *   grad in a fixed order at its end, so the result is deterministic. */
 typedef struct swfd_adjoint_args {
     int32_t n_shots;
     int32_t step_hi, step_lo;
@@ -139,6 +141,7 @@ typedef struct swfd_adjoint_args {
     int32_t hist_first, hist_every, hist_slots;
     float image_scale;
     float *grad;
+    float *workspace; /* swfd_adjoint_workspace_floats() floats, needed with hist */
 } swfd_adjoint_args;
 
 /* Thread-local message of the last failure. */
@@ -188,6 +191,9 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
 /* Multi-step adjoint run; see swfd_adjoint_args. */
 int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
                  const swfd_adjoint_args *args, void *stream);
+
+/* Workspace floats needed by swfd_adjoint with imaging. */
+size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots);
 
 /* Workspace floats needed by swfd_residual. */
 size_t swfd_residual_workspace_floats(int32_t n_steps, int32_t n_shots, int32_t n_rec);

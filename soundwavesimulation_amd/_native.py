@@ -61,14 +61,16 @@ class AdjointArgs(ctypes.Structure):
                 ("init_old", _fp), ("init_u", _fp), ("work0", _fp), ("work1", _fp),
                 ("work2", _fp), ("final_old", _fp), ("final_u", _fp), ("residual", _fp),
                 ("hist", _fp), ("hist_first", _c_int), ("hist_every", _c_int),
-                ("hist_slots", _c_int), ("image_scale", _c_float), ("grad", _fp)]
+                ("hist_slots", _c_int), ("image_scale", _c_float), ("grad", _fp),
+                ("workspace", _fp)]
 
 
 #: every symbol include/swfd.h declares
 EXPORTS = (
     "swfd_last_error", "swfd_version", "swfd_medium_floats", "swfd_prepare_medium",
     "swfd_medium_view", "swfd_step", "swfd_gather", "swfd_scatter", "swfd_laplacian",
-    "swfd_forward", "swfd_adjoint", "swfd_residual_workspace_floats", "swfd_residual",
+    "swfd_forward", "swfd_adjoint", "swfd_adjoint_workspace_floats",
+    "swfd_residual_workspace_floats", "swfd_residual",
 )
 
 _lib = None
@@ -101,6 +103,8 @@ def load_library(path=LIBRARY):
     lib.swfd_laplacian.argtypes = [P(Grid), _c_int, vp, vp, vp]
     lib.swfd_forward.argtypes = [P(Grid), P(Medium), P(Receivers), P(ForwardArgs), vp]
     lib.swfd_adjoint.argtypes = [P(Grid), P(Medium), P(Receivers), P(AdjointArgs), vp]
+    lib.swfd_adjoint_workspace_floats.restype = ctypes.c_size_t
+    lib.swfd_adjoint_workspace_floats.argtypes = [_c_int, _c_int, _c_int]
     lib.swfd_residual_workspace_floats.restype = ctypes.c_size_t
     lib.swfd_residual_workspace_floats.argtypes = [_c_int, _c_int, _c_int]
     lib.swfd_residual.argtypes = [_c_int, _c_int, _c_int, vp, vp, ctypes.c_int64,

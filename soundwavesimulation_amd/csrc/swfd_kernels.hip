@@ -54,6 +54,9 @@ int fail(int code, const std::string &msg)
     } while (0)
 
 constexpr int kLanes = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kRows = 8;              // rows per lane strip
+constexpr int kMaxImageGroups = 8;    // shot groups with separate imaging partials
 constexpr int kXcds = 8;
 constexpr int kGatherThreads = 256;
 
@@ -77,7 +80,7 @@ struct StepParams {
     int n_rec;
     // imaging
     const float *img, *g_in;
-    float *g_out;
+    float *g_out, *g_part;
     float img_scale;
     // gather of gat_field (S, ny, nx) into gat_out (S, R)
     const float *gat_field;
@@ -217,7 +220,7 @@ __device__ __forceinline__ void gather_pairs(const StepParams &p, int t)
 // row, left column, right column. Kept out of line and called before any
 // stencil state is live, so it does not raise the stencil's register budget.
 template <bool ADJ>
-__device__ __noinline__ void edge_work(const StepParams &p, int t)
+__device__ __forceinline__ void edge_work(const StepParams &p, int t)
 {
     const int W = p.nx, H = p.ny;
     const int nb = 2 * (W + H) - 4;
@@ -257,6 +260,55 @@ enum : unsigned {
     kGather = 32u,
 };
 
+// Neighbour exchange inside a wavefront through DPP wavefront shifts: lane i
+// receives lane i-1 (west) or lane i+1 (east); the lane at the wavefront's
+// edge receives `halo` instead. Every lane must execute these (no divergence).
+__device__ __forceinline__ float from_west(float v, float halo)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(halo), __float_as_int(v),
+                                                      0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float from_east(float v, float halo)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(halo), __float_as_int(v),
+                                                      0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
+// Pointers in the global address space: loads become global_load with a
+// scalar (per-wavefront) row base and one 32-bit lane offset, instead of flat
+// loads with a 64-bit address per lane and row.
+#define SWFD_GLOBAL __attribute__((address_space(1)))
+typedef const float SWFD_GLOBAL gcf;
+typedef float SWFD_GLOBAL gf;
+typedef const int SWFD_GLOBAL gci;
+
+__device__ __forceinline__ gcf *gload_ptr(const float *p) { return (gcf *)p; }
+
+// Raw buffer view of `bytes` bytes at p (word 3 = 0x00020000 for gfx9-family
+// raw 32-bit access), and a 32-bit load at lane byte offset voff plus
+// wavefront-uniform byte offset soff.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ gf *gstore_ptr(float *p) { return (gf *)p; }
+
+// One wavefront = one work item: 64 columns x RB rows x a group of shots.
+// Lane l owns column x0 + l and walks the RB rows of the strip; vertical
+// neighbours come from its own registers, horizontal ones from the adjacent
+// lanes by DPP, and only lanes 0 and 63 load the column just outside the
+// strip. Coefficients of the strip stay in registers across the shot loop.
+// All indices except the lane's column are uniform over the wavefront, so row
+// addresses live in scalar registers. Boundary cells are skipped here and
+// written by the edge workgroups. Imaging (adjoint) adds the shot group's
+// partial sum of the previous step's lam * Lap(forward field) into its own
+// slice g_part[group].
 template <bool ADJ, int RB, unsigned F>
 __global__ void __launch_bounds__(256) step_kernel(StepParams p)
 {
@@ -272,42 +324,65 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
         }
     }
     // XCD-aware numbering: consecutive work items stay on one XCD.
-    const int w = (b % kXcds) * p.chunk + b / kXcds;
-    if (w >= p.n_work) return;
-    const int tile = w / p.n_groups;
-    const int grp = w - tile * p.n_groups;
-    const int ty = tile / p.tiles_x;
-    const int tx = tile - ty * p.tiles_x;
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x / kLanes;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
+    const int item = wblk * kWavesPerBlock + wave;
+    if (item >= p.n_work) return;  // uniform over the wavefront
+    const int grp = item % p.n_groups;
+    const int t = item / p.n_groups;
+    const int cg = t % p.tiles_x;
+    const int strip = t / p.tiles_x;
     const int W = p.nx, H = p.ny;
-    const int x = tx * kLanes + lane;
-    const int y0 = (ty * p.waves + wave) * RB;
-    if (x >= W || y0 >= H) return;
+    const int x0 = cg * kLanes;
+    const int x = x0 + lane;
+    const int y0 = strip * RB;
+    const bool xin = x < W;
+    const int hx = lane == 0 ? x0 - 1 : x0 + kLanes;
+    const bool hneed = ((lane == 0) & (x0 > 0)) | ((lane == kLanes - 1) & (x0 + kLanes < W));
     const size_t plane = (size_t)H * W;
 
-    // Medium coefficients of this strip, kept across the shot loop.
-    float coef[RB];
-    float c2col[RB + 2], c2w[RB], c2e[RB];
-    int cmap[RB];
+    // Loads are buffer loads at clamped offsets (always inside the field):
+    // one lane-offset VGPR for every row, row offsets in scalar registers,
+    // and out-of-domain neighbours zeroed by a select afterwards.
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int xo = min(x, W - 1) * 4;
+    const int ho = min(max(hx, 0), W - 1) * 4;
+    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * W * 4; };
+
+    float coef[RB];  // dt^2 c^2 (forward) or c^2 (adjoint) of the lane's cells
+    float c2v[2];    // adjoint: c^2 on the rows just above and below the strip
+    float c2h[RB];   // adjoint: c^2 on the halo column of lanes 0 / 63
+    {
+        const __amdgpu_buffer_rsrc_t cf = make_rsrc(ADJ ? p.c2 : p.c2dt2, plane_bytes);
 #pragma unroll
-    for (int i = 0; i < RB; ++i) {
-        const int yy = min(y0 + i, H - 1);
-        const size_t ci = (size_t)yy * W + x;
-        if constexpr (!ADJ) {
-            coef[i] = p.c2dt2[ci];
-        } else {
-            c2w[i] = x > 0 ? p.c2[ci - 1] : 0.0f;
-            c2e[i] = x < W - 1 ? p.c2[ci + 1] : 0.0f;
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            const float a = bload(cf, xo, row_off(yy));
+            coef[i] = (xin & (yy < H)) ? a : 0.0f;
+            if constexpr (ADJ) {
+                const float h = bload(cf, ho, row_off(yy));
+                c2h[i] = (hneed & (yy < H)) ? h : 0.0f;
+            }
         }
-        if constexpr ((F & kResid) != 0) cmap[i] = p.cell_map[ci];
+        if constexpr (ADJ) {
+            const float a = bload(cf, xo, row_off(y0 - 1));
+            const float b = bload(cf, xo, row_off(y0 + RB));
+            c2v[0] = (xin & (y0 >= 1)) ? a : 0.0f;
+            c2v[1] = (xin & (y0 + RB < H)) ? b : 0.0f;
+        }
     }
-    if constexpr (ADJ) {
+    bool inject = false;
+    if constexpr ((F & kResid) != 0) {
+        const __amdgpu_buffer_rsrc_t cm = make_rsrc((const float *)p.cell_map, plane_bytes);
+        bool mine = false;
 #pragma unroll
-        for (int i = 0; i < RB + 2; ++i) {
-            const int yy = y0 - 1 + i;
-            c2col[i] = (yy >= 0 && yy < H) ? p.c2[(size_t)yy * W + x] : 0.0f;
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            const int e = __float_as_int(bload(cm, xo, row_off(yy)));
+            mine |= xin & (yy < H) & (e >= 0);
         }
+        inject = __any(mine);
     }
     float gacc[RB];
 #pragma unroll
@@ -317,106 +392,146 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
     const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
     for (int s = s_begin; s < s_end; ++s) {
         const size_t soff = (size_t)s * plane;
-        const float *u = p.f_cur + soff;
-        const float *uo = p.f_old + soff;
-        float col[RB + 2];
+        const __amdgpu_buffer_rsrc_t ru = make_rsrc(p.f_cur + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t ro = make_rsrc(p.f_old + soff, plane_bytes);
+        float col[RB + 2], hal[RB], old[RB];
 #pragma unroll
         for (int i = 0; i < RB + 2; ++i) {
             const int yy = y0 - 1 + i;
-            col[i] = (yy >= 0 && yy < H) ? u[(size_t)yy * W + x] : 0.0f;
+            const float a = bload(ru, xo, row_off(yy));
+            col[i] = (xin & (yy >= 0) & (yy < H)) ? a : 0.0f;
         }
-        int sr = -1, sc = -1;
-        if constexpr ((F & kPoint) != 0) {
-            sr = p.src_rows[s];
-            sc = p.src_cols[s];
-        }
-        float v[RB];
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
             const int yy = y0 + i;
-            v[i] = 0.0f;
-            if (yy < H) {
-                const size_t ci = (size_t)yy * W + x;
-                const float uw = x > 0 ? u[ci - 1] : 0.0f;
-                const float ue = x < W - 1 ? u[ci + 1] : 0.0f;
-                const float old = uo[ci];
-                float val;
-                if constexpr (!ADJ) {
-                    val = __fmaf_rn(coef[i],
-                                    lap5(uw, ue, col[i], col[i + 2], col[i + 1], p.idx2,
-                                         p.idy2, p.cen),
-                                    2.0f * col[i + 1] - old);
-                } else {
-                    const float pw = c2w[i] * uw;
-                    const float pe = c2e[i] * ue;
-                    const float ps = c2col[i] * col[i];
-                    const float pn = c2col[i + 2] * col[i + 2];
-                    val = __fmaf_rn(p.dt2,
-                                    lap5(pw, pe, ps, pn, c2col[i + 1] * col[i + 1], p.idx2,
-                                         p.idy2, p.cen),
-                                    2.0f * col[i + 1] - old);
-                }
-                if constexpr ((F & kPoint) != 0) {
-                    if (yy == sr && x == sc) val += p.src_values[p.src_index];
-                }
-                if constexpr ((F & kDense) != 0) val += p.dense[soff + ci];
-                if constexpr ((F & kResid) != 0) {
-                    if (cmap[i] >= 0) {
-                        const float *res = p.resid + (size_t)s * p.n_rec;
-                        float acc = 0.0f;
-                        for (int k = p.entry_start[cmap[i]]; k < p.entry_start[cmap[i] + 1];
-                             ++k)
-                            acc += res[p.pair_rec[k]] * p.pair_w[k];
-                        val += acc;
-                    }
-                }
-                v[i] = val;
-            }
+            const float h = bload(ru, ho, row_off(yy));
+            const float o = bload(ro, xo, row_off(yy));
+            hal[i] = (hneed & (yy < H)) ? h : 0.0f;
+            old[i] = (xin & (yy < H)) ? o : 0.0f;
         }
-        float *out = p.f_out + soff;
-#pragma unroll
-        for (int i = 0; i < RB; ++i) {
-            const int yy = y0 + i;
-            const bool boundary = (x == 0) || (x == W - 1) || (yy == 0) || (yy == H - 1);
-            if (yy < H && !boundary) {
-                out[(size_t)yy * W + x] = v[i];
-                if constexpr ((F & kOut2) != 0) p.f_out2[soff + (size_t)yy * W + x] = v[i];
-            }
-        }
-        // Imaging of the previous (later-in-time) adjoint step: its field is
-        // this step's input u, already Mur-updated, held in col[].
+        // forward field of the imaging condition, loaded with the rest
+        float fc[RB + 2], fh[RB];
         if constexpr (ADJ && (F & kImage) != 0) {
-            const float *f = p.img + soff;
-            float fc[RB + 2];
+            const __amdgpu_buffer_rsrc_t rf = make_rsrc(p.img + soff, plane_bytes);
 #pragma unroll
             for (int i = 0; i < RB + 2; ++i) {
                 const int yy = y0 - 1 + i;
-                fc[i] = (yy >= 0 && yy < H) ? f[(size_t)yy * W + x] : 0.0f;
+                const float a = bload(rf, xo, row_off(yy));
+                fc[i] = (xin & (yy >= 0) & (yy < H)) ? a : 0.0f;
             }
 #pragma unroll
             for (int i = 0; i < RB; ++i) {
                 const int yy = y0 + i;
-                if (yy < H) {
-                    const size_t ci = (size_t)yy * W + x;
-                    const float fw = x > 0 ? f[ci - 1] : 0.0f;
-                    const float fe = x < W - 1 ? f[ci + 1] : 0.0f;
-                    const float lf =
-                        lap5(fw, fe, fc[i], fc[i + 2], fc[i + 1], p.idx2, p.idy2, p.cen);
-                    gacc[i] = __fmaf_rn(col[i + 1], lf, gacc[i]);
+                const float h = bload(rf, ho, row_off(yy));
+                fh[i] = (hneed & (yy < H)) ? h : 0.0f;
+            }
+        }
+        float v[RB];
+        if constexpr (!ADJ) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const float uw = from_west(col[i + 1], hal[i]);
+                const float ue = from_east(col[i + 1], hal[i]);
+                v[i] = __fmaf_rn(coef[i],
+                                 lap5(uw, ue, col[i], col[i + 2], col[i + 1], p.idx2, p.idy2,
+                                      p.cen),
+                                 2.0f * col[i + 1] - old[i]);
+            }
+        } else {
+            float pc[RB + 2];
+            pc[0] = c2v[0] * col[0];
+            pc[RB + 1] = c2v[1] * col[RB + 1];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) pc[i + 1] = coef[i] * col[i + 1];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const float ph = c2h[i] * hal[i];
+                const float pw = from_west(pc[i + 1], ph);
+                const float pe = from_east(pc[i + 1], ph);
+                v[i] = __fmaf_rn(p.dt2,
+                                 lap5(pw, pe, pc[i], pc[i + 2], pc[i + 1], p.idx2, p.idy2, p.cen),
+                                 2.0f * col[i + 1] - old[i]);
+            }
+        }
+        if constexpr ((F & kPoint) != 0) {
+            const int sr = p.src_rows[s], sc = p.src_cols[s];
+            if (sr >= y0 && sr < y0 + RB && x == sc) {
+                const float val = p.src_values[p.src_index];
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    if (y0 + i == sr) v[i] += val;
+            }
+        }
+        if constexpr ((F & kDense) != 0) {
+            const __amdgpu_buffer_rsrc_t rd = make_rsrc(p.dense + soff, plane_bytes);
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const int yy = y0 + i;
+                const float d = bload(rd, xo, row_off(yy));
+                v[i] += (xin & (yy < H)) ? d : 0.0f;
+            }
+        }
+        if constexpr ((F & kResid) != 0) {
+            if (inject) {
+                const float *res = p.resid + (size_t)s * p.n_rec;
+#pragma unroll
+                for (int i = 0; i < RB; ++i) {
+                    const int yy = y0 + i;
+                    const int e = (xin & (yy < H)) ? p.cell_map[(size_t)yy * W + x] : -1;
+                    if (e >= 0) {
+                        float acc = 0.0f;
+                        for (int k = p.entry_start[e]; k < p.entry_start[e + 1]; ++k)
+                            acc += res[p.pair_rec[k]] * p.pair_w[k];
+                        v[i] += acc;
+                    }
                 }
+            }
+        }
+        gf *out = gstore_ptr(p.f_out) + soff;
+        const bool xint = xin && x > 0 && x < W - 1;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int yy = y0 + i;
+            if (xint && yy > 0 && yy < H - 1) {
+                const size_t ci = (size_t)yy * W + x;
+                out[ci] = v[i];
+                if constexpr ((F & kOut2) != 0) (gstore_ptr(p.f_out2) + soff)[ci] = v[i];
+            }
+        }
+        // Imaging of the previous (later-in-time) adjoint step: its field is
+        // this step's input u (already Mur-updated), held in col[].
+        if constexpr (ADJ && (F & kImage) != 0) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const float fw = from_west(fc[i + 1], fh[i]);
+                const float fe = from_east(fc[i + 1], fh[i]);
+                const float lf = lap5(fw, fe, fc[i], fc[i + 2], fc[i + 1], p.idx2, p.idy2, p.cen);
+                gacc[i] = __fmaf_rn(col[i + 1], lf, gacc[i]);
             }
         }
     }
     if constexpr (ADJ && (F & kImage) != 0) {
+        gf *gp = gstore_ptr(p.g_part) + (size_t)grp * plane;
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
             const int yy = y0 + i;
-            if (yy < H) {
-                const size_t ci = (size_t)yy * W + x;
-                p.g_out[ci] = p.g_in[ci] + p.img_scale * gacc[i];
+            if (xin & (yy < H)) {
+                gf *cell = gp + (size_t)yy * W + x;
+                *cell = __fmaf_rn(p.img_scale, gacc[i], *cell);
             }
         }
     }
+}
+
+// g[i] += sum over groups of g_part[k][i], in group order.
+__global__ void __launch_bounds__(256) reduce_groups_kernel(size_t plane, int n_groups,
+                                                            const float *g_part, float *g)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane) return;
+    float acc = 0.0f;
+    for (int k = 0; k < n_groups; ++k) acc += g_part[(size_t)k * plane + i];
+    g[i] += acc;
 }
 
 __global__ void __launch_bounds__(kGatherThreads) gather_kernel(StepParams p)
@@ -602,7 +717,7 @@ __global__ void __launch_bounds__(256) misfit_kernel(int n_partials, const float
 // ---------------------------------------------------------------------------
 
 struct LaunchShape {
-    int rb, waves, tiles_x, tiles_y, n_groups, shots_per_group;
+    int tiles_x, strips, n_groups, shots_per_group;
 };
 
 int g_cu_count = 0;
@@ -621,36 +736,20 @@ int cu_count()
     return g_cu_count;
 }
 
-// Tile shape and shot grouping for an (ny, nx, S) step. With imaging every
-// workgroup owns all shots of its tile (the shot sum stays in registers, in a
-// fixed order), so parallelism comes from tiles alone; otherwise shots are
-// split into groups until the launch holds about 16 waves per CU.
+// Work items of an (ny, nx, S) step: 64-column groups x kRows-row strips x
+// shot groups. Shots are split until the launch holds about 32 wavefronts per
+// CU; with imaging at most kMaxImageGroups groups (one partial each).
 LaunchShape choose_shape(int ny, int nx, int n_shots, bool image)
 {
     LaunchShape sh{};
     sh.tiles_x = (nx + kLanes - 1) / kLanes;
-    const long target_waves = 16L * cu_count();
-    const int rbs[3] = {4, 4, 2};
-    const int wvs[3] = {4, 2, 1};
-    int pick = 0;
-    if (image) {
-        for (pick = 0; pick < 2; ++pick) {
-            const int rows = rbs[pick] * wvs[pick];
-            const long tiles = (long)sh.tiles_x * ((ny + rows - 1) / rows);
-            if (tiles * wvs[pick] >= target_waves) break;
-        }
-    }
-    sh.rb = rbs[pick];
-    sh.waves = wvs[pick];
-    const int rows = sh.rb * sh.waves;
-    sh.tiles_y = (ny + rows - 1) / rows;
-    const long tiles = (long)sh.tiles_x * sh.tiles_y;
-    int groups = 1;
-    if (!image) {
-        const long want = (target_waves + tiles * sh.waves - 1) / (tiles * sh.waves);
-        groups = (int)std::min<long>(n_shots, std::max<long>(1, want));
-    }
-    sh.shots_per_group = (n_shots + groups - 1) / groups;
+    sh.strips = (ny + kRows - 1) / kRows;
+    const long cells = (long)sh.tiles_x * sh.strips;
+    const long target = 32L * cu_count();
+    long groups = (target + cells - 1) / cells;
+    groups = std::max<long>(1, std::min<long>(groups, n_shots));
+    if (image) groups = std::min<long>(groups, kMaxImageGroups);
+    sh.shots_per_group = (int)((n_shots + groups - 1) / groups);
     sh.n_groups = (n_shots + sh.shots_per_group - 1) / sh.shots_per_group;
     return sh;
 }
@@ -660,9 +759,10 @@ void fill_geometry(StepParams &p, const LaunchShape &sh)
     p.tiles_x = sh.tiles_x;
     p.n_groups = sh.n_groups;
     p.shots_per_group = sh.shots_per_group;
-    p.waves = sh.waves;
-    p.n_work = sh.tiles_x * sh.tiles_y * sh.n_groups;
-    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    p.waves = kWavesPerBlock;
+    p.n_work = sh.tiles_x * sh.strips * sh.n_groups;
+    const int blocks = (p.n_work + kWavesPerBlock - 1) / kWavesPerBlock;
+    p.chunk = (blocks + kXcds - 1) / kXcds;
     p.n_stencil_blocks = p.chunk * kXcds;
 }
 
@@ -749,27 +849,23 @@ int launch_step(StepParams &p, bool adjoint, hipStream_t stream, int gather_pair
     const bool image = adjoint && p.img != nullptr;
     const LaunchShape sh = choose_shape(p.ny, p.nx, p.n_shots, image);
     fill_geometry(p, sh);
-    int gather_blocks = gather_pairs_count > 0 ? (gather_pairs_count + 255) / 256 : 0;
-    const dim3 block(kLanes * sh.waves);
-    if (gather_blocks > 0 && block.x != 256) {
-        // gather workgroups assume 256 lanes: use a separate launch instead
-        hipLaunchKernelGGL(gather_kernel, dim3(gather_blocks), dim3(256), 0, stream, p);
-        gather_blocks = 0;
-    }
+    const int gather_blocks = gather_pairs_count > 0 ? (gather_pairs_count + 255) / 256 : 0;
+    const dim3 block(kLanes * kWavesPerBlock);
     const long edge_items = (long)(2 * (p.nx + p.ny) - 4) * p.n_shots;
     p.n_edge_blocks = (int)((edge_items + block.x - 1) / block.x);
     const dim3 grid(p.n_edge_blocks + p.n_stencil_blocks + gather_blocks);
     const unsigned f = step_flags(p, gather_blocks > 0);
-    bool ok;
-    if (adjoint)
-        ok = sh.rb == 4 ? launch_variant<true, 4>(f, grid, block, stream, p)
-                        : launch_variant<true, 2>(f, grid, block, stream, p);
-    else
-        ok = sh.rb == 4 ? launch_variant<false, 4>(f, grid, block, stream, p)
-                        : launch_variant<false, 2>(f, grid, block, stream, p);
+    const bool ok = adjoint ? launch_variant<true, kRows>(f, grid, block, stream, p)
+                            : launch_variant<false, kRows>(f, grid, block, stream, p);
     if (!ok) return fail(SWFD_EINVAL, "unsupported combination of step sources / outputs");
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
+}
+
+// Number of imaging partial slices a step of this shape uses.
+int image_groups(int ny, int nx, int n_shots)
+{
+    return choose_shape(ny, nx, n_shots, true).n_groups;
 }
 
 // Imaging of the field `lam` (S, ny, nx) against `img` into g_out.
@@ -1083,8 +1179,13 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
     if ((rc = check_receivers(rec, true))) return rc;
     if (a->hist != nullptr && (a->grad == nullptr || a->hist_every < 1 || a->hist_slots < 1))
         return fail(SWFD_EINVAL, "adjoint: imaging needs grad, hist_every and hist_slots");
+    if (a->hist != nullptr && a->workspace == nullptr)
+        return fail(SWFD_EINVAL, "adjoint: imaging needs the workspace");
     hipStream_t st = as_stream(stream);
     const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
+    const size_t plane = (size_t)grid->ny * grid->nx;
+    const int n_part = image_groups(grid->ny, grid->nx, a->n_shots);
+    bool partials_used = false;
     const size_t sr = (size_t)a->n_shots * rec->n;
     float *const work[3] = {a->work0, a->work1, a->work2};
     const float *p_old = a->init_old;
@@ -1113,15 +1214,26 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
             if (slot >= a->hist_slots)
                 return fail(SWFD_EINVAL, "adjoint: history slot out of range");
             if (slot >= 0) {
+                if (!partials_used) {
+                    const size_t nz = (size_t)n_part * plane;
+                    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256),
+                                       0, st, a->workspace, nz, 0.0f);
+                    SWFD_HIP_CHECK(hipGetLastError());
+                    partials_used = true;
+                }
                 p.img = a->hist + (size_t)slot * field;
-                p.g_in = a->grad;
-                p.g_out = a->grad;
+                p.g_part = a->workspace;
                 p.img_scale = a->image_scale;
             }
         }
         if ((rc = launch_step(p, true, st, 0))) return rc;
         p_old = p_cur;
         p_cur = out;
+    }
+    if (partials_used) {
+        hipLaunchKernelGGL(reduce_groups_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256),
+                           0, st, plane, n_part, a->workspace, a->grad);
+        SWFD_HIP_CHECK(hipGetLastError());
     }
     if (a->step_hi > a->step_lo) {
         const long slot = image_slot(a->step_lo);
@@ -1132,6 +1244,11 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
             return rc;
     }
     return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+}
+
+size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots)
+{
+    return (size_t)kMaxImageGroups * (size_t)ny * (size_t)nx;
 }
 
 size_t swfd_residual_workspace_floats(int32_t n_steps, int32_t n_shots, int32_t n_rec)

@@ -24,7 +24,7 @@ import torch
 
 from .. import _native
 from ..torch_backend import (TorchFDTD2DBatch, _source_values_tensor, adjoint_run,
-                             auto_batch_size, forward_run, simulate_shot)
+                             adjoint_workspace, auto_batch_size, forward_run)
 from .medium import c_from_m, m_from_c
 from .misfit import tikhonov, total_variation
 
@@ -123,6 +123,7 @@ class CheckpointedGradient:
         self.ckpt = (torch.empty((max(n_ckpt, 2), *shape), device=dev, dtype=_F32)
                      if n_ckpt > 0 else None)
         self.hist = torch.empty((n_hist, *shape), device=dev, dtype=_F32)
+        self.workspace = adjoint_workspace(solver, self.n_shots)
         self.samples = torch.empty((self.n_steps, self.n_shots, sampler.n_elements),
                                    device=dev, dtype=_F32)
         self.rows = None
@@ -161,7 +162,8 @@ class CheckpointedGradient:
                             final=False)
             adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
                         state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
-                        hist_every=1, image_scale=image_scale, grad=grad)
+                        hist_every=1, image_scale=image_scale, grad=grad,
+                        workspace=self.workspace)
         return grad
 
 

@@ -499,8 +499,15 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
     return (work[0], work[1]) if final else None
 
 
+def adjoint_workspace(solver, n_shots):
+    ny, nx = solver.model.shape
+    n = int(_native.lib().swfd_adjoint_workspace_floats(ny, nx, n_shots))
+    return torch.empty(n, device=solver.model.device, dtype=_F32)
+
+
 def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, state, work,
-                hist=None, hist_first=0, hist_every=1, image_scale=0.0, grad=None):
+                hist=None, hist_first=0, hist_every=1, image_scale=0.0, grad=None,
+                workspace=None):
     """Adjoint steps ``step_hi-1 .. step_lo`` in one native call.
 
     ``state`` = (lam_old, lam) before the run; the state after the run is
@@ -509,6 +516,8 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
     """
     dev = solver.model.device
     solver._sync_medium()
+    if hist is not None and workspace is None:
+        workspace = adjoint_workspace(solver, n_shots)
     args = _native.AdjointArgs(
         n_shots=n_shots, step_hi=step_hi, step_lo=step_lo,
         init_old=_native.ptr(state[0]), init_u=_native.ptr(state[1]),
@@ -516,7 +525,8 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
         final_old=_native.ptr(state[0]), final_u=_native.ptr(state[1]),
         residual=_native.ptr(residual_nsr), hist=_native.ptr(hist), hist_first=hist_first,
         hist_every=hist_every, hist_slots=0 if hist is None else hist.shape[0],
-        image_scale=float(image_scale), grad=_native.ptr(grad))
+        image_scale=float(image_scale), grad=_native.ptr(grad),
+        workspace=_native.ptr(workspace))
     _native.check(_native.lib().swfd_adjoint(solver._grid, solver._medium, sampler._rec, args,
                                              _native.stream_of(dev)))
 

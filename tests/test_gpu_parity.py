@@ -187,7 +187,7 @@ def test_store_every_two_matches_reference_approximation():
     assert err <= 5e-4
 
 
-def test_checkpointed_gradient_equals_stored_history_bitwise():
+def test_checkpointed_gradient_matches_stored_history():
     g32 = load("ring41_fp32.npz")
     _, array, solver, sampler = ring41_setup(c_from_m(g32["m0"]))
     N = int(g32["n_steps"])
@@ -217,7 +217,21 @@ def test_checkpointed_gradient_equals_stored_history_bitwise():
     problem.store_every = 1
     problem._adjoint_accumulate_batch(batched, wave, residual, g_stored, residual_layout="nsr")
     for g in grads:
-        assert torch.equal(g, g_stored)
+        # same images; only the order of the per-segment partial sums differs
+        assert rel(g.cpu().numpy(), g_stored.cpu().numpy()) < 1e-6
+    # recomputation reproduces the stored forward fields bit for bit
+    eng = CheckpointedGradient(solver, sampler, 6, N, checkpoint_every=16)
+    eng.forward(rows, cols, values)
+    from soundwavesimulation_amd.torch_backend import forward_run
+    j = 3
+    n0 = j * 16
+    eng.hist[0].copy_(eng.ckpt[2 * (j - 1)])
+    eng.hist[1].copy_(eng.ckpt[2 * (j - 1) + 1])
+    forward_run(solver, 6, rows, cols, values, 15, init=(eng.hist[0], eng.hist[1]),
+                hist=eng.hist, hist_first=n0 - 2, hist_every=1, step_begin=n0, work=eng.work,
+                final=False)
+    for k in range(17):
+        assert torch.equal(eng.hist[k], wave[n0 - 2 + k]), k
 
 
 def test_grad128_matches_reference():
