@@ -56,8 +56,8 @@ def parse():
     p.add_argument("--parallel-shots", default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-torch-reference", action="store_true")
-    p.add_argument("--cpu-shots", type=int, default=2)
-    p.add_argument("--cpu-steps", type=int, default=300)
+    p.add_argument("--cpu-shots", type=int, default=4)
+    p.add_argument("--cpu-steps", type=int, default=1000)
     p.add_argument("--dry-run", action="store_true",
                    help="build the host-side geometry only and print it (no GPU)")
     a = p.parse_args()

@@ -13,7 +13,8 @@ import os
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY = os.path.join(HERE, "libswfd.so")
+# SWFD_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIBRARY = os.environ.get("SWFD_LIBRARY") or os.path.join(HERE, "libswfd.so")
 
 SWFD_OK = 0
 SWFD_EINVAL = 1

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -90,6 +91,7 @@ struct StepParams {
     // launch geometry
     int tiles_x, n_groups, shots_per_group, waves;
     int n_work, chunk, n_stencil_blocks, n_edge_blocks;
+    int rows, order;
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -299,6 +301,32 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int s
 }
 __device__ __forceinline__ gf *gstore_ptr(float *p) { return (gf *)p; }
 
+// Work-item order (p.order): 0 = strip, column group, shot group (fastest);
+// 1 = shot group, strip, column group (fastest): a workgroup's wavefronts
+// cover adjacent columns and consecutive strips follow each other;
+// 2 = strip, shot group, column group (fastest).
+__device__ __forceinline__ void decode_item(const StepParams &p, int item, int &grp, int &cg,
+                                            int &strip)
+{
+    const int strips = (p.ny + p.rows - 1) / p.rows;
+    if (p.order == 1) {
+        cg = item % p.tiles_x;
+        const int t = item / p.tiles_x;
+        strip = t % strips;
+        grp = t / strips;
+    } else if (p.order == 2) {
+        cg = item % p.tiles_x;
+        const int t = item / p.tiles_x;
+        grp = t % p.n_groups;
+        strip = t / p.n_groups;
+    } else {
+        grp = item % p.n_groups;
+        const int t = item / p.n_groups;
+        cg = t % p.tiles_x;
+        strip = t / p.tiles_x;
+    }
+}
+
 // One wavefront = one work item: 64 columns x RB rows x a group of shots.
 // Lane l owns column x0 + l and walks the RB rows of the strip; vertical
 // neighbours come from its own registers, horizontal ones from the adjacent
@@ -329,10 +357,8 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
     const int item = wblk * kWavesPerBlock + wave;
     if (item >= p.n_work) return;  // uniform over the wavefront
-    const int grp = item % p.n_groups;
-    const int t = item / p.n_groups;
-    const int cg = t % p.tiles_x;
-    const int strip = t / p.tiles_x;
+    int grp, cg, strip;
+    decode_item(p, item, grp, cg, strip);
     const int W = p.nx, H = p.ny;
     const int x0 = cg * kLanes;
     const int x = x0 + lane;
@@ -739,14 +765,26 @@ int cu_count()
 // Work items of an (ny, nx, S) step: 64-column groups x kRows-row strips x
 // shot groups. Shots are split until the launch holds about 32 wavefronts per
 // CU; with imaging at most kMaxImageGroups groups (one partial each).
+int env_int(const char *name, int fallback)
+{
+    const char *v = std::getenv(name);
+    return (v != nullptr && *v != '\0') ? std::atoi(v) : fallback;
+}
+
+// Launch-shape overrides for tuning sweeps (0 = automatic):
+// SWFD_SHOTS_PER_GROUP fixes the shots per wavefront work item,
+// SWFD_WAVES_PER_CU sets the target work items per CU.
 LaunchShape choose_shape(int ny, int nx, int n_shots, bool image)
 {
+    static const int spg_override = env_int("SWFD_SHOTS_PER_GROUP", 0);
+    static const int waves_per_cu = env_int("SWFD_WAVES_PER_CU", 128);
     LaunchShape sh{};
     sh.tiles_x = (nx + kLanes - 1) / kLanes;
     sh.strips = (ny + kRows - 1) / kRows;
     const long cells = (long)sh.tiles_x * sh.strips;
-    const long target = 32L * cu_count();
+    const long target = (long)waves_per_cu * cu_count();
     long groups = (target + cells - 1) / cells;
+    if (spg_override > 0) groups = (n_shots + spg_override - 1) / spg_override;
     groups = std::max<long>(1, std::min<long>(groups, n_shots));
     if (image) groups = std::min<long>(groups, kMaxImageGroups);
     sh.shots_per_group = (int)((n_shots + groups - 1) / groups);
@@ -760,6 +798,9 @@ void fill_geometry(StepParams &p, const LaunchShape &sh)
     p.n_groups = sh.n_groups;
     p.shots_per_group = sh.shots_per_group;
     p.waves = kWavesPerBlock;
+    p.rows = kRows;
+    static const int order = env_int("SWFD_ORDER", 1);
+    p.order = order;
     p.n_work = sh.tiles_x * sh.strips * sh.n_groups;
     const int blocks = (p.n_work + kWavesPerBlock - 1) / kWavesPerBlock;
     p.chunk = (blocks + kXcds - 1) / kXcds;
