@@ -97,8 +97,10 @@ typedef struct swfd_step_args {
  * sources (simulate_shot torch_backend.py:443-474, simulate_shots_batch
  * :736-770, forward_store(_batch) invert_ring.py:395-455).
  *   init_old/init_u: state before step_begin (read only).
- *   work0..2: scratch fields; final_old/final_u (optional) receive the state
- *   after the last step (they may be work0/work1 or the init buffers).
+ *   work0..2: scratch fields; work3 (optional) a fourth one: with it the run
+ *   advances two steps per launch (same results, fewer field reads).
+ *   final_old/final_u (optional) receive the state after the last step (they
+ *   may be work0/work1 or the init buffers).
  *   src_values: per global step, length >= step_begin + n_steps.
  *   samples (optional): (N_total, S, R), row n = gather after step n.
  *   hist (optional): output of step n is stored in slot (n - hist_first) /
@@ -109,7 +111,7 @@ typedef struct swfd_forward_args {
     int32_t n_shots;
     int32_t step_begin, n_steps;
     const float *init_old, *init_u;
-    float *work0, *work1, *work2;
+    float *work0, *work1, *work2, *work3;
     float *final_old, *final_u;
     const int32_t *src_rows, *src_cols;
     const float *src_values;
@@ -134,7 +136,7 @@ typedef struct swfd_adjoint_args {
     int32_t n_shots;
     int32_t step_hi, step_lo;
     const float *init_old, *init_u;
-    float *work0, *work1, *work2;
+    float *work0, *work1, *work2, *work3; /* work3 optional, as for swfd_forward */
     float *final_old, *final_u;
     const float *residual;
     const float *hist;

@@ -51,7 +51,7 @@ class StepArgs(ctypes.Structure):
 class ForwardArgs(ctypes.Structure):
     _fields_ = [("n_shots", _c_int), ("step_begin", _c_int), ("n_steps", _c_int),
                 ("init_old", _fp), ("init_u", _fp), ("work0", _fp), ("work1", _fp),
-                ("work2", _fp), ("final_old", _fp), ("final_u", _fp), ("src_rows", _fp),
+                ("work2", _fp), ("work3", _fp), ("final_old", _fp), ("final_u", _fp), ("src_rows", _fp),
                 ("src_cols", _fp), ("src_values", _fp), ("samples", _fp), ("hist", _fp),
                 ("hist_first", _c_int), ("hist_every", _c_int), ("hist_slots", _c_int),
                 ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int)]
@@ -60,7 +60,7 @@ class ForwardArgs(ctypes.Structure):
 class AdjointArgs(ctypes.Structure):
     _fields_ = [("n_shots", _c_int), ("step_hi", _c_int), ("step_lo", _c_int),
                 ("init_old", _fp), ("init_u", _fp), ("work0", _fp), ("work1", _fp),
-                ("work2", _fp), ("final_old", _fp), ("final_u", _fp), ("residual", _fp),
+                ("work2", _fp), ("work3", _fp), ("final_old", _fp), ("final_u", _fp), ("residual", _fp),
                 ("hist", _fp), ("hist_first", _c_int), ("hist_every", _c_int),
                 ("hist_slots", _c_int), ("image_scale", _c_float), ("grad", _fp),
                 ("workspace", _fp)]

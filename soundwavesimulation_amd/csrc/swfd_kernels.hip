@@ -92,6 +92,12 @@ struct StepParams {
     int tiles_x, n_groups, shots_per_group, waves;
     int n_work, chunk, n_stencil_blocks, n_edge_blocks;
     int rows, order;
+    // two-step launches: outputs of the second step, a second gather, and
+    // the marching geometry (column blocks of 62, row segments)
+    float *f_out_b, *f_out2_b;
+    const float *gat_field2;
+    float *gat_out2;
+    int n_cb, n_seg, seg;
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -203,18 +209,26 @@ __device__ float boundary_value(const StepParams &p, int s, size_t soff, int y, 
 __device__ __forceinline__ void gather_pairs(const StepParams &p, int t)
 {
     const int R = p.n_rec;
-    if (t >= p.n_shots * R) return;
+    const int total = p.n_shots * R;
+    const float *field = p.gat_field;
+    float *dest = p.gat_out;
+    if (t >= total) {
+        t -= total;
+        if (p.gat_field2 == nullptr || t >= total) return;
+        field = p.gat_field2;
+        dest = p.gat_out2;
+    }
     const int s = t / R;
     const int r = t - s * R;
     const int W = p.nx;
-    const float *f = p.gat_field + (size_t)s * ((size_t)p.ny * W);
+    const float *f = field + (size_t)s * ((size_t)p.ny * W);
     const size_t i = (size_t)p.rec_row0[r] * W + p.rec_col0[r];
     const float *w = p.rec_w;
     float acc = f[i] * w[r];
     acc = acc + f[i + 1] * w[R + r];
     acc = acc + f[i + W] * w[2 * R + r];
     acc = acc + f[i + W + 1] * w[3 * R + r];
-    p.gat_out[t] = acc;
+    dest[t] = acc;
 }
 
 // Boundary ring of one (shot, cell) item: item t covers shot t / NB and the
@@ -547,6 +561,332 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Two forward steps per launch (temporal blocking)
+// ---------------------------------------------------------------------------
+//
+// A wavefront covers 64 columns [xs, xs+64), one per lane, and a segment of
+// rows [r0, r1), and marches down the rows for each shot of its group. The
+// first step (u^{n+1} = C from u^{n-1} = A, u^n = B) runs one row ahead of the
+// second (u^{n+2} = D from B, C), so each B, A and coefficient row is loaded
+// once, and per two steps the kernel reads two fields and writes two, in whole
+// 256-byte rows. The second step needs C one column beyond each side of the
+// block: lanes 0 and 63 compute those two halo columns as well (same
+// arithmetic, neighbours from their own registers and one extra load), with
+// their outer B neighbours loaded by lanes 1 and 62. Mur edges, corners and
+// point sources are applied in the march with exactly the single-step
+// kernel's arithmetic, so results are bit-identical to two single steps; the
+// top row, whose Mur update needs the row below, is finalised one row later.
+// Blocks start at multiples of 64, except that the last block ends at column
+// W-1 (it may overlap the previous block and stores only its own columns), so
+// column W-1 is never on lane 0 and its Mur update finds its west neighbour.
+
+constexpr int kSegRows = 64;         // rows a two-step wavefront marches
+#ifndef SWFD_PREFETCH_ROWS
+#define SWFD_PREFETCH_ROWS 2
+#endif
+constexpr int kPf = SWFD_PREFETCH_ROWS;  // rows loaded ahead of the march
+
+// Mur update of a top/bottom edge cell (x in [1, W-2]) and of the corners of
+// that row, from the row's current-time values `cur_r`, the interior-side
+// row's current and next values `cur_in`, `nxt_in`, and the side edge values
+// of the interior-side row (`nxt_in` at x = 0 / W-1 already Mur-updated).
+// `ci` selects corners 0/1 (top) or 2/3 (bottom). Every lane must call it.
+__device__ __forceinline__ float mur_row(const float *mur, int W, int H, int x, float k_row_x,
+                                         float cur_r, float cur_in, float nxt_in, int ci)
+{
+    // edge cells: u'[r][x] = u[in][x] + k (u'[in][x] - u[r][x])
+    float v = cur_in + k_row_x * (nxt_in - cur_r);
+    const float ve = from_east(v, 0.0f), vw = from_west(v, 0.0f);
+    const float ce = from_east(cur_r, 0.0f), cw = from_west(cur_r, 0.0f);
+    const float *kx = mur + 2 * H + 2 * W;
+    const float *ky = kx + 4;
+    if (x == 0) {
+        const float from_x = ce + kx[ci] * (ve - cur_r);
+        const float from_y = cur_in + ky[ci] * (nxt_in - cur_r);
+        v = 0.5f * (from_x + from_y);
+    } else if (x == W - 1) {
+        const float from_x = cw + kx[ci + 1] * (vw - cur_r);
+        const float from_y = cur_in + ky[ci + 1] * (nxt_in - cur_r);
+        v = 0.5f * (from_x + from_y);
+    }
+    return v;
+}
+
+// Mur update of the left / right edge cells of an interior row (1 <= y <= H-2);
+// k_side is the row's left coefficient on column 0 and its right coefficient
+// on column W-1 (other lanes ignore it).
+__device__ __forceinline__ float mur_sides(float v, float cur, int x, int W, float k_side)
+{
+    const float ve = from_east(v, 0.0f), vw = from_west(v, 0.0f);
+    const float ce = from_east(cur, 0.0f), cw = from_west(cur, 0.0f);
+    if (x == 0) return ce + k_side * (ve - cur);
+    if (x == W - 1) return cw + k_side * (vw - cur);
+    return v;
+}
+
+template <bool EDGE, unsigned F>
+__device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp, int cb, int seg)
+{
+    const int W = p.nx, H = p.ny;
+    const int own0 = cb * kLanes;
+    const int xs = min(own0, max(W - kLanes, 0));
+    const int x = xs + lane;
+    const bool xin = x < W;
+    const bool owned = xin & (x >= own0);
+    const int xo = min(x, W - 1) * 4;
+    // halo columns (lane 0: xs-1, lane 63: xs+64) and the B column loaded
+    // beside them (lane 1: xs-2, lane 62: xs+65); other lanes use their own
+    const bool hl = lane == 0, hr = lane == kLanes - 1;
+    const int xh = hl ? xs - 1 : (hr ? xs + kLanes : x);
+    const int xq = lane == 1 ? xs - 2 : (lane == kLanes - 2 ? xs + kLanes + 1 : xh);
+    const bool xh_in = (xh >= 0) & (xh < W);
+    const bool xq_in = (xq >= 0) & (xq < W);
+    const int ho = min(max(xh, 0), W - 1) * 4;
+    const int qo = min(max(xq, 0), W - 1) * 4;
+    const int r0 = seg * p.seg;
+    const int r1 = (seg == p.n_seg - 1) ? H : r0 + p.seg;
+    const int y_start = r0 == 0 ? 0 : r0 - 1;
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * W * 4; };
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2dt2, plane_bytes);
+    const float *mur = p.mur;
+    // per-row side coefficients: mur[y] (left) for column 0, mur[H + y]
+    // (right) for the other lanes, read as a row slot like the fields
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
+    const int mo = (x == 0) ? 0 : H * 4;
+    auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
+    float kb_x = 0.0f, kt_x = 0.0f, kb_h = 0.0f, kt_h = 0.0f;
+    // corner coefficients of column W-1 for the right halo
+    float kx1 = 0.0f, ky1 = 0.0f, kx3 = 0.0f, ky3 = 0.0f;
+    const bool h_right_edge = hr & (xh == W - 1);
+    if constexpr (EDGE) {
+        kb_x = mur[2 * H + xo / 4];
+        kt_x = mur[2 * H + W + xo / 4];
+        kb_h = mur[2 * H + ho / 4];
+        kt_h = mur[2 * H + W + ho / 4];
+        const float *kx = mur + 2 * H + 2 * W;
+        kx1 = kx[1];
+        ky1 = kx[4 + 1];
+        kx3 = kx[3];
+        ky3 = kx[4 + 3];
+    }
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        gf *outc = gstore_ptr(p.f_out) + soff;
+        gf *outd = gstore_ptr(p.f_out_b) + soff;
+        int sr = -1, sc = -1;
+        float s1 = 0.0f, s2 = 0.0f;
+        if constexpr ((F & kPoint) != 0) {
+            sr = p.src_rows[s];
+            sc = p.src_cols[s];
+            s1 = p.src_values[p.src_index];
+            s2 = p.src_values[p.src_index + 1];
+        }
+        float bm, b0, qm1, q0;
+        {
+            const float t0 = bload(rb, xo, row_off(y_start - 1));
+            const float t1 = bload(rb, xo, row_off(y_start));
+            const float t2 = bload(rb, qo, row_off(y_start - 1));
+            const float t3 = bload(rb, qo, row_off(y_start));
+            bm = (xin & (y_start >= 1)) ? t0 : 0.0f;
+            b0 = xin ? t1 : 0.0f;
+            qm1 = (xq_in & (y_start >= 1)) ? t2 : 0.0f;
+            q0 = xq_in ? t3 : 0.0f;
+        }
+        // Rows arrive kPf iterations ahead in kPf register slots used in
+        // turn: a slot is consumed and refilled in place, so no load in
+        // flight is copied and the wait for it is never forced early. Slot
+        // loads: B at row y+1 (block and halo columns), A and the
+        // coefficients at row y (block and halo columns). Row offsets are
+        // clamped, so loads past the last row read valid memory; their
+        // values are masked. (Slots are issued in order, as in the steady
+        // state, so the first row waits only for its own slot.)
+        float qb[kPf], qq[kPf], qa[kPf], qk[kPf], qha[kPf], qhk[kPf], qm[kPf];
+#pragma unroll
+        for (int i = 0; i < kPf; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            qb[i] = bload(rb, xo, row_off(y_start + 1 + i));
+            qq[i] = bload(rb, qo, row_off(y_start + 1 + i));
+            qa[i] = bload(ra, xo, row_off(y_start + i));
+            qk[i] = bload(rk, xo, row_off(y_start + i));
+            qha[i] = bload(ra, ho, row_off(y_start + i));
+            qhk[i] = bload(rk, ho, row_off(y_start + i));
+            qm[i] = 0.0f;
+            if constexpr (EDGE) qm[i] = bload(rm, mo, mrow(y_start + i));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        float cm2 = 0.0f, cm1 = 0.0f, km1 = 0.0f, dm1 = 0.0f, sm1 = 0.0f, hcm1 = 0.0f;
+        auto row = [&](int y, float &sb, float &sq, float &sa, float &sk, float &sha, float &shk,
+                       float &sm) {
+            const bool row_in = y < H;
+            float bp = (xin & (y + 1 < H)) ? sb : 0.0f;
+            float qp = (xq_in & (y + 1 < H)) ? sq : 0.0f;
+            float a0 = (xin & row_in) ? sa : 0.0f;
+            float k0 = (xin & row_in) ? sk : 0.0f;
+            float ha0 = (xh_in & row_in) ? sha : 0.0f;
+            float hk0 = (xh_in & row_in) ? shk : 0.0f;
+            float s0 = sm;
+            // Materialise the values here (empty asm, no instruction) and
+            // keep the refill after them, so each slot's register is reused
+            // instead of copied on the back edge.
+            asm volatile("" : "+v"(bp), "+v"(qp), "+v"(a0), "+v"(k0), "+v"(ha0), "+v"(hk0),
+                         "+v"(s0));
+            __builtin_amdgcn_sched_barrier(0);
+            sb = bload(rb, xo, row_off(y + kPf + 1));
+            sq = bload(rb, qo, row_off(y + kPf + 1));
+            sa = bload(ra, xo, row_off(y + kPf));
+            sk = bload(rk, xo, row_off(y + kPf));
+            sha = bload(ra, ho, row_off(y + kPf));
+            shk = bload(rk, ho, row_off(y + kPf));
+            if constexpr (EDGE) sm = bload(rm, mo, mrow(y + kPf));
+            // ---- first step, row y: block columns, then the halo columns
+            float c0 = 0.0f, hc = 0.0f;
+            if (row_in) {
+                const float bw = from_west(b0, q0), be = from_east(b0, q0);
+                c0 = __fmaf_rn(k0, lap5(bw, be, bm, bp, b0, p.idx2, p.idy2, p.cen),
+                               2.0f * b0 - a0);
+                if constexpr ((F & kPoint) != 0) {
+                    if (y == sr && x == sc) c0 += s1;
+                }
+                if constexpr (EDGE) {
+                    if (y >= 1 && y <= H - 2) c0 = mur_sides(c0, b0, x, W, s0);
+                    if (y == H - 1) c0 = mur_row(mur, W, H, x, kt_x, b0, bm, cm1, 2);
+                }
+                // halo: lane 0 has B(xs) as its east and B(xs-2) (lane 1) as
+                // its west neighbour; lane 63 has B(xs+63) as its west and
+                // B(xs+65) (lane 62) as its east neighbour
+                const float qe = from_east(q0, 0.0f), qw = from_west(q0, 0.0f);
+                const float hw = hl ? qe : b0;
+                const float he = hl ? b0 : qw;
+                hc = __fmaf_rn(hk0, lap5(hw, he, qm1, qp, q0, p.idx2, p.idy2, p.cen),
+                               2.0f * q0 - ha0);
+                if constexpr ((F & kPoint) != 0) {
+                    if (y == sr && xh == sc) hc += s1;
+                }
+                if constexpr (EDGE) {
+                    // right halo on column W-1: Mur from lane 63's own block
+                    // values at column W-2 (b0, c0); never the left halo
+                    if (h_right_edge) {
+                        if (y >= 1 && y <= H - 2) {
+                            hc = b0 + s0 * (c0 - q0);
+                        } else if (y == H - 1) {
+                            const float from_x = b0 + kx3 * (c0 - q0);
+                            const float from_y = qm1 + ky3 * (hcm1 - q0);
+                            hc = 0.5f * (from_x + from_y);
+                        }
+                    } else if (y == H - 1) {
+                        hc = qm1 + kt_h * (hcm1 - q0);
+                    }
+                    if (y == 1) {
+                        // row 0 (bottom edge) now that row 1 is final
+                        cm1 = mur_row(mur, W, H, x, kb_x, bm, b0, c0, 0);
+                        if (h_right_edge) {
+                            const float from_x = bm + kx1 * (cm1 - qm1);
+                            const float from_y = q0 + ky1 * (hc - qm1);
+                            hcm1 = 0.5f * (from_x + from_y);
+                        } else {
+                            hcm1 = q0 + kb_h * (hc - qm1);
+                        }
+                        if (owned & (r0 == 0)) {
+                            outc[xo / 4] = cm1;
+                            if constexpr ((F & kOut2) != 0)
+                                if (p.f_out2 != nullptr) (gstore_ptr(p.f_out2) + soff)[xo / 4] = cm1;
+                        }
+                    }
+                }
+                if (owned & (y >= r0) & (y < r1) & (y > 0)) {
+                    outc[(size_t)y * W + x] = c0;
+                    if constexpr ((F & kOut2) != 0)
+                        if (p.f_out2 != nullptr) (gstore_ptr(p.f_out2) + soff)[(size_t)y * W + x] = c0;
+                }
+            }
+            // ---- second step, row y - 1
+            const int r = y - 1;
+            float d = 0.0f;
+            if (r >= r0 && r < r1) {
+                const float cw = from_west(cm1, hcm1), ce = from_east(cm1, hcm1);
+                d = __fmaf_rn(km1, lap5(cw, ce, cm2, c0, cm1, p.idx2, p.idy2, p.cen),
+                              2.0f * cm1 - bm);
+                if constexpr ((F & kPoint) != 0) {
+                    if (r == sr && x == sc) d += s2;
+                }
+                if constexpr (EDGE) {
+                    if (r >= 1 && r <= H - 2) d = mur_sides(d, cm1, x, W, sm1);
+                    if (r == H - 1) d = mur_row(mur, W, H, x, kt_x, cm1, cm2, dm1, 2);
+                    if (r == 1) {
+                        const float d0 = mur_row(mur, W, H, x, kb_x, cm2, cm1, d, 0);
+                        if (owned) {
+                            outd[xo / 4] = d0;
+                            if constexpr ((F & kOut2) != 0)
+                                if (p.f_out2_b != nullptr) (gstore_ptr(p.f_out2_b) + soff)[xo / 4] = d0;
+                        }
+                    }
+                }
+                if (owned & (r > 0)) {
+                    outd[(size_t)r * W + x] = d;
+                    if constexpr ((F & kOut2) != 0)
+                        if (p.f_out2_b != nullptr) (gstore_ptr(p.f_out2_b) + soff)[(size_t)r * W + x] = d;
+                }
+            }
+            cm2 = cm1;
+            cm1 = c0;
+            hcm1 = hc;
+            bm = b0;
+            b0 = bp;
+            qm1 = q0;
+            q0 = qp;
+            km1 = k0;
+            dm1 = d;
+            sm1 = s0;
+        };
+        // rows in groups of kPf; rows past r1 store nothing (both steps'
+        // stores are bounded by r1) and read clamped rows
+        for (int y = y_start; y <= r1; y += kPf) {
+#pragma unroll
+            for (int i = 0; i < kPf; ++i)
+                row(y + i, qb[i], qq[i], qa[i], qk[i], qha[i], qhk[i], qm[i]);
+        }
+    }
+}
+
+template <unsigned F>
+__global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
+{
+    const int b = blockIdx.x;
+    if constexpr ((F & kGather) != 0) {
+        if (b >= p.n_stencil_blocks) {
+            gather_pairs(p, (b - p.n_stencil_blocks) * (int)blockDim.x + (int)threadIdx.x);
+            return;
+        }
+    }
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
+    const int item = wblk * kWavesPerBlock + wave;
+    if (item >= p.n_work) return;
+    // shot group fastest: the waves of a workgroup share coefficient rows
+    const int grp = item % p.n_groups;
+    const int t = item / p.n_groups;
+    const int cb = t % p.n_cb;
+    const int seg = t / p.n_cb;
+    const int xs = min(cb * kLanes, max(p.nx - kLanes, 0));
+    const int r0 = seg * p.seg;
+    const int r1 = (seg == p.n_seg - 1) ? p.ny : r0 + p.seg;
+    // Mur work: column 0 or W-1 among the block and halo columns, or the
+    // first / last row among the segment's rows and its halo rows
+    const bool edge = (xs <= 1) || (xs + kLanes >= p.nx - 1) || (r0 == 0) || (r1 >= p.ny - 1);
+    if (edge)
+        fwd2_wave<true, F>(p, lane, grp, cb, seg);
+    else
+        fwd2_wave<false, F>(p, lane, grp, cb, seg);
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
@@ -903,6 +1243,55 @@ int launch_step(StepParams &p, bool adjoint, hipStream_t stream, int gather_pair
     return SWFD_OK;
 }
 
+// Two forward steps in one launch: outputs p.f_out (first step) and
+// p.f_out_b (second); p.f_out2 / p.f_out2_b optional copies. Work items are
+// (shot group, 62-column block, row segment) marches.
+int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
+{
+    static const int seg_rows = std::max(2, env_int("SWFD_SEG_ROWS", kSegRows));
+    static const int waves_per_cu = env_int("SWFD_TB_WAVES_PER_CU", 256);
+    static const int spg_override = env_int("SWFD_TB_SHOTS_PER_GROUP", 0);
+    p.n_cb = (p.nx + kLanes - 1) / kLanes;
+    p.seg = seg_rows;
+    p.n_seg = std::max(1, p.ny / seg_rows);
+    const long cells = (long)p.n_cb * p.n_seg;
+    const long target = (long)waves_per_cu * cu_count();
+    long groups = (target + cells - 1) / cells;
+    if (spg_override > 0) groups = (p.n_shots + spg_override - 1) / spg_override;
+    groups = std::max<long>(1, std::min<long>(groups, p.n_shots));
+    p.shots_per_group = (int)((p.n_shots + groups - 1) / groups);
+    p.n_groups = (p.n_shots + p.shots_per_group - 1) / p.shots_per_group;
+    p.n_work = (int)(cells * p.n_groups);
+    const int blocks = (p.n_work + kWavesPerBlock - 1) / kWavesPerBlock;
+    p.chunk = (blocks + kXcds - 1) / kXcds;
+    p.n_stencil_blocks = p.chunk * kXcds;
+    const int gather_blocks = gather_pairs_count > 0 ? (gather_pairs_count + 255) / 256 : 0;
+    const dim3 block(kLanes * kWavesPerBlock);
+    const dim3 grid(p.n_stencil_blocks + gather_blocks);
+    unsigned f = kPoint;
+    if (p.f_out2 != nullptr || p.f_out2_b != nullptr) f |= kOut2;
+    if (gather_blocks > 0) f |= kGather;
+    if (p.src_rows == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
+        return fail(SWFD_EINVAL, "two-step launch needs point sources and two outputs");
+    switch (f) {
+        case kPoint:
+            hipLaunchKernelGGL((fwd2_kernel<kPoint>), grid, block, 0, stream, p);
+            break;
+        case kPoint | kGather:
+            hipLaunchKernelGGL((fwd2_kernel<kPoint | kGather>), grid, block, 0, stream, p);
+            break;
+        case kPoint | kOut2:
+            hipLaunchKernelGGL((fwd2_kernel<kPoint | kOut2>), grid, block, 0, stream, p);
+            break;
+        default:
+            hipLaunchKernelGGL((fwd2_kernel<kPoint | kOut2 | kGather>), grid, block, 0, stream,
+                               p);
+            break;
+    }
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
 // Number of imaging partial slices a step of this shape uses.
 int image_groups(int ny, int nx, int n_shots)
 {
@@ -968,7 +1357,7 @@ int copy_field(float *dst, const float *src, size_t n, hipStream_t stream)
 // Move the state (p_old, p_cur) into (dst_old, dst_cur) using `spare` as a
 // temporary when the two destinations hold each other's data.
 int settle_state(const float *p_old, const float *p_cur, float *dst_old, float *dst_cur,
-                 float *const work[3], size_t n, hipStream_t stream)
+                 float *const work[4], size_t n, hipStream_t stream)
 {
     int rc;
     if (dst_old == nullptr && dst_cur == nullptr) return SWFD_OK;
@@ -991,10 +1380,12 @@ int settle_state(const float *p_old, const float *p_cur, float *dst_old, float *
     return SWFD_OK;
 }
 
-float *free_work(float *const work[3], const float *a, const float *b)
+// A work buffer that is none of a, b, c (work[3] may be NULL).
+float *free_work(float *const work[4], const float *a, const float *b,
+                 const float *c = nullptr)
 {
-    for (int i = 0; i < 3; ++i)
-        if (work[i] != a && work[i] != b) return work[i];
+    for (int i = 0; i < 4; ++i)
+        if (work[i] != nullptr && work[i] != a && work[i] != b && work[i] != c) return work[i];
     return nullptr;
 }
 
@@ -1155,52 +1546,95 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
     hipStream_t st = as_stream(stream);
     const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
     const size_t sr = (size_t)a->n_shots * (a->samples != nullptr ? rec->n : 0);
-    float *const work[3] = {a->work0, a->work1, a->work2};
-    const float *p_old = a->init_old;
-    const float *p_cur = a->init_u;
-    for (int k = 0; k < a->n_steps; ++k) {
-        const int n = a->step_begin + k;
-        float *out = nullptr;
+    float *const work[4] = {a->work0, a->work1, a->work2, a->work3};
+    static const bool two_step_env = env_int("SWFD_TWO_STEP", 1) != 0;
+    const bool two_step = two_step_env && a->work3 != nullptr;
+    // output of step n: its history slot, else a work buffer chosen later
+    auto hist_out = [&](int n) -> float * {
         if (a->hist != nullptr && n >= a->hist_first && (n - a->hist_first) % a->hist_every == 0) {
             const int slot = (n - a->hist_first) / a->hist_every;
-            if (slot < a->hist_slots) out = a->hist + (size_t)slot * field;
+            if (slot < a->hist_slots) return a->hist + (size_t)slot * field;
         }
-        if (out == nullptr) out = free_work(work, p_old, p_cur);
-        float *out2 = nullptr;
-        if (a->ckpt != nullptr) {
-            const int K = a->ckpt_every;
-            const int j2 = n + 2, j1 = n + 1;
-            int slot = -1;
-            if (j2 % K == 0 && j2 / K >= 1) slot = 2 * (j2 / K - 1);
-            else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
-            if (slot >= 0 && slot < a->ckpt_slots) out2 = a->ckpt + (size_t)slot * field;
+        return nullptr;
+    };
+    // checkpoint copy of the output of step n: the state before step jK is
+    // (output of jK-2, output of jK-1)
+    auto ckpt_out = [&](int n) -> float * {
+        if (a->ckpt == nullptr) return nullptr;
+        const int K = a->ckpt_every;
+        const int j2 = n + 2, j1 = n + 1;
+        int slot = -1;
+        if (j2 % K == 0 && j2 / K >= 1) slot = 2 * (j2 / K - 1);
+        else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
+        return (slot >= 0 && slot < a->ckpt_slots) ? a->ckpt + (size_t)slot * field : nullptr;
+    };
+    // fields whose receiver samples are still to be taken (the last one or two
+    // outputs); they are gathered by the next launch, which reads them anyway
+    const float *pend_field[2] = {nullptr, nullptr};
+    int pend_step[2] = {0, 0};
+    int n_pend = 0;
+    auto attach_gathers = [&](StepParams &p) -> int {
+        if (a->samples == nullptr || n_pend == 0) return 0;
+        p.gat_field = pend_field[0];
+        p.gat_out = a->samples + (size_t)pend_step[0] * sr;
+        if (n_pend == 2) {
+            p.gat_field2 = pend_field[1];
+            p.gat_out2 = a->samples + (size_t)pend_step[1] * sr;
         }
+        return n_pend * a->n_shots * rec->n;
+    };
+    const float *p_old = a->init_old;
+    const float *p_cur = a->init_u;
+    int k = 0;
+    while (k < a->n_steps) {
+        const int n = a->step_begin + k;
         StepParams p;
         base_params(p, grid, medium, a->n_shots);
         attach_receivers(p, a->samples != nullptr ? rec : nullptr);
         p.f_old = p_old;
         p.f_cur = p_cur;
-        p.f_out = out;
-        p.f_out2 = out2;
         p.src_rows = a->src_rows;
         p.src_cols = a->src_cols;
         p.src_values = a->src_values;
         p.src_index = n;
-        int gather_count = 0;
-        if (a->samples != nullptr && k > 0) {
-            p.gat_field = p_cur;
-            p.gat_out = a->samples + (size_t)(n - 1) * sr;
-            gather_count = a->n_shots * rec->n;
+        const int gather_count = attach_gathers(p);
+        if (two_step && k + 1 < a->n_steps) {
+            float *out_c = hist_out(n);
+            if (out_c == nullptr) out_c = free_work(work, p_old, p_cur);
+            float *out_d = hist_out(n + 1);
+            if (out_d == nullptr) out_d = free_work(work, p_old, p_cur, out_c);
+            p.f_out = out_c;
+            p.f_out_b = out_d;
+            p.f_out2 = ckpt_out(n);
+            p.f_out2_b = ckpt_out(n + 1);
+            if ((rc = launch_pair(p, st, gather_count))) return rc;
+            pend_field[0] = out_c;
+            pend_step[0] = n;
+            pend_field[1] = out_d;
+            pend_step[1] = n + 1;
+            n_pend = 2;
+            p_old = out_c;
+            p_cur = out_d;
+            k += 2;
+        } else {
+            float *out = hist_out(n);
+            if (out == nullptr) out = free_work(work, p_old, p_cur);
+            p.f_out = out;
+            p.f_out2 = ckpt_out(n);
+            if ((rc = launch_step(p, false, st, gather_count))) return rc;
+            pend_field[0] = out;
+            pend_step[0] = n;
+            n_pend = 1;
+            p_old = p_cur;
+            p_cur = out;
+            k += 1;
         }
-        if ((rc = launch_step(p, false, st, gather_count))) return rc;
-        p_old = p_cur;
-        p_cur = out;
     }
-    if (a->samples != nullptr && a->n_steps > 0) {
-        const int last = a->step_begin + a->n_steps - 1;
-        if ((rc = swfd_gather(grid, rec, a->n_shots, p_cur, a->samples + (size_t)last * sr,
-                              stream)))
-            return rc;
+    if (a->samples != nullptr) {
+        for (int i = 0; i < n_pend; ++i)
+            if ((rc = swfd_gather(grid, rec, a->n_shots, pend_field[i],
+                                  a->samples + (size_t)pend_step[i] * sr, stream)))
+                return rc;
     }
     return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
 }
@@ -1228,7 +1662,7 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
     const int n_part = image_groups(grid->ny, grid->nx, a->n_shots);
     bool partials_used = false;
     const size_t sr = (size_t)a->n_shots * rec->n;
-    float *const work[3] = {a->work0, a->work1, a->work2};
+    float *const work[4] = {a->work0, a->work1, a->work2, a->work3};
     const float *p_old = a->init_old;
     const float *p_cur = a->init_u;
     // Step m images lam' (after step m) against the forward field after step

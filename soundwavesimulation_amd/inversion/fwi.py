@@ -53,14 +53,14 @@ def gradient_bytes_per_shot(n_steps, grid_shape, n_receivers, store_every=1, wav
     """Device memory one gradient shot needs.
 
     ``store_every == 1`` with full-precision history: checkpointed plan,
-    ``(6 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
+    ``(7 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
     Otherwise the reference's stored-history estimate (invert_ring.py:463-473).
     """
     cells = grid_shape[0] * grid_shape[1]
     if int(store_every) == 1 and wave_dtype in (None, _F32):
         K = checkpoint_every or default_checkpoint_every(n_steps)
         _, n_ckpt, n_hist = segment_plan(n_steps, K)
-        return (6 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
+        return (7 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
     wave_bytes = torch.empty((), dtype=wave_dtype or _F32).element_size()
     return (stored_steps(n_steps, store_every) * cells * wave_bytes + 12 * cells * 4
             + 4 * n_steps * n_receivers * 4)
@@ -118,7 +118,7 @@ class CheckpointedGradient:
         self.n_seg, n_ckpt, n_hist = segment_plan(self.n_steps, self.K)
         dev = solver.model.device
         shape = (self.n_shots, *solver.model.shape)
-        self.work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
+        self.work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
         self.lam = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
         self.ckpt = (torch.empty((max(n_ckpt, 2), *shape), device=dev, dtype=_F32)
                      if n_ckpt > 0 else None)

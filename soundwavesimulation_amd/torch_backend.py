@@ -463,13 +463,14 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
 
     ``init`` is the state before ``step_begin`` (zeros if None); ``samples``
     (N, S, R), ``hist`` (slots, S, ny, nx) and ``ckpt`` (slots, S, ny, nx) are
-    optional outputs (see ``swfd_forward_args``).
+    optional outputs (see ``swfd_forward_args``). ``work`` holds three or four
+    scratch fields; with four the run advances two steps per launch.
     """
     model = solver.model
     dev = model.device
     shape = (n_shots, *model.shape)
     if work is None:
-        work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
+        work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
     if init is None:
         work[0].zero_()
         work[1].zero_()
@@ -479,6 +480,7 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
         n_shots=n_shots, step_begin=step_begin, n_steps=n_steps,
         init_old=_native.ptr(init[0]), init_u=_native.ptr(init[1]),
         work0=_native.ptr(work[0]), work1=_native.ptr(work[1]), work2=_native.ptr(work[2]),
+        work3=_native.ptr(work[3]) if len(work) > 3 else None,
         final_old=_native.ptr(work[0]) if final else None,
         final_u=_native.ptr(work[1]) if final else None,
         src_rows=_native.ptr(rows_i32), src_cols=_native.ptr(cols_i32),
@@ -522,6 +524,7 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
         n_shots=n_shots, step_hi=step_hi, step_lo=step_lo,
         init_old=_native.ptr(state[0]), init_u=_native.ptr(state[1]),
         work0=_native.ptr(work[0]), work1=_native.ptr(work[1]), work2=_native.ptr(work[2]),
+        work3=_native.ptr(work[3]) if len(work) > 3 else None,
         final_old=_native.ptr(state[0]), final_u=_native.ptr(state[1]),
         residual=_native.ptr(residual_nsr), hist=_native.ptr(hist), hist_first=hist_first,
         hist_every=hist_every, hist_slots=0 if hist is None else hist.shape[0],

@@ -1,0 +1,117 @@
+"""Two-step forward launches against single-step launches, bit for bit.
+
+``swfd_forward`` advances two steps per launch when it is given a fourth work
+buffer. The two-step kernel evaluates the same float32 expressions in the same
+order as the single-step kernel (stencil, point source, Mur edges and corners),
+so samples, history, checkpoints and the final state must be identical
+(``torch.equal``), not merely close. The shapes exercise column blocks that
+end on the right boundary (nx = 63, 125), row segments with a merged tail,
+grids smaller than one segment, sources on and next to the boundary, odd step
+counts (a trailing single step) and a non-zero first step.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from soundwavesimulation_amd import RingArray, axis_centers, stable_dt
+from soundwavesimulation_amd.torch_backend import (TorchFDTD2D, TorchRingSampler,
+                                                   TorchScalarWave2D, forward_run)
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda", 0)
+F32 = torch.float32
+
+
+def _medium(ny, nx, seed, spacing=1e-3):
+    rng = np.random.default_rng(seed)
+    c = 1450.0 + 150.0 * rng.random((ny, nx))
+    dt = stable_dt(float(c.max()), spacing, spacing, cfl=0.3)
+    solver = TorchFDTD2D(TorchScalarWave2D(c, spacing, device=DEV), dt)
+    return solver, spacing
+
+
+def _run(solver, sampler, rows, cols, values, n_steps, n_work, *, init=None, step_begin=0,
+         hist_slots=0, hist_first=0, hist_every=1, ckpt_slots=0, ckpt_every=0):
+    S = rows.numel()
+    shape = (S, *solver.model.shape)
+    work = [torch.full(shape, float("nan"), device=DEV, dtype=F32) for _ in range(n_work)]
+    samples = None
+    if sampler is not None:
+        samples = torch.full((step_begin + n_steps, S, sampler.n_elements), float("nan"),
+                             device=DEV, dtype=F32)
+    hist = (torch.zeros((hist_slots, *shape), device=DEV, dtype=F32) if hist_slots else None)
+    ckpt = (torch.zeros((ckpt_slots, *shape), device=DEV, dtype=F32) if ckpt_slots else None)
+    final = forward_run(solver, S, rows, cols, values, n_steps, init=init, samples=samples,
+                        sampler=sampler, hist=hist, hist_first=hist_first,
+                        hist_every=hist_every, ckpt=ckpt, ckpt_every=ckpt_every,
+                        step_begin=step_begin, work=work, final=True)
+    torch.cuda.synchronize()
+    out = {"old": final[0].clone(), "cur": final[1].clone()}
+    if samples is not None:
+        out["samples"] = samples[step_begin:].clone()
+    if hist is not None:
+        out["hist"] = hist
+    if ckpt is not None:
+        out["ckpt"] = ckpt
+    return out
+
+
+def _assert_identical(a, b, label):
+    for key in a:
+        assert torch.equal(a[key], b[key]), (
+            f"{label}: {key} differs, max |diff| "
+            f"{float((a[key] - b[key]).abs().max()):.3e}")
+
+
+def _pulse(n, seed):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randn(n, generator=g, dtype=F32).to(DEV)
+
+
+@pytest.mark.parametrize("ny,nx,n_steps", [
+    (64, 64, 41),     # one row segment, odd step count
+    (200, 63, 30),    # column W-1 on the last lane of the first block; merged tail segment
+    (257, 125, 24),   # column W-1 on the last lane of the second block
+    (130, 190, 19),   # boundary-adjacent sources, three column blocks
+    (5, 7, 9),        # smaller than one segment and one column block
+])
+def test_two_step_forward_is_bitwise_single_step(ny, nx, n_steps):
+    print(f"two-step forward {ny}x{nx}, {n_steps} steps", flush=True)
+    solver, sp = _medium(ny, nx, seed=ny * 1000 + nx)
+    rows = [1, ny - 2, ny // 2, 0, ny - 1, 1]
+    cols = [1, nx - 2, nx // 2, nx // 3, 2, nx - 1]
+    rows_t = torch.tensor(rows, dtype=torch.int32, device=DEV)
+    cols_t = torch.tensor(cols, dtype=torch.int32, device=DEV)
+    values = _pulse(n_steps, seed=n_steps)
+    sampler = None
+    if min(ny, nx) >= 40:
+        x, y = axis_centers(nx, sp), axis_centers(ny, sp)
+        radius = 0.4 * min(ny, nx) * sp
+        sampler = TorchRingSampler(RingArray(12, radius, x, y), device=DEV)
+    one = _run(solver, sampler, rows_t, cols_t, values, n_steps, 3)
+    two = _run(solver, sampler, rows_t, cols_t, values, n_steps, 4)
+    assert bool(torch.isfinite(two["cur"]).all())
+    assert float(two["cur"].abs().max()) > 0.0
+    _assert_identical(one, two, f"{ny}x{nx}")
+
+
+def test_two_step_history_checkpoints_and_offset_start():
+    ny, nx, n_steps, begin = 150, 140, 23, 4
+    print(f"two-step forward with history/checkpoints {ny}x{nx}", flush=True)
+    solver, sp = _medium(ny, nx, seed=7)
+    x, y = axis_centers(nx, sp), axis_centers(ny, sp)
+    array = RingArray(8, 0.4 * min(nx, ny) * sp, x, y)
+    sampler = TorchRingSampler(array, device=DEV)
+    rc = [array.inject_rows_cols(i) for i in range(3)]
+    rows_t = torch.tensor([r for r, _ in rc], dtype=torch.int32, device=DEV)
+    cols_t = torch.tensor([c for _, c in rc], dtype=torch.int32, device=DEV)
+    values = _pulse(begin + n_steps, seed=3)
+    g = torch.Generator().manual_seed(11)
+    init = tuple(torch.randn((3, ny, nx), generator=g, dtype=F32).to(DEV) for _ in range(2))
+    kw = dict(init=init, step_begin=begin, hist_slots=6, hist_first=begin + 3, hist_every=3,
+              ckpt_slots=8, ckpt_every=5)
+    one = _run(solver, sampler, rows_t, cols_t, values, n_steps, 3, **kw)
+    two = _run(solver, sampler, rows_t, cols_t, values, n_steps, 4, **kw)
+    assert float(two["hist"].abs().sum()) > 0.0 and float(two["ckpt"].abs().sum()) > 0.0
+    _assert_identical(one, two, "history/checkpoints")
