@@ -10,8 +10,9 @@ resident in HBM before the timed region.
 
 Printed JSON (one line): Gcell/s = 2 * ny * nx * steps * shots / seconds per
 gradient, plus
-  roofline      -- the adjoint step kernel: 20 B per cell-update (read lam,
-                   lam_old, c^2, forward field; write lam_next) over its
+  roofline      -- the adjoint kernel (two steps per launch): 20 B per
+                   cell-update (read lam, lam_old, c^2, forward field; write
+                   lam_next) times the cell-updates of one launch, over its
                    average launch time, timed with events on torch's stream;
   cpu_baseline  -- the float64 C oracle (OpenMP, all host cores) on a
                    bounded sample of the same workload;
@@ -113,46 +114,53 @@ def build_problem(a, dev):
     return problem, m0, c_true, dt, h
 
 
-def time_adjoint_kernel(problem, n_launch=20):
-    """Average launch time of the fused adjoint step kernel (residual + imaging)."""
+def time_adjoint_kernel(problem, n_steps=20):
+    """Average launch time of the fused adjoint kernel (residual + imaging).
+
+    The run advances two adjoint steps per launch, so ``n_steps`` (even) steps
+    are ``n_steps / 2`` launches; the fill and reduction of the partial images
+    around them are included (two small kernels, well under 1 % here).
+    """
     from soundwavesimulation_amd.torch_backend import adjoint_run
 
     S = problem.parallel_shots
     engine = problem._engine(S)
-    n_launch = min(n_launch, engine.K)  # imaging reads history slots 2..n_launch
+    n_steps = min(n_steps, engine.K) // 2 * 2  # imaging reads history slots up to n_steps
+    n_launch = n_steps // 2 if len(engine.lam) > 3 else n_steps
     dev = problem.model.device
     R = problem.sampler.n_elements
-    residual = torch.randn((n_launch, S, R), device=dev) * 1e-3
+    residual = torch.randn((n_steps, S, R), device=dev) * 1e-3
     grad = torch.zeros(problem.model.shape, device=dev)
     for t in engine.lam[:2]:
         t.zero_()
     kw = dict(state=(engine.lam[0], engine.lam[1]), work=engine.lam, hist=engine.hist,
               hist_first=-2, hist_every=1, image_scale=problem.solver.dt2, grad=grad)
-    adjoint_run(problem.solver, problem.sampler, S, residual, n_launch, 0, **kw)  # warm
+    adjoint_run(problem.solver, problem.sampler, S, residual, n_steps, 0, **kw)  # warm
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
-    adjoint_run(problem.solver, problem.sampler, S, residual, n_launch, 0, **kw)
+    adjoint_run(problem.solver, problem.sampler, S, residual, n_steps, 0, **kw)
     stop.record()
     torch.cuda.synchronize()
-    return start.elapsed_time(stop) / 1e3 / n_launch, S
+    return start.elapsed_time(stop) / 1e3 / n_launch, S, n_steps // n_launch
 
 
-def time_forward_kernel(problem, n_launch=20):
+def time_forward_kernel(problem, n_steps=20):
+    """Average time per forward step (two steps per launch with four work fields)."""
     from soundwavesimulation_amd.torch_backend import forward_run
 
     S = problem.parallel_shots
     engine = problem._engine(S)
     rows, cols = problem._rows_cols(np.arange(S))
     kw = dict(work=engine.work, final=False)
-    forward_run(problem.solver, S, rows, cols, problem.source_values, n_launch, **kw)
+    forward_run(problem.solver, S, rows, cols, problem.source_values, n_steps, **kw)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
-    forward_run(problem.solver, S, rows, cols, problem.source_values, n_launch, **kw)
+    forward_run(problem.solver, S, rows, cols, problem.source_values, n_steps, **kw)
     stop.record()
     torch.cuda.synchronize()
-    return start.elapsed_time(stop) / 1e3 / n_launch, S
+    return start.elapsed_time(stop) / 1e3 / n_steps, S
 
 
 def cpu_baseline(a, c_true, dt, h):
@@ -308,26 +316,31 @@ def main():
     cells = 2.0 * n * n * a.nsteps * a.shots
     gcells = cells / sec / 1e9
 
-    t_adj, S = time_adjoint_kernel(problem)
+    t_adj, S, steps_per_launch = time_adjoint_kernel(problem)
     t_fwd, _ = time_forward_kernel(problem)
     units = S * n * n
-    adj_bytes = 20.0 * units
+    # 20 B per adjoint cell-update (read lam, lam_old, c^2, forward field;
+    # write lam_next), steps_per_launch steps per launch
+    adj_bytes = 20.0 * units * steps_per_launch
     achieved = adj_bytes / t_adj / 1e9
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "traffic_adjoint_step_1024x1024.json")
-    if os.path.exists(tpath) and n == 1024:
+    tpath = os.path.join(HERE, "profiles", "traffic_adjoint_pair_1024x1024.json")
+    if os.path.exists(tpath) and n == 1024 and steps_per_launch == 2:
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch_per_shot")
         if traffic is not None:
             traffic = traffic * S
+    kernel = ("adj2_kernel<residual+imaging> (two adjoint steps per launch)"
+              if steps_per_launch == 2 else "step_kernel<adjoint, residual+imaging>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "step_kernel<adjoint, residual+imaging>",
-                "launch_ms": round(t_adj * 1e3, 4), "shots_per_launch": S,
-                "forward_launch_ms": round(t_fwd * 1e3, 4),
+                "kernel": kernel, "launch_ms": round(t_adj * 1e3, 4),
+                "steps_per_launch": steps_per_launch, "shots_per_launch": S,
+                "forward_step_ms": round(t_fwd * 1e3, 4),
                 "forward_gbs_16B": round(16.0 * units / t_fwd / 1e9, 1)}
-    print(f"# adjoint kernel {t_adj*1e3:.3f} ms/launch ({achieved:.0f} GB/s at 20 B/cell); "
-          f"forward {t_fwd*1e3:.3f} ms/launch", file=sys.stderr, flush=True)
+    print(f"# adjoint kernel {t_adj*1e3:.3f} ms/launch of {steps_per_launch} steps "
+          f"({achieved:.0f} GB/s at 20 B/cell-update); forward {t_fwd*1e3:.3f} ms/step",
+          file=sys.stderr, flush=True)
 
     extra = {}
     if not a.no_torch_reference:

@@ -98,6 +98,10 @@ struct StepParams {
     const float *gat_field2;
     float *gat_out2;
     int n_cb, n_seg, seg;
+    // two adjoint steps per launch: residual and imaging field of the second
+    // step, shots per wavefront, rows of the imaging accumulator
+    const float *resid_b, *img_b;
+    int shots_per_wave, acc_rows;
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -583,7 +587,16 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
 // W-1 (it may overlap the previous block and stores only its own columns), so
 // column W-1 is never on lane 0 and its Mur update finds its west neighbour.
 
-constexpr int kSegRows = 64;         // rows a two-step wavefront marches
+constexpr int kSegRows = 64;         // rows a two-step wavefront marches (at least)
+
+// Rows [r0, r1) of segment `seg`: n_seg near-equal parts of H, each of at
+// least p.seg >= 2 rows (the top row's Mur update in the second step reads
+// the row below it, so a segment must hold both).
+__device__ __forceinline__ void seg_rows(const StepParams &p, int seg, int &r0, int &r1)
+{
+    r0 = (int)(((long)seg * p.ny) / p.n_seg);
+    r1 = (int)(((long)(seg + 1) * p.ny) / p.n_seg);
+}
 #ifndef SWFD_PREFETCH_ROWS
 #define SWFD_PREFETCH_ROWS 2
 #endif
@@ -646,8 +659,8 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
     const bool xq_in = (xq >= 0) & (xq < W);
     const int ho = min(max(xh, 0), W - 1) * 4;
     const int qo = min(max(xq, 0), W - 1) * 4;
-    const int r0 = seg * p.seg;
-    const int r1 = (seg == p.n_seg - 1) ? H : r0 + p.seg;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
     const int y_start = r0 == 0 ? 0 : r0 - 1;
     const size_t plane = (size_t)H * W;
     const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
@@ -740,13 +753,16 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
             asm volatile("" : "+v"(bp), "+v"(qp), "+v"(a0), "+v"(k0), "+v"(ha0), "+v"(hk0),
                          "+v"(s0));
             __builtin_amdgcn_sched_barrier(0);
-            sb = bload(rb, xo, row_off(y + kPf + 1));
-            sq = bload(rb, qo, row_off(y + kPf + 1));
-            sa = bload(ra, xo, row_off(y + kPf));
-            sk = bload(rk, xo, row_off(y + kPf));
-            sha = bload(ra, ho, row_off(y + kPf));
-            shk = bload(rk, ho, row_off(y + kPf));
-            if constexpr (EDGE) sm = bload(rm, mo, mrow(y + kPf));
+            // rows past r1 are not needed: reload row r1 instead (a cache hit;
+            // a conditional refill would make every wait a full one)
+            const int yf = min(y + kPf, r1);
+            sb = bload(rb, xo, row_off(yf + 1));
+            sq = bload(rb, qo, row_off(yf + 1));
+            sa = bload(ra, xo, row_off(yf));
+            sk = bload(rk, xo, row_off(yf));
+            sha = bload(ra, ho, row_off(yf));
+            shk = bload(rk, ho, row_off(yf));
+            if constexpr (EDGE) sm = bload(rm, mo, mrow(yf));
             // ---- first step, row y: block columns, then the halo columns
             float c0 = 0.0f, hc = 0.0f;
             if (row_in) {
@@ -878,8 +894,8 @@ __global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
     const int cb = t % p.n_cb;
     const int seg = t / p.n_cb;
     const int xs = min(cb * kLanes, max(p.nx - kLanes, 0));
-    const int r0 = seg * p.seg;
-    const int r1 = (seg == p.n_seg - 1) ? p.ny : r0 + p.seg;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
     // Mur work: column 0 or W-1 among the block and halo columns, or the
     // first / last row among the segment's rows and its halo rows
     const bool edge = (xs <= 1) || (xs + kLanes >= p.nx - 1) || (r0 == 0) || (r1 >= p.ny - 1);
@@ -887,6 +903,378 @@ __global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
         fwd2_wave<true, F>(p, lane, grp, cb, seg);
     else
         fwd2_wave<false, F>(p, lane, grp, cb, seg);
+}
+
+// ---------------------------------------------------------------------------
+// Two adjoint steps per launch (temporal blocking)
+// ---------------------------------------------------------------------------
+//
+// Steps n (first) and n-1 (second) of the reverse run, from A = lam before
+// step n (older) and B = lam after it... that is, the state (A, B) of the
+// reverse run: C = 2B - A + dt^2 Lap(c^2 B) + residual_n, D = 2C - B +
+// dt^2 Lap(c^2 C) + residual_{n-1}, each followed by the forward Mur update.
+// Same 64-column marches as the two-step forward kernel; Lap(c^2 .) works on
+// products c^2 * lam kept per row, so the halo columns also need c^2 beside
+// them (loaded by lanes 1 / 62 like B). Bilinear residual injection goes
+// through the scatter table in the single-step kernel's order, on the block
+// and halo columns. Per cell and step the arithmetic equals the single-step
+// kernel's, so C and D are bit-identical to two single steps.
+//
+// Imaging: C is imaged against F1 (forward field after step n-1) and D
+// against F2 (after step n-2), as g += scale sum_s (C Lap F1 + D Lap F2).
+// The four wavefronts of a workgroup take disjoint shots of one shot group at
+// the same rows and columns; each adds its shots' terms per cell into its own
+// LDS rows, and at the end the workgroup adds the four in a fixed order into
+// the group's partial image (at most kMaxImageGroups), so the gradient is
+// deterministic. A cell's C term is added one row after C is final and its D
+// term one row after that (row 0 of each step is finalised a row late).
+
+template <bool EDGE, bool IMG>
+__device__ __forceinline__ void adj2_wave(const StepParams &p, int lane, int wave, int grp,
+                                          int cb, int seg, float *acc)
+{
+    const int W = p.nx, H = p.ny;
+    const int own0 = cb * kLanes;
+    const int xs = min(own0, max(W - kLanes, 0));
+    const int x = xs + lane;
+    const bool xin = x < W;
+    const bool owned = xin & (x >= own0);
+    const int xo = min(x, W - 1) * 4;
+    const bool hl = lane == 0, hr = lane == kLanes - 1;
+    const int xh = hl ? xs - 1 : (hr ? xs + kLanes : x);
+    const int xq = lane == 1 ? xs - 2 : (lane == kLanes - 2 ? xs + kLanes + 1 : xh);
+    const bool xh_in = (xh >= 0) & (xh < W);
+    const bool xq_in = (xq >= 0) & (xq < W);
+    const int ho = min(max(xh, 0), W - 1) * 4;
+    const int qo = min(max(xq, 0), W - 1) * 4;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_start = r0 == 0 ? 0 : r0 - 1;
+    const int y_end = r1 + 1;  // the D term of row r1 - 1 is imaged at row r1 + 1
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * W * 4; };
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
+    const __amdgpu_buffer_rsrc_t re = make_rsrc((const float *)p.cell_map, plane_bytes);
+    const float *mur = p.mur;
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
+    const int mo = (x == 0) ? 0 : H * 4;
+    auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
+    const float dt2 = p.dt2;
+    float kb_x = 0.0f, kt_x = 0.0f, kb_h = 0.0f, kt_h = 0.0f;
+    float kx1 = 0.0f, ky1 = 0.0f, kx3 = 0.0f, ky3 = 0.0f;
+    const bool h_right_edge = hr & (xh == W - 1);
+    if constexpr (EDGE) {
+        kb_x = mur[2 * H + xo / 4];
+        kt_x = mur[2 * H + W + xo / 4];
+        kb_h = mur[2 * H + ho / 4];
+        kt_h = mur[2 * H + W + ho / 4];
+        const float *kx = mur + 2 * H + 2 * W;
+        kx1 = kx[1];
+        ky1 = kx[4 + 1];
+        kx3 = kx[3];
+        ky3 = kx[4 + 3];
+    }
+    // does any block or halo cell of these rows receive a residual?
+    bool inject;
+    {
+        bool mine = false;
+        for (int yy = y_start; yy <= r1 && yy < H; ++yy) {
+            const int e = __float_as_int(bload(re, xo, row_off(yy)));
+            const int eh = __float_as_int(bload(re, ho, row_off(yy)));
+            mine |= (xin & (e >= 0)) | (xh_in & (eh >= 0));
+        }
+        inject = __any(mine);
+    }
+    const int n_rows = r1 - r0;
+    if constexpr (IMG) {
+        for (int i = 0; i < n_rows; ++i) acc[i * kLanes + lane] = 0.0f;
+    }
+    const float *img1 = IMG ? p.img : nullptr;
+    const float *img2 = IMG ? p.img_b : nullptr;
+    const int s_begin = grp * p.shots_per_group + wave * p.shots_per_wave;
+    const int s_end = min(min(p.n_shots, s_begin + p.shots_per_wave),
+                          (grp + 1) * p.shots_per_group);
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rf1 = make_rsrc(img1 != nullptr ? img1 + soff : p.f_cur,
+                                                     plane_bytes);
+        const __amdgpu_buffer_rsrc_t rf2 = make_rsrc(img2 != nullptr ? img2 + soff : p.f_cur,
+                                                     plane_bytes);
+        const bool use1 = img1 != nullptr, use2 = img2 != nullptr;
+        gf *outc = gstore_ptr(p.f_out) + soff;
+        gf *outd = gstore_ptr(p.f_out_b) + soff;
+        const float *res1 = p.resid + (size_t)s * p.n_rec;
+        const float *res2 = p.resid_b + (size_t)s * p.n_rec;
+        // residual injected at a cell: sum over its scatter entries, in order
+        auto injected = [&](const float *res, int e) -> float {
+            float a = 0.0f;
+            for (int k = p.entry_start[e]; k < p.entry_start[e + 1]; ++k)
+                a += res[p.pair_rec[k]] * p.pair_w[k];
+            return a;
+        };
+        float bm, b0, qm1, q0, km, k0, kqm1, kq0;
+        float f1m2 = 0.0f, f1m1 = 0.0f, f1hm1 = 0.0f, f2m3 = 0.0f, f2m2 = 0.0f, f2hm2 = 0.0f;
+        {
+            const float t0 = bload(rb, xo, row_off(y_start - 1));
+            const float t1 = bload(rb, xo, row_off(y_start));
+            const float t2 = bload(rb, qo, row_off(y_start - 1));
+            const float t3 = bload(rb, qo, row_off(y_start));
+            const float u0 = bload(rk, xo, row_off(y_start - 1));
+            const float u1 = bload(rk, xo, row_off(y_start));
+            const float u2 = bload(rk, qo, row_off(y_start - 1));
+            const float u3 = bload(rk, qo, row_off(y_start));
+            const bool up = y_start >= 1;
+            bm = (xin & up) ? t0 : 0.0f;
+            b0 = xin ? t1 : 0.0f;
+            qm1 = (xq_in & up) ? t2 : 0.0f;
+            q0 = xq_in ? t3 : 0.0f;
+            km = (xin & up) ? u0 : 0.0f;
+            k0 = xin ? u1 : 0.0f;
+            kqm1 = (xq_in & up) ? u2 : 0.0f;
+            kq0 = xq_in ? u3 : 0.0f;
+            if constexpr (IMG) {
+                const float v0 = bload(rf1, xo, row_off(y_start - 2));
+                const float v1 = bload(rf1, xo, row_off(y_start - 1));
+                const float v2 = bload(rf1, ho, row_off(y_start - 1));
+                const float w0 = bload(rf2, xo, row_off(y_start - 3));
+                const float w1 = bload(rf2, xo, row_off(y_start - 2));
+                const float w2 = bload(rf2, ho, row_off(y_start - 2));
+                f1m2 = (xin & (y_start >= 2)) ? v0 : 0.0f;
+                f1m1 = (xin & up) ? v1 : 0.0f;
+                f1hm1 = (xh_in & up) ? v2 : 0.0f;
+                f2m3 = (xin & (y_start >= 3)) ? w0 : 0.0f;
+                f2m2 = (xin & (y_start >= 2)) ? w1 : 0.0f;
+                f2hm2 = (xh_in & (y_start >= 2)) ? w2 : 0.0f;
+            }
+        }
+        // Slots, kPf rows ahead, consumed and refilled in place (as in the
+        // forward kernel): B, c^2 at row y+1 (block and halo/outer columns);
+        // A and the scatter-table entry at row y (block and halo columns);
+        // F1 at row y and F2 at row y-1 (block and halo columns).
+        float qb[kPf], qq[kPf], qk[kPf], qkq[kPf], qa[kPf], qha[kPf], qe[kPf], qeh[kPf];
+        float qf1[kPf], qf1h[kPf], qf2[kPf], qf2h[kPf], qm[kPf];
+        auto fill = [&](int i, int y) {
+            qb[i] = bload(rb, xo, row_off(y + 1));
+            qq[i] = bload(rb, qo, row_off(y + 1));
+            qk[i] = bload(rk, xo, row_off(y + 1));
+            qkq[i] = bload(rk, qo, row_off(y + 1));
+            qa[i] = bload(ra, xo, row_off(y));
+            qha[i] = bload(ra, ho, row_off(y));
+            qe[i] = 0.0f;
+            qeh[i] = 0.0f;
+            if (inject) {
+                qe[i] = bload(re, xo, row_off(y));
+                qeh[i] = bload(re, ho, row_off(y));
+            }
+            qf1[i] = qf1h[i] = qf2[i] = qf2h[i] = 0.0f;
+            if constexpr (IMG) {
+                qf1[i] = bload(rf1, xo, row_off(y));
+                qf1h[i] = bload(rf1, ho, row_off(y));
+                qf2[i] = bload(rf2, xo, row_off(y - 1));
+                qf2h[i] = bload(rf2, ho, row_off(y - 1));
+            }
+            qm[i] = 0.0f;
+            if constexpr (EDGE) qm[i] = bload(rm, mo, mrow(y));
+        };
+#pragma unroll
+        for (int i = 0; i < kPf; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            fill(i, y_start + i);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        float cm2 = 0.0f, cm1 = 0.0f, hcm1 = 0.0f, pcm2 = 0.0f, pcm1 = 0.0f, hpcm1 = 0.0f;
+        float dm1 = 0.0f, sm1 = 0.0f, t1_prev = 0.0f;
+        int em1 = -1;
+        auto row = [&](int y, int i) {
+            const bool row_in = y < H;
+            const bool nx_in = y + 1 < H;
+            float bp = (xin & nx_in) ? qb[i] : 0.0f;
+            float qp = (xq_in & nx_in) ? qq[i] : 0.0f;
+            float kp = (xin & nx_in) ? qk[i] : 0.0f;
+            float kqp = (xq_in & nx_in) ? qkq[i] : 0.0f;
+            float a0 = (xin & row_in) ? qa[i] : 0.0f;
+            float ha0 = (xh_in & row_in) ? qha[i] : 0.0f;
+            int e0 = (xin & row_in) ? __float_as_int(qe[i]) : -1;
+            int eh0 = (xh_in & row_in) ? __float_as_int(qeh[i]) : -1;
+            float f10 = (xin & row_in) ? qf1[i] : 0.0f;
+            float f1h0 = (xh_in & row_in) ? qf1h[i] : 0.0f;
+            const bool f2_in = (y - 1 >= 0) & (y - 1 < H);
+            float f2m1 = (xin & f2_in) ? qf2[i] : 0.0f;
+            float f2hm1 = (xh_in & f2_in) ? qf2h[i] : 0.0f;
+            float s0 = qm[i];
+            asm volatile("" : "+v"(bp), "+v"(qp), "+v"(kp), "+v"(kqp), "+v"(a0), "+v"(ha0),
+                         "+v"(e0), "+v"(eh0), "+v"(f10), "+v"(f1h0), "+v"(f2m1), "+v"(f2hm1),
+                         "+v"(s0));
+            __builtin_amdgcn_sched_barrier(0);
+            // rows past y_end are not needed: reload row y_end instead (a cache
+            // hit; a conditional refill would make every wait a full one)
+            fill(i, min(y + kPf, y_end));
+            // ---- first step (step n), row y: block columns, then halo columns
+            const float P0 = k0 * b0;
+            const float ph0 = kq0 * q0;
+            float c0 = 0.0f, hc = 0.0f;
+            if (row_in) {
+                const float Pm = km * bm, Pp = kp * bp;
+                const float pw = from_west(P0, ph0), pe = from_east(P0, ph0);
+                c0 = __fmaf_rn(dt2, lap5(pw, pe, Pm, Pp, P0, p.idx2, p.idy2, p.cen),
+                               2.0f * b0 - a0);
+                if (inject && e0 >= 0) c0 += injected(res1, e0);
+                if constexpr (EDGE) {
+                    if (y >= 1 && y <= H - 2) c0 = mur_sides(c0, b0, x, W, s0);
+                    if (y == H - 1) c0 = mur_row(mur, W, H, x, kt_x, b0, bm, cm1, 2);
+                }
+                const float phe = from_east(ph0, 0.0f), phw = from_west(ph0, 0.0f);
+                const float hw = hl ? phe : P0;
+                const float he = hl ? P0 : phw;
+                hc = __fmaf_rn(dt2, lap5(hw, he, kqm1 * qm1, kqp * qp, ph0, p.idx2, p.idy2,
+                                         p.cen),
+                               2.0f * q0 - ha0);
+                if (inject && eh0 >= 0) hc += injected(res1, eh0);
+                if constexpr (EDGE) {
+                    if (h_right_edge) {
+                        if (y >= 1 && y <= H - 2) {
+                            hc = b0 + s0 * (c0 - q0);
+                        } else if (y == H - 1) {
+                            const float from_x = b0 + kx3 * (c0 - q0);
+                            const float from_y = qm1 + ky3 * (hcm1 - q0);
+                            hc = 0.5f * (from_x + from_y);
+                        }
+                    } else if (y == H - 1) {
+                        hc = qm1 + kt_h * (hcm1 - q0);
+                    }
+                    if (y == 1) {
+                        cm1 = mur_row(mur, W, H, x, kb_x, bm, b0, c0, 0);
+                        if (h_right_edge) {
+                            const float from_x = bm + kx1 * (cm1 - qm1);
+                            const float from_y = q0 + ky1 * (hc - qm1);
+                            hcm1 = 0.5f * (from_x + from_y);
+                        } else {
+                            hcm1 = q0 + kb_h * (hc - qm1);
+                        }
+                        pcm1 = km * cm1;
+                        hpcm1 = kqm1 * hcm1;
+                        if (owned & (r0 == 0)) outc[xo / 4] = cm1;
+                    }
+                }
+                if (owned & (y >= r0) & (y < r1) & (y > 0)) outc[(size_t)y * W + x] = c0;
+            }
+            const float pc0 = k0 * c0;
+            const float hpc0 = kq0 * hc;
+            // ---- second step (step n-1), row r = y - 1
+            const int r = y - 1;
+            float d = 0.0f;
+            if (r >= r0 && r < r1) {
+                const float pw = from_west(pcm1, hpcm1), pe = from_east(pcm1, hpcm1);
+                d = __fmaf_rn(dt2, lap5(pw, pe, pcm2, pc0, pcm1, p.idx2, p.idy2, p.cen),
+                              2.0f * cm1 - bm);
+                if (inject && em1 >= 0) d += injected(res2, em1);
+                if constexpr (EDGE) {
+                    if (r >= 1 && r <= H - 2) d = mur_sides(d, cm1, x, W, sm1);
+                    if (r == H - 1) d = mur_row(mur, W, H, x, kt_x, cm1, cm2, dm1, 2);
+                    if (r == 1) {
+                        const float d0 = mur_row(mur, W, H, x, kb_x, cm2, cm1, d, 0);
+                        if (owned) outd[xo / 4] = d0;
+                        dm1 = d0;  // row 0, final, for its imaging below
+                    }
+                }
+                if (owned & (r > 0)) outd[(size_t)r * W + x] = d;
+            }
+            // ---- imaging: C of row y-1 (final now) and D of row y-2
+            if constexpr (IMG) {
+                float t1 = 0.0f, t2 = 0.0f;
+                if (use1) {
+                    const float fw = from_west(f1m1, f1hm1), fe = from_east(f1m1, f1hm1);
+                    t1 = cm1 * lap5(fw, fe, f1m2, f10, f1m1, p.idx2, p.idy2, p.cen);
+                }
+                if (use2) {
+                    const float fw = from_west(f2m2, f2hm2), fe = from_east(f2m2, f2hm2);
+                    t2 = dm1 * lap5(fw, fe, f2m3, f2m1, f2m2, p.idx2, p.idy2, p.cen);
+                }
+                const int ra2 = y - 2;
+                if (ra2 >= r0 && ra2 < r1) {
+                    float *cell = acc + (ra2 - r0) * kLanes + lane;
+                    *cell += t1_prev + t2;
+                }
+                t1_prev = t1;
+                f1m2 = f1m1;
+                f1m1 = f10;
+                f1hm1 = f1h0;
+                f2m3 = f2m2;
+                f2m2 = f2m1;
+                f2hm2 = f2hm1;
+            }
+            cm2 = cm1;
+            cm1 = c0;
+            hcm1 = hc;
+            pcm2 = pcm1;
+            pcm1 = pc0;
+            hpcm1 = hpc0;
+            bm = b0;
+            b0 = bp;
+            qm1 = q0;
+            q0 = qp;
+            km = k0;
+            k0 = kp;
+            kqm1 = kq0;
+            kq0 = kqp;
+            dm1 = d;
+            sm1 = s0;
+            em1 = e0;
+        };
+        for (int y = y_start; y <= y_end; y += kPf) {
+#pragma unroll
+            for (int i = 0; i < kPf; ++i) row(y + i, i);
+        }
+    }
+}
+
+template <unsigned F>
+__global__ void __launch_bounds__(256) adj2_kernel(StepParams p)
+{
+    extern __shared__ float adj2_acc[];
+    constexpr bool IMG = (F & kImage) != 0;
+    const int b = blockIdx.x;
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
+    if (wblk >= p.n_work) return;  // uniform over the workgroup
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
+    const int grp = wblk % p.n_groups;
+    const int t = wblk / p.n_groups;
+    const int cb = t % p.n_cb;
+    const int seg = t / p.n_cb;
+    const int xs = min(cb * kLanes, max(p.nx - kLanes, 0));
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const bool edge = (xs <= 1) || (xs + kLanes >= p.nx - 1) || (r0 == 0) || (r1 >= p.ny - 1);
+    float *acc = adj2_acc + (size_t)wave * p.acc_rows * kLanes;
+    if (edge)
+        adj2_wave<true, IMG>(p, lane, wave, grp, cb, seg, acc);
+    else
+        adj2_wave<false, IMG>(p, lane, wave, grp, cb, seg, acc);
+    if constexpr (IMG) {
+        if (p.img == nullptr && p.img_b == nullptr) return;  // uniform
+        __syncthreads();
+        // partial[grp] += scale * (wave 0 + wave 1 + wave 2 + wave 3), per cell
+        const int W = p.nx;
+        const int own0 = cb * kLanes;
+        const int x = xs + lane;
+        const bool owned = (x < W) & (x >= own0);
+        gf *gp = gstore_ptr(p.g_part) + (size_t)grp * ((size_t)p.ny * W);
+        const int n_rows = r1 - r0;
+        for (int i = wave; i < n_rows; i += kWavesPerBlock) {
+            float sum = adj2_acc[i * kLanes + lane];
+#pragma unroll
+            for (int w = 1; w < kWavesPerBlock; ++w)
+                sum += adj2_acc[((size_t)w * p.acc_rows + i) * kLanes + lane];
+            if (owned) {
+                gf *cell = gp + (size_t)(r0 + i) * W + x;
+                *cell = __fmaf_rn(p.img_scale, sum, *cell);
+            }
+        }
+    }
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
@@ -1292,6 +1680,64 @@ int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
     return SWFD_OK;
 }
 
+// Shape of a two-step adjoint launch: workgroups of four wavefronts take one
+// shot group at one (column block, row segment); each wavefront marches
+// shots_per_wave shots. With imaging there are at most kMaxImageGroups shot
+// groups (one partial image each).
+struct Adj2Shape {
+    int n_cb, n_seg, seg, n_groups, shots_per_group, shots_per_wave, acc_rows;
+};
+
+Adj2Shape adj2_shape(int ny, int nx, int n_shots)
+{
+    static const int seg_rows = std::max(2, env_int("SWFD_ADJ_SEG_ROWS", 32));
+    static const int groups_env = env_int("SWFD_ADJ_GROUPS", kMaxImageGroups);
+    Adj2Shape a{};
+    a.n_cb = (nx + kLanes - 1) / kLanes;
+    a.seg = seg_rows;
+    a.n_seg = std::max(1, ny / seg_rows);
+    a.acc_rows = (ny + a.n_seg - 1) / a.n_seg;
+    const int groups = std::max(1, std::min(std::min(groups_env, kMaxImageGroups), n_shots));
+    a.shots_per_group = (n_shots + groups - 1) / groups;
+    a.shots_per_wave = (a.shots_per_group + kWavesPerBlock - 1) / kWavesPerBlock;
+    a.shots_per_group = a.shots_per_wave * kWavesPerBlock;
+    a.n_groups = (n_shots + a.shots_per_group - 1) / a.shots_per_group;
+    return a;
+}
+
+// Two adjoint steps in one launch: p.f_out (first step) and p.f_out_b
+// (second), residuals p.resid / p.resid_b, imaging fields p.img / p.img_b
+// (either may be NULL) into p.g_part.
+int launch_adj_pair(StepParams &p, hipStream_t stream)
+{
+    const Adj2Shape a = adj2_shape(p.ny, p.nx, p.n_shots);
+    p.n_cb = a.n_cb;
+    p.n_seg = a.n_seg;
+    p.seg = a.seg;
+    p.n_groups = a.n_groups;
+    p.shots_per_group = a.shots_per_group;
+    p.shots_per_wave = a.shots_per_wave;
+    p.acc_rows = a.acc_rows;
+    p.n_work = a.n_cb * a.n_seg * a.n_groups;
+    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    if (p.resid == nullptr || p.resid_b == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
+        return fail(SWFD_EINVAL, "two-step adjoint launch needs residuals and two outputs");
+    const bool image = p.img != nullptr || p.img_b != nullptr;
+    if (image && p.g_part == nullptr)
+        return fail(SWFD_EINVAL, "two-step adjoint imaging needs the partial images");
+    const dim3 block(kLanes * kWavesPerBlock);
+    const dim3 grid((unsigned)(p.chunk * kXcds));
+    if (image) {
+        const size_t lds = (size_t)kWavesPerBlock * a.acc_rows * kLanes * sizeof(float);
+        if (lds > 64 * 1024) return fail(SWFD_EINVAL, "two-step adjoint: row segments too long");
+        hipLaunchKernelGGL((adj2_kernel<kResid | kImage>), grid, block, lds, stream, p);
+    } else {
+        hipLaunchKernelGGL((adj2_kernel<kResid>), grid, block, 0, stream, p);
+    }
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
 // Number of imaging partial slices a step of this shape uses.
 int image_groups(int ny, int nx, int n_shots)
 {
@@ -1659,7 +2105,10 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
     hipStream_t st = as_stream(stream);
     const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
     const size_t plane = (size_t)grid->ny * grid->nx;
-    const int n_part = image_groups(grid->ny, grid->nx, a->n_shots);
+    static const bool two_step_env = env_int("SWFD_TWO_STEP", 1) != 0;
+    const bool two_step = two_step_env && a->work3 != nullptr;
+    const int n_part = two_step ? adj2_shape(grid->ny, grid->nx, a->n_shots).n_groups
+                                : image_groups(grid->ny, grid->nx, a->n_shots);
     bool partials_used = false;
     const size_t sr = (size_t)a->n_shots * rec->n;
     float *const work[4] = {a->work0, a->work1, a->work2, a->work3};
@@ -1674,28 +2123,61 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         if (j < a->hist_first || (j - a->hist_first) % a->hist_every != 0) return -1;
         return (j - a->hist_first) / a->hist_every;
     };
-    for (int n = a->step_hi - 1; n >= a->step_lo; --n) {
-        float *out = free_work(work, p_old, p_cur);
-        if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+    auto use_partials = [&]() -> int {
+        if (!partials_used) {
+            const size_t nz = (size_t)n_part * plane;
+            hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st,
+                               a->workspace, nz, 0.0f);
+            SWFD_HIP_CHECK(hipGetLastError());
+            partials_used = true;
+        }
+        return SWFD_OK;
+    };
+    // Two-step launches image their own outputs (steps n and n-1); a single
+    // step images the step before it, and the last step is imaged after the
+    // loop. With two-step launches only the last step (when the count is odd)
+    // runs alone, so no step is imaged twice.
+    int n = a->step_hi - 1;
+    while (n >= a->step_lo) {
         StepParams p;
         base_params(p, grid, medium, a->n_shots);
         attach_receivers(p, rec);
         p.f_old = p_old;
         p.f_cur = p_cur;
-        p.f_out = out;
         p.resid = a->residual + (size_t)n * sr;
-        if (n < a->step_hi - 1) {
+        if (two_step && n - 1 >= a->step_lo) {
+            float *out_c = free_work(work, p_old, p_cur);
+            float *out_d = free_work(work, p_old, p_cur, out_c);
+            if (out_c == nullptr || out_d == nullptr)
+                return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+            p.f_out = out_c;
+            p.f_out_b = out_d;
+            p.resid_b = a->residual + (size_t)(n - 1) * sr;
+            const long s1 = image_slot(n), s2 = image_slot(n - 1);
+            if (s1 >= a->hist_slots || s2 >= a->hist_slots)
+                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+            if (s1 >= 0) p.img = a->hist + (size_t)s1 * field;
+            if (s2 >= 0) p.img_b = a->hist + (size_t)s2 * field;
+            if (s1 >= 0 || s2 >= 0) {
+                if ((rc = use_partials())) return rc;
+                p.g_part = a->workspace;
+                p.img_scale = a->image_scale;
+            }
+            if ((rc = launch_adj_pair(p, st))) return rc;
+            p_old = out_c;
+            p_cur = out_d;
+            n -= 2;
+            continue;
+        }
+        float *out = free_work(work, p_old, p_cur);
+        if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+        p.f_out = out;
+        if (!two_step && n < a->step_hi - 1) {
             const long slot = image_slot(n + 1);
             if (slot >= a->hist_slots)
                 return fail(SWFD_EINVAL, "adjoint: history slot out of range");
             if (slot >= 0) {
-                if (!partials_used) {
-                    const size_t nz = (size_t)n_part * plane;
-                    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256),
-                                       0, st, a->workspace, nz, 0.0f);
-                    SWFD_HIP_CHECK(hipGetLastError());
-                    partials_used = true;
-                }
+                if ((rc = use_partials())) return rc;
                 p.img = a->hist + (size_t)slot * field;
                 p.g_part = a->workspace;
                 p.img_scale = a->image_scale;
@@ -1704,13 +2186,16 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         if ((rc = launch_step(p, true, st, 0))) return rc;
         p_old = p_cur;
         p_cur = out;
+        n -= 1;
     }
     if (partials_used) {
         hipLaunchKernelGGL(reduce_groups_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256),
                            0, st, plane, n_part, a->workspace, a->grad);
         SWFD_HIP_CHECK(hipGetLastError());
     }
-    if (a->step_hi > a->step_lo) {
+    // the last step, unless a two-step launch imaged it
+    const bool last_imaged = two_step && (a->step_hi - a->step_lo) % 2 == 0;
+    if (a->step_hi > a->step_lo && !last_imaged) {
         const long slot = image_slot(a->step_lo);
         if (slot >= a->hist_slots) return fail(SWFD_EINVAL, "adjoint: history slot out of range");
         if (slot >= 0 &&

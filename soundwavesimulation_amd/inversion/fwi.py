@@ -53,14 +53,14 @@ def gradient_bytes_per_shot(n_steps, grid_shape, n_receivers, store_every=1, wav
     """Device memory one gradient shot needs.
 
     ``store_every == 1`` with full-precision history: checkpointed plan,
-    ``(7 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
+    ``(8 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
     Otherwise the reference's stored-history estimate (invert_ring.py:463-473).
     """
     cells = grid_shape[0] * grid_shape[1]
     if int(store_every) == 1 and wave_dtype in (None, _F32):
         K = checkpoint_every or default_checkpoint_every(n_steps)
         _, n_ckpt, n_hist = segment_plan(n_steps, K)
-        return (7 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
+        return (8 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
     wave_bytes = torch.empty((), dtype=wave_dtype or _F32).element_size()
     return (stored_steps(n_steps, store_every) * cells * wave_bytes + 12 * cells * 4
             + 4 * n_steps * n_receivers * 4)
@@ -119,7 +119,7 @@ class CheckpointedGradient:
         dev = solver.model.device
         shape = (self.n_shots, *solver.model.shape)
         self.work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
-        self.lam = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(3)]
+        self.lam = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
         self.ckpt = (torch.empty((max(n_ckpt, 2), *shape), device=dev, dtype=_F32)
                      if n_ckpt > 0 else None)
         self.hist = torch.empty((n_hist, *shape), device=dev, dtype=_F32)
@@ -422,7 +422,7 @@ class TorchLeastSquaresFWI:
                         else residual.contiguous())
         dev = self.model.device
         shape = (S, *self.model.shape)
-        lam = [torch.zeros(shape, device=dev, dtype=_F32) for _ in range(3)]
+        lam = [torch.zeros(shape, device=dev, dtype=_F32) for _ in range(4)]
         adjoint_run(self.solver, self.sampler, S, residual_nsr, self.n_steps, 0,
                     state=(lam[0], lam[1]), work=lam, hist=wave.contiguous(), hist_first=0,
                     hist_every=self.store_every, image_scale=self.store_every * batched.dt2,

@@ -513,8 +513,9 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
     """Adjoint steps ``step_hi-1 .. step_lo`` in one native call.
 
     ``state`` = (lam_old, lam) before the run; the state after the run is
-    written back into ``state``'s tensors. ``work`` holds three scratch fields
-    (may include the state tensors).
+    written back into ``state``'s tensors. ``work`` holds three or four
+    scratch fields (may include the state tensors); with four the run takes
+    two steps per launch.
     """
     dev = solver.model.device
     solver._sync_medium()
