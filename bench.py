@@ -59,6 +59,9 @@ def parse():
     p.add_argument("--no-torch-reference", action="store_true")
     p.add_argument("--cpu-shots", type=int, default=4)
     p.add_argument("--cpu-steps", type=int, default=1000)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run only a plain streaming kernel of known size (c = a + b over two "
+                        "2 GiB float32 inputs, 5 launches), to scale HBM traffic counters")
     p.add_argument("--dry-run", action="store_true",
                    help="build the host-side geometry only and print it (no GPU)")
     a = p.parse_args()
@@ -286,6 +289,17 @@ def main():
               "(shots would shard across GPUs with one all-reduce of the gradient).",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        n = 2 ** 29
+        x = torch.ones(n, device="cuda")
+        y = torch.ones(n, device="cuda")
+        z = torch.empty(n, device="cuda")
+        for _ in range(5):
+            torch.add(x, y, out=z)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"read_bytes": 2 * 4 * n, "write_bytes": 4 * n,
+                                               "launches": 5}}))
+        return
     if a.dry_run:
         c_true, x, dt, h, radius, pulse, array = host_geometry(a)
         print(json.dumps({"grid": a.grid, "dt": dt, "radius_m": radius, "shots": a.shots,
