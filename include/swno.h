@@ -1,0 +1,109 @@
+/* swno.h -- C ABI of the spectral neural operator kernels (MI355X, gfx950).
+ *
+ * The learned surrogate has no counterpart in the reference's code: its
+ * design document (TFNO_Full_Ring_Plan.tex:200-209, 252-254) describes a
+ * Fourier neural operator that maps measurements to slowness squared. This
+ * library holds the device kernels the host-side operator
+ * (soundwavesimulation_amd/surrogate/operator.py) is built from: one batched
+ * strided GEMM on bf16 MFMA with fp32 accumulation, which carries every
+ * contraction (truncated DFTs, per-mode complex mixing, pointwise linears and
+ * all their gradients), and the elementwise pieces around it.
+ *
+ * All pointers are device pointers to float32 unless stated; every entry
+ * point enqueues on `stream` (a hipStream_t) and returns SWNO_OK or an error
+ * code with a message in swno_last_error(). Nothing here allocates.
+ */
+#ifndef SWNO_H
+#define SWNO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWNO_OK 0
+#define SWNO_EINVAL 1
+#define SWNO_EHIP 2
+
+/* C[b] = alpha * A[b] B[b] + beta * C[b] for b < batch, with
+ *   A[b](m, k) = A[b * a_bs + m * a_rs + k * a_cs]   (M x K)
+ *   B[b](k, n) = B[b * b_bs + k * b_rs + n * b_cs]   (K x N)
+ *   C[b](m, n) = C[b * c_bs + m * c_rs + n * c_cs]   (M x N)
+ * Operands are rounded to bf16 (nearest even) as they are staged; products
+ * accumulate in fp32. k_split > 1 splits K into that many slices whose fp32
+ * partials go to `workspace` (swno_gemm_workspace_floats) and are summed in a
+ * fixed order, so results do not depend on scheduling. beta == 0 ignores the
+ * old contents of C (which may then be uninitialised). */
+typedef struct swno_gemm_args {
+    int32_t M, N, K, batch, k_split;
+    const float *A;
+    int64_t a_rs, a_cs, a_bs;
+    const float *B;
+    int64_t b_rs, b_cs, b_bs;
+    float *C;
+    int64_t c_rs, c_cs, c_bs;
+    float alpha, beta;
+} swno_gemm_args;
+
+const char *swno_last_error(void);
+
+size_t swno_gemm_workspace_floats(const swno_gemm_args *g);
+
+int swno_gemm(const swno_gemm_args *g, float *workspace, void *stream);
+
+/* pre[r, c] = x[r, c] + bias[c] (bias may be NULL); y = GELU(pre) when
+ * gelu != 0, else y = pre. GELU is the erf form 0.5 x (1 + erf(x / sqrt 2)).
+ * pre may equal x; y may equal pre only when gelu == 0. */
+int swno_bias_act(int64_t rows, int32_t cols, const float *x, const float *bias, float *pre,
+                  float *y, int32_t gelu, void *stream);
+
+/* dx[i] = dy[i] * GELU'(pre[i]) (dx may equal dy). */
+int swno_gelu_backward(int64_t n, const float *pre, const float *dy, float *dx, void *stream);
+
+/* out[c] = beta * out[c] + sum_r x[r, c], summed in a fixed order through
+ * `workspace` (swno_col_sum_workspace_floats floats). */
+size_t swno_col_sum_workspace_floats(int64_t rows, int32_t cols);
+int swno_col_sum(int64_t rows, int32_t cols, const float *x, float beta, float *out,
+                 float *workspace, void *stream);
+
+/* Spectral repacks between the truncated transforms and the mode mixing, for
+ * `batch` samples, `modes` retained modes per axis and `ch` channels:
+ *   op 0 (combine):   P (batch, 2m, 2m, ch) -> X (batch, m, m, 2 ch)
+ *        X_re = P[q][j] + P[m+q][m+j], X_im = P[q][m+j] - P[m+q][j]
+ *   op 1 (combine^T): dX -> dP (the transpose of op 0)
+ *   op 2 (expand):    Y (batch, m, m, 2 ch) -> U (batch, 2m, 2m, ch)
+ *        U[q][j] = Y_re, U[q][m+j] = Y_im, U[m+q][j] = Y_im, U[m+q][m+j] = -Y_re
+ *   op 3 (expand^T):  dU -> dY
+ * where re / im are the first / second `ch` entries of the last axis. */
+int swno_spectral_pack(int32_t op, int32_t batch, int32_t modes, int32_t ch, const float *in,
+                       float *out, void *stream);
+
+/* Mode-mixing weights: rm[mode] = [[re, im], [-im, re]] (2ch x 2ch) from
+ * re/im (modes2, ch, ch), and the gradient back: d_re = dR11 + dR22,
+ * d_im = dR12 - dR21 (accumulated into d_re / d_im when accumulate != 0). */
+int swno_mix_assemble(int32_t modes2, int32_t ch, const float *re, const float *im, float *rm,
+                      void *stream);
+int swno_mix_assemble_backward(int32_t modes2, int32_t ch, const float *drm, float *d_re,
+                               float *d_im, int32_t accumulate, void *stream);
+
+/* Masked mean squared error: loss[0] = sum_i w_i (p_i - t_i)^2 / sum_i w_i
+ * with w = mask (NULL: all ones), loss[1] = 1 / sum_i w_i (loss holds two
+ * floats), grad = d loss[0] / d p. Deterministic two-pass sum through
+ * `workspace` (swno_mse_workspace_floats floats). */
+size_t swno_mse_workspace_floats(int64_t n);
+int swno_mse(int64_t n, const float *pred, const float *target, const float *mask, float *loss,
+             float *grad, float *workspace, void *stream);
+
+/* One AdamW update of n parameters (decoupled weight decay, bias-corrected
+ * moments; step >= 1). */
+int swno_adamw(int64_t n, float *param, const float *grad, float *m, float *v, float lr,
+               float beta1, float beta2, float eps, float weight_decay, int32_t step,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWNO_H */

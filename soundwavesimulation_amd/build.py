@@ -1,9 +1,15 @@
-"""Compile ``csrc/swfd_kernels.hip`` into ``libswfd.so`` for gfx950.
+"""Compile the HIP sources into shared libraries for gfx950.
+
+* ``csrc/swfd_kernels.hip`` -> ``libswfd.so`` (FDTD forward / adjoint, C ABI
+  in ``include/swfd.h``);
+* ``csrc/swno_kernels.hip`` -> ``libswno.so`` (spectral neural operator
+  kernels, C ABI in ``include/swno.h``).
 
 ``python -m soundwavesimulation_amd.build`` (or ``__graft_entry__.build()``)
-runs one ``hipcc`` command; the library lands next to this file so it ships
-with the source tree. Packed-FP32 instruction selection is disabled and FMA
-contraction is off (the kernels place their fused multiply-adds explicitly).
+runs one ``hipcc`` command per library; the libraries land next to this file
+so they ship with the source tree. Packed-FP32 instruction selection is
+disabled and FMA contraction is off (the kernels place their fused
+multiply-adds explicitly).
 """
 
 from __future__ import annotations
@@ -17,28 +23,31 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 SOURCE = os.path.join(HERE, "csrc", "swfd_kernels.hip")
 LIBRARY = os.path.join(HERE, "libswfd.so")
+NO_SOURCE = os.path.join(HERE, "csrc", "swno_kernels.hip")
+NO_LIBRARY = os.path.join(HERE, "libswno.so")
 ARCH = os.environ.get("SWFD_OFFLOAD_ARCH", "gfx950")
 
 
-def hipcc_command(output=LIBRARY, arch=ARCH):
+def hipcc_command(output=LIBRARY, arch=ARCH, source=SOURCE):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     return [
         hipcc, f"--offload-arch={arch}", "-O3", "-std=c++17", "-fPIC", "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(REPO, "include"),
-        "-o", output, SOURCE,
+        "-o", output, source,
     ]
 
 
 def build(verbose=True):
-    """Compile the library (always recompiles) and return its path."""
-    tmp = LIBRARY + ".tmp"
-    cmd = hipcc_command(tmp)
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIBRARY)
+    """Compile both libraries (always recompiles); return the FDTD library's path."""
+    for source, library in ((SOURCE, LIBRARY), (NO_SOURCE, NO_LIBRARY)):
+        tmp = library + ".tmp"
+        cmd = hipcc_command(tmp, source=source)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, library)
     return LIBRARY
 
 
