@@ -211,6 +211,53 @@ int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *
                   const int32_t *obs_index, const int32_t *mute, float *residual,
                   float *workspace, float *misfit, void *stream);
 
+/* Batched runs (no counterpart in the reference, whose batching stops at the
+ * shots of one problem, torch_backend.py:601-770): n independent problems,
+ * each with its own grid, medium, receivers and step range, advance in
+ * lockstep, and at every tick the two-step launches of all problems that
+ * have one go out as ONE batched kernel launch (a per-problem parameter
+ * table; odd trailing steps run as single-problem launches). Results equal
+ * n separate swfd_forward / swfd_adjoint calls bit for bit.
+ *
+ * Three phases, so that the launch phase can be captured in a HIP graph:
+ *   *_table_bytes: size of the device table the run needs (host only);
+ *   *_prepare:     writes the tables into `table` (device memory of that
+ *                  size) with a blocking copy; call again when any argument
+ *                  (pointer, step range, tuning) changes;
+ *   swfd_*_batch:  launches, reading the prepared tables; asynchronous on
+ *                  `stream`, and contains no host synchronisation.
+ * The argument arrays hold one entry per problem. */
+int swfd_forward_batch_table_bytes(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                                   const swfd_receivers *recs, const swfd_forward_args *args,
+                                   size_t *bytes);
+int swfd_forward_batch_prepare(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                               const swfd_receivers *recs, const swfd_forward_args *args,
+                               void *table, size_t table_bytes);
+int swfd_forward_batch(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                       const swfd_receivers *recs, const swfd_forward_args *args,
+                       const void *table, size_t table_bytes, void *stream);
+int swfd_adjoint_batch_table_bytes(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                                   const swfd_receivers *recs, const swfd_adjoint_args *args,
+                                   size_t *bytes);
+int swfd_adjoint_batch_prepare(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                               const swfd_receivers *recs, const swfd_adjoint_args *args,
+                               void *table, size_t table_bytes);
+int swfd_adjoint_batch(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                       const swfd_receivers *recs, const swfd_adjoint_args *args,
+                       const void *table, size_t table_bytes, void *stream);
+
+/* Launch-shape tuning for one grid size (no counterpart in the reference):
+ *   kind 0: two-step forward; seg_rows = rows per march, param = target
+ *           work items per CU;
+ *   kind 1: two-step adjoint; seg_rows (<= 32), param = shot groups (<= 8);
+ *   kind 2: temporal depth: depth 1 = single-step kernels, 2 = two-step.
+ * Values <= 0 keep the defaults. swfd_tune_clear forgets every entry. The
+ * table is process-wide host state: set it before running (not while another
+ * host thread launches). */
+int swfd_tune(int32_t kind, int32_t ny, int32_t nx, int32_t seg_rows, int32_t param,
+              int32_t depth);
+void swfd_tune_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

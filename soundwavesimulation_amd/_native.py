@@ -71,7 +71,9 @@ EXPORTS = (
     "swfd_last_error", "swfd_version", "swfd_medium_floats", "swfd_prepare_medium",
     "swfd_medium_view", "swfd_step", "swfd_gather", "swfd_scatter", "swfd_laplacian",
     "swfd_forward", "swfd_adjoint", "swfd_adjoint_workspace_floats",
-    "swfd_residual_workspace_floats", "swfd_residual",
+    "swfd_residual_workspace_floats", "swfd_residual", "swfd_forward_batch_table_bytes",
+    "swfd_forward_batch_prepare", "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
+    "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
 )
 
 _lib = None
@@ -110,8 +112,22 @@ def load_library(path=LIBRARY):
     lib.swfd_residual_workspace_floats.argtypes = [_c_int, _c_int, _c_int]
     lib.swfd_residual.argtypes = [_c_int, _c_int, _c_int, vp, vp, ctypes.c_int64,
                                   ctypes.c_int64, vp, vp, vp, vp, vp, vp]
+    size_p = P(ctypes.c_size_t)
+    for kind, args_t in (("forward", ForwardArgs), ("adjoint", AdjointArgs)):
+        getattr(lib, f"swfd_{kind}_batch_table_bytes").argtypes = [
+            _c_int, P(Grid), P(Medium), P(Receivers), P(args_t), size_p]
+        getattr(lib, f"swfd_{kind}_batch_prepare").argtypes = [
+            _c_int, P(Grid), P(Medium), P(Receivers), P(args_t), vp, ctypes.c_size_t]
+        getattr(lib, f"swfd_{kind}_batch").argtypes = [
+            _c_int, P(Grid), P(Medium), P(Receivers), P(args_t), vp, ctypes.c_size_t, vp]
+    lib.swfd_tune.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]
+    lib.swfd_tune_clear.restype = None
+    lib.swfd_tune_clear.argtypes = []
     for name in ("swfd_prepare_medium", "swfd_step", "swfd_gather", "swfd_scatter",
-                 "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual"):
+                 "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
+                 "swfd_forward_batch_table_bytes", "swfd_forward_batch_prepare",
+                 "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
+                 "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

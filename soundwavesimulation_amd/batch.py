@@ -1,0 +1,260 @@
+"""Many independent problems in lockstep: batched launches, HIP graphs, tuning.
+
+The reference batches the shots of one problem (``torch_backend.py:601-770``);
+here several problems -- each with its own grid, medium, ring, shot count and
+step count -- advance together, and at every tick the two-step launches of
+all of them go out as one batched kernel launch (``swfd_forward_batch`` /
+``swfd_adjoint_batch`` in ``include/swfd.h``).
+
+* ``BatchedForward`` / ``BatchedAdjoint``: one prepared lockstep run over a
+  list of per-problem run descriptions (the keyword arguments of
+  ``torch_backend.forward_run`` / ``adjoint_run``). ``prepare()`` writes the
+  per-tick parameter tables to the device; ``launch()`` only enqueues kernels,
+  so it can be captured in a graph.
+* ``UnevenGradients``: the checkpointed FWI gradient of every problem (forward
+  with checkpoints, residual and misfit, then segment recomputation and
+  adjoint with imaging), scheduled so that every problem's segments run
+  back to front and the phase of one problem never waits for another's; the
+  whole sequence is built once and can be captured into a
+  ``torch.cuda.CUDAGraph`` (a HIP graph on ROCm) and replayed.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from .inversion.fwi import default_checkpoint_every, segment_plan
+from .torch_backend import _source_values_tensor
+
+_F32 = torch.float32
+
+
+def _ptr(t):
+    return _native.ptr(t)
+
+
+def _forward_args(solver, n_shots, rows, cols, values, n_steps, *, init, work, samples=None,
+                  hist=None, hist_first=0, hist_every=1, ckpt=None, ckpt_every=0, step_begin=0,
+                  final=None):
+    return _native.ForwardArgs(
+        n_shots=n_shots, step_begin=step_begin, n_steps=n_steps,
+        init_old=_ptr(init[0]), init_u=_ptr(init[1]),
+        work0=_ptr(work[0]), work1=_ptr(work[1]), work2=_ptr(work[2]),
+        work3=_ptr(work[3]) if len(work) > 3 else None,
+        final_old=_ptr(final[0]) if final is not None else None,
+        final_u=_ptr(final[1]) if final is not None else None,
+        src_rows=_ptr(rows), src_cols=_ptr(cols), src_values=_ptr(values),
+        samples=_ptr(samples), hist=_ptr(hist), hist_first=hist_first, hist_every=hist_every,
+        hist_slots=0 if hist is None else hist.shape[0], ckpt=_ptr(ckpt), ckpt_every=ckpt_every,
+        ckpt_slots=0 if ckpt is None else ckpt.shape[0])
+
+
+def _adjoint_args(n_shots, residual, step_hi, step_lo, *, state, work, hist=None, hist_first=0,
+                  hist_every=1, image_scale=0.0, grad=None, workspace=None):
+    return _native.AdjointArgs(
+        n_shots=n_shots, step_hi=step_hi, step_lo=step_lo,
+        init_old=_ptr(state[0]), init_u=_ptr(state[1]),
+        work0=_ptr(work[0]), work1=_ptr(work[1]), work2=_ptr(work[2]),
+        work3=_ptr(work[3]) if len(work) > 3 else None,
+        final_old=_ptr(state[0]), final_u=_ptr(state[1]), residual=_ptr(residual),
+        hist=_ptr(hist), hist_first=hist_first, hist_every=hist_every,
+        hist_slots=0 if hist is None else hist.shape[0], image_scale=float(image_scale),
+        grad=_ptr(grad), workspace=_ptr(workspace))
+
+
+class _Prepared:
+    """Common part of the batched forward / adjoint runs."""
+
+    kind = ""
+
+    def __init__(self, solvers, samplers, args, device):
+        n = len(args)
+        self.n = n
+        self.device = device
+        self._keep = (solvers, samplers)
+        for s in solvers:
+            s._sync_medium()
+        self.grids = (_native.Grid * n)(*[s._grid for s in solvers])
+        self.media = (_native.Medium * n)(*[s._medium for s in solvers])
+        recs = [smp._rec if smp is not None else _native.Receivers() for smp in samplers]
+        self.recs = (_native.Receivers * n)(*recs)
+        args_t = _native.ForwardArgs if self.kind == "forward" else _native.AdjointArgs
+        self.args = (args_t * n)(*args)
+        lib = _native.lib()
+        nbytes = ctypes.c_size_t(0)
+        _native.check(getattr(lib, f"swfd_{self.kind}_batch_table_bytes")(
+            n, self.grids, self.media, self.recs, self.args, ctypes.byref(nbytes)))
+        self.table_bytes = int(nbytes.value)
+        self.table = torch.empty(max(self.table_bytes, 256), dtype=torch.uint8, device=device)
+
+    def prepare(self):
+        """Write the per-tick tables to the device (blocking)."""
+        _native.check(getattr(_native.lib(), f"swfd_{self.kind}_batch_prepare")(
+            self.n, self.grids, self.media, self.recs, self.args, _ptr(self.table),
+            self.table.numel()))
+        return self
+
+    def launch(self):
+        """Enqueue the run on torch's current stream (no host synchronisation)."""
+        _native.check(getattr(_native.lib(), f"swfd_{self.kind}_batch")(
+            self.n, self.grids, self.media, self.recs, self.args, _ptr(self.table),
+            self.table.numel(), _native.stream_of(self.device)))
+
+
+class BatchedForward(_Prepared):
+    """Lockstep forward runs; ``runs`` = list of (solver, sampler or None, kwargs)."""
+
+    kind = "forward"
+
+    def __init__(self, runs):
+        solvers = [r[0] for r in runs]
+        samplers = [r[1] for r in runs]
+        args = [_forward_args(r[0], **r[2]) for r in runs]
+        super().__init__(solvers, samplers, args, solvers[0].model.device)
+
+
+class BatchedAdjoint(_Prepared):
+    """Lockstep adjoint runs; ``runs`` = list of (solver, sampler, kwargs)."""
+
+    kind = "adjoint"
+
+    def __init__(self, runs):
+        solvers = [r[0] for r in runs]
+        samplers = [r[1] for r in runs]
+        args = [_adjoint_args(**r[2]) for r in runs]
+        super().__init__(solvers, samplers, args, solvers[0].model.device)
+
+
+class GradientProblem:
+    """One FWI gradient problem of an uneven batch (all ring elements fire)."""
+
+    def __init__(self, solver, sampler, array, pulse, n_steps, observed, checkpoint_every=None):
+        self.solver, self.sampler, self.array = solver, sampler, array
+        dev = solver.model.device
+        self.S = array.n_elements
+        self.N = int(n_steps)
+        self.K = int(checkpoint_every or default_checkpoint_every(self.N))
+        self.n_seg, n_ckpt, n_hist = segment_plan(self.N, self.K)
+        shape = (self.S, *solver.model.shape)
+        e = lambda *s: torch.empty(s, device=dev, dtype=_F32)  # noqa: E731
+        self.work = [e(*shape) for _ in range(4)]
+        # The adjoint state shares the forward work fields: the forward is
+        # over before the first adjoint step, and the segment recomputation
+        # writes every output into the history (its work fields stay idle).
+        self.lam = self.work
+        self.ckpt = e(max(n_ckpt, 2), *shape) if n_ckpt > 0 else None
+        self.hist = e(n_hist, *shape)
+        ny, nx = solver.model.shape
+        self.workspace = e(int(_native.lib().swfd_adjoint_workspace_floats(ny, nx, self.S)))
+        R = sampler.n_elements
+        self.samples = e(self.N, self.S, R)
+        self.residual = e(self.N, self.S, R)
+        self.res_ws = e(int(_native.lib().swfd_residual_workspace_floats(self.N, self.S, R)) + 1)
+        self.misfit = e(1)
+        self.grad = torch.zeros(solver.model.shape, device=dev, dtype=_F32)
+        rc = [array.inject_rows_cols(i) for i in range(self.S)]
+        self.rows = torch.as_tensor(np.array([r for r, _ in rc], dtype=np.int32), device=dev)
+        self.cols = torch.as_tensor(np.array([c for _, c in rc], dtype=np.int32), device=dev)
+        self.index = torch.arange(self.S, dtype=torch.int32, device=dev)
+        self.values = _source_values_tensor(torch.as_tensor(pulse), self.N, dev)
+        self.observed = observed.contiguous()
+        self.cells = 2.0 * ny * nx * self.N * self.S
+        self.memory_bytes = sum(t.numel() * 4 for t in
+                                [*self.work, self.hist, self.workspace, self.samples,
+                                 self.residual] + ([self.ckpt] if self.ckpt is not None else []))
+
+    def segment(self, j):
+        n0 = j * self.K
+        return n0, min(n0 + self.K, self.N)
+
+    def residual_and_misfit(self):
+        """Residual (N, S, R) and misfit into preallocated buffers (capturable)."""
+        lib = _native.lib()
+        R = self.sampler.n_elements
+        obs = self.observed
+        _native.check(lib.swfd_residual(
+            self.N, self.S, R, _ptr(self.samples), _ptr(obs), obs.stride(0), obs.stride(1),
+            _ptr(self.index), _ptr(self.index), _ptr(self.residual), _ptr(self.res_ws),
+            _ptr(self.misfit), _native.stream_of(self.solver.model.device)))
+
+
+class UnevenGradients:
+    """FWI gradients of many uneven problems, batched per tick.
+
+    Schedule: one batched forward over all problems (each for its own step
+    count, with samples and checkpoints); the residual of each problem; then
+    phase i = 0, 1, ... runs, for every problem with a segment left, its
+    segment ``n_seg - 1 - i`` (checkpoint copy, batched recomputation, batched
+    adjoint with imaging). Short problems finish after fewer phases.
+    """
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        self.phases = max(p.n_seg for p in self.problems)
+        self.forward = BatchedForward([
+            (p.solver, p.sampler, dict(n_shots=p.S, rows=p.rows, cols=p.cols, values=p.values,
+                                       n_steps=p.N, init=(p.work[0], p.work[1]), work=p.work,
+                                       samples=p.samples, ckpt=p.ckpt,
+                                       ckpt_every=p.K if p.ckpt is not None else 0))
+            for p in self.problems]).prepare()
+        self.recompute, self.adjoint, self.members = [], [], []
+        for i in range(self.phases):
+            mem = [p for p in self.problems if p.n_seg - 1 - i >= 0]
+            fwd_runs, adj_runs = [], []
+            for p in mem:
+                j = p.n_seg - 1 - i
+                n0, n1 = p.segment(j)
+                n_re = n1 - 1 - n0
+                if n_re > 0:
+                    fwd_runs.append((p.solver, None, dict(
+                        n_shots=p.S, rows=p.rows, cols=p.cols, values=p.values, n_steps=n_re,
+                        init=(p.hist[0], p.hist[1]), work=p.work, hist=p.hist,
+                        hist_first=n0 - 2, hist_every=1, step_begin=n0)))
+                adj_runs.append((p.solver, p.sampler, dict(
+                    n_shots=p.S, residual=p.residual, step_hi=n1, step_lo=n0,
+                    state=(p.lam[0], p.lam[1]), work=p.lam, hist=p.hist, hist_first=n0 - 2,
+                    hist_every=1, image_scale=p.solver.dt2, grad=p.grad,
+                    workspace=p.workspace)))
+            self.members.append(mem)
+            self.recompute.append(BatchedForward(fwd_runs).prepare() if fwd_runs else None)
+            self.adjoint.append(BatchedAdjoint(adj_runs).prepare())
+
+    def run(self, events=None):
+        """Enqueue every gradient. ``events`` (dict problem index -> torch
+        event) are recorded right after each problem's last launch."""
+        for p in self.problems:
+            p.work[0].zero_()
+            p.work[1].zero_()
+        self.forward.launch()
+        for p in self.problems:
+            p.residual_and_misfit()
+            p.grad.zero_()
+            p.lam[0].zero_()
+            p.lam[1].zero_()
+        for i in range(self.phases):
+            for p in self.members[i]:
+                j = p.n_seg - 1 - i
+                if j == 0:
+                    p.hist[0].zero_()
+                    p.hist[1].zero_()
+                else:
+                    p.hist[0].copy_(p.ckpt[2 * (j - 1)])
+                    p.hist[1].copy_(p.ckpt[2 * (j - 1) + 1])
+            if self.recompute[i] is not None:
+                self.recompute[i].launch()
+            self.adjoint[i].launch()
+            if events is not None:
+                for p in self.members[i]:
+                    if p.n_seg - 1 - i == 0:
+                        events[self.problems.index(p)].record()
+
+    @property
+    def cells(self):
+        return sum(p.cells for p in self.problems)
+
+
+__all__ = ["BatchedForward", "BatchedAdjoint", "GradientProblem", "UnevenGradients"]

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "swfd.h"
 
@@ -873,10 +874,10 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
     }
 }
 
+// Workgroup b of a two-step forward launch of one problem.
 template <unsigned F>
-__global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
+__device__ __forceinline__ void fwd2_block(const StepParams &p, int b)
 {
-    const int b = blockIdx.x;
     if constexpr ((F & kGather) != 0) {
         if (b >= p.n_stencil_blocks) {
             gather_pairs(p, (b - p.n_stencil_blocks) * (int)blockDim.x + (int)threadIdx.x);
@@ -903,6 +904,33 @@ __global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
         fwd2_wave<true, F>(p, lane, grp, cb, seg);
     else
         fwd2_wave<false, F>(p, lane, grp, cb, seg);
+}
+
+template <unsigned F>
+__global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
+{
+    fwd2_block<F>(p, (int)blockIdx.x);
+}
+
+// Problem of workgroup b in a batched launch: entry e owns workgroups
+// [first[e], first[e + 1]) (a wavefront-uniform scan of a short table).
+__device__ __forceinline__ int batch_entry(const int *first, int n, int b)
+{
+    int e = 0;
+    while (e + 1 < n && first[e + 1] <= b) ++e;
+    return e;
+}
+
+// One launch advancing several problems by two forward steps each: entry e
+// of `table` holds problem e's launch parameters (as the single-problem
+// kernel receives them).
+template <unsigned F>
+__global__ void __launch_bounds__(256) fwd2_batch_kernel(const StepParams *table, const int *first,
+                                                         int n)
+{
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    fwd2_block<F>(table[e], b - first[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1231,12 +1259,13 @@ __device__ __forceinline__ void adj2_wave(const StepParams &p, int lane, int wav
     }
 }
 
+extern __shared__ float adj2_acc[];
+
+// Workgroup b of a two-step adjoint launch of one problem.
 template <unsigned F>
-__global__ void __launch_bounds__(256) adj2_kernel(StepParams p)
+__device__ __forceinline__ void adj2_block(const StepParams &p, int b)
 {
-    extern __shared__ float adj2_acc[];
     constexpr bool IMG = (F & kImage) != 0;
-    const int b = blockIdx.x;
     const int wblk = (b % kXcds) * p.chunk + b / kXcds;
     if (wblk >= p.n_work) return;  // uniform over the workgroup
     const int lane = threadIdx.x & (kLanes - 1);
@@ -1275,6 +1304,23 @@ __global__ void __launch_bounds__(256) adj2_kernel(StepParams p)
             }
         }
     }
+}
+
+template <unsigned F>
+__global__ void __launch_bounds__(256) adj2_kernel(StepParams p)
+{
+    adj2_block<F>(p, (int)blockIdx.x);
+}
+
+// Batched two-step adjoint launch (see fwd2_batch_kernel); the LDS of the
+// launch is sized for the largest imaging accumulator of its entries.
+template <unsigned F>
+__global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table, const int *first,
+                                                         int n)
+{
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    adj2_block<F>(table[e], b - first[e]);
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
@@ -1631,14 +1677,32 @@ int launch_step(StepParams &p, bool adjoint, hipStream_t stream, int gather_pair
     return SWFD_OK;
 }
 
-// Two forward steps in one launch: outputs p.f_out (first step) and
-// p.f_out_b (second); p.f_out2 / p.f_out2_b optional copies. Work items are
-// (shot group, 62-column block, row segment) marches.
-int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
+// Tuning table (filled by swfd_tune, e.g. from an autotuner): per kind and
+// grid size, the row-segment height, a launch-shape parameter and the
+// temporal depth (1: single-step kernels, 2: two-step kernels).
+struct TuneEntry {
+    int kind, ny, nx, seg_rows, param, depth;
+};
+std::vector<TuneEntry> g_tune;
+
+const TuneEntry *find_tune(int kind, int ny, int nx)
 {
-    static const int seg_rows = std::max(2, env_int("SWFD_SEG_ROWS", kSegRows));
-    static const int waves_per_cu = env_int("SWFD_TB_WAVES_PER_CU", 256);
+    for (const TuneEntry &t : g_tune)
+        if (t.kind == kind && t.ny == ny && t.nx == nx) return &t;
+    return nullptr;
+}
+
+// Geometry of a two-step forward launch (work items: shot group x 64-column
+// block x row segment); returns its workgroup count, gather workgroups
+// included.
+int pair_geometry(StepParams &p, int gather_pairs_count)
+{
+    static const int seg_env = std::max(2, env_int("SWFD_SEG_ROWS", kSegRows));
+    static const int wpc_env = env_int("SWFD_TB_WAVES_PER_CU", 256);
     static const int spg_override = env_int("SWFD_TB_SHOTS_PER_GROUP", 0);
+    const TuneEntry *t = find_tune(0, p.ny, p.nx);
+    const int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    const int waves_per_cu = t != nullptr && t->param > 0 ? t->param : wpc_env;
     p.n_cb = (p.nx + kLanes - 1) / kLanes;
     p.seg = seg_rows;
     p.n_seg = std::max(1, p.ny / seg_rows);
@@ -1654,28 +1718,49 @@ int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
     p.chunk = (blocks + kXcds - 1) / kXcds;
     p.n_stencil_blocks = p.chunk * kXcds;
     const int gather_blocks = gather_pairs_count > 0 ? (gather_pairs_count + 255) / 256 : 0;
-    const dim3 block(kLanes * kWavesPerBlock);
-    const dim3 grid(p.n_stencil_blocks + gather_blocks);
+    return p.n_stencil_blocks + gather_blocks;
+}
+
+unsigned pair_flags(const StepParams &p, bool gather)
+{
     unsigned f = kPoint;
     if (p.f_out2 != nullptr || p.f_out2_b != nullptr) f |= kOut2;
-    if (gather_blocks > 0) f |= kGather;
-    if (p.src_rows == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
-        return fail(SWFD_EINVAL, "two-step launch needs point sources and two outputs");
+    if (gather) f |= kGather;
+    return f;
+}
+
+template <typename K>
+void launch_fwd2_variant(unsigned f, dim3 grid, hipStream_t st, K &&launch)
+{
     switch (f) {
-        case kPoint:
-            hipLaunchKernelGGL((fwd2_kernel<kPoint>), grid, block, 0, stream, p);
-            break;
+        case kPoint: launch(fwd2_kernel<kPoint>, fwd2_batch_kernel<kPoint>); break;
         case kPoint | kGather:
-            hipLaunchKernelGGL((fwd2_kernel<kPoint | kGather>), grid, block, 0, stream, p);
+            launch(fwd2_kernel<kPoint | kGather>, fwd2_batch_kernel<kPoint | kGather>);
             break;
         case kPoint | kOut2:
-            hipLaunchKernelGGL((fwd2_kernel<kPoint | kOut2>), grid, block, 0, stream, p);
+            launch(fwd2_kernel<kPoint | kOut2>, fwd2_batch_kernel<kPoint | kOut2>);
             break;
         default:
-            hipLaunchKernelGGL((fwd2_kernel<kPoint | kOut2 | kGather>), grid, block, 0, stream,
-                               p);
+            launch(fwd2_kernel<kPoint | kOut2 | kGather>,
+                   fwd2_batch_kernel<kPoint | kOut2 | kGather>);
             break;
     }
+    (void)grid;
+    (void)st;
+}
+
+// Two forward steps in one launch: outputs p.f_out (first step) and
+// p.f_out_b (second); p.f_out2 / p.f_out2_b optional copies.
+int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
+{
+    if (p.src_rows == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
+        return fail(SWFD_EINVAL, "two-step launch needs point sources and two outputs");
+    const dim3 grid(pair_geometry(p, gather_pairs_count));
+    const dim3 block(kLanes * kWavesPerBlock);
+    const unsigned f = pair_flags(p, gather_pairs_count > 0);
+    launch_fwd2_variant(f, grid, stream, [&](auto single, auto) {
+        hipLaunchKernelGGL(single, grid, block, 0, stream, p);
+    });
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
 }
@@ -1690,14 +1775,19 @@ struct Adj2Shape {
 
 Adj2Shape adj2_shape(int ny, int nx, int n_shots)
 {
-    static const int seg_rows = std::max(2, env_int("SWFD_ADJ_SEG_ROWS", 32));
+    static const int seg_env = std::max(2, env_int("SWFD_ADJ_SEG_ROWS", 32));
     static const int groups_env = env_int("SWFD_ADJ_GROUPS", kMaxImageGroups);
+    const TuneEntry *t = find_tune(1, ny, nx);
+    int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    const int groups_param = t != nullptr && t->param > 0 ? t->param : groups_env;
+    // the imaging accumulator (4 wavefronts x rows x 64 floats) must fit 64 KiB
+    seg_rows = std::min(seg_rows, 32);
     Adj2Shape a{};
     a.n_cb = (nx + kLanes - 1) / kLanes;
     a.seg = seg_rows;
     a.n_seg = std::max(1, ny / seg_rows);
     a.acc_rows = (ny + a.n_seg - 1) / a.n_seg;
-    const int groups = std::max(1, std::min(std::min(groups_env, kMaxImageGroups), n_shots));
+    const int groups = std::max(1, std::min(std::min(groups_param, kMaxImageGroups), n_shots));
     a.shots_per_group = (n_shots + groups - 1) / groups;
     a.shots_per_wave = (a.shots_per_group + kWavesPerBlock - 1) / kWavesPerBlock;
     a.shots_per_group = a.shots_per_wave * kWavesPerBlock;
@@ -1705,10 +1795,9 @@ Adj2Shape adj2_shape(int ny, int nx, int n_shots)
     return a;
 }
 
-// Two adjoint steps in one launch: p.f_out (first step) and p.f_out_b
-// (second), residuals p.resid / p.resid_b, imaging fields p.img / p.img_b
-// (either may be NULL) into p.g_part.
-int launch_adj_pair(StepParams &p, hipStream_t stream)
+// Geometry of a two-step adjoint launch; returns its workgroup count and the
+// LDS bytes it needs (0 without imaging).
+int adj_pair_geometry(StepParams &p, size_t &lds)
 {
     const Adj2Shape a = adj2_shape(p.ny, p.nx, p.n_shots);
     p.n_cb = a.n_cb;
@@ -1720,20 +1809,29 @@ int launch_adj_pair(StepParams &p, hipStream_t stream)
     p.acc_rows = a.acc_rows;
     p.n_work = a.n_cb * a.n_seg * a.n_groups;
     p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    const bool image = p.img != nullptr || p.img_b != nullptr;
+    lds = image ? (size_t)kWavesPerBlock * a.acc_rows * kLanes * sizeof(float) : 0;
+    return p.chunk * kXcds;
+}
+
+// Two adjoint steps in one launch: p.f_out (first step) and p.f_out_b
+// (second), residuals p.resid / p.resid_b, imaging fields p.img / p.img_b
+// (either may be NULL) into p.g_part.
+int launch_adj_pair(StepParams &p, hipStream_t stream)
+{
     if (p.resid == nullptr || p.resid_b == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
         return fail(SWFD_EINVAL, "two-step adjoint launch needs residuals and two outputs");
     const bool image = p.img != nullptr || p.img_b != nullptr;
     if (image && p.g_part == nullptr)
         return fail(SWFD_EINVAL, "two-step adjoint imaging needs the partial images");
+    size_t lds = 0;
+    const dim3 grid((unsigned)adj_pair_geometry(p, lds));
     const dim3 block(kLanes * kWavesPerBlock);
-    const dim3 grid((unsigned)(p.chunk * kXcds));
-    if (image) {
-        const size_t lds = (size_t)kWavesPerBlock * a.acc_rows * kLanes * sizeof(float);
-        if (lds > 64 * 1024) return fail(SWFD_EINVAL, "two-step adjoint: row segments too long");
+    if (lds > 64 * 1024) return fail(SWFD_EINVAL, "two-step adjoint: row segments too long");
+    if (image)
         hipLaunchKernelGGL((adj2_kernel<kResid | kImage>), grid, block, lds, stream, p);
-    } else {
+    else
         hipLaunchKernelGGL((adj2_kernel<kResid>), grid, block, 0, stream, p);
-    }
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
 }
@@ -1833,6 +1931,516 @@ float *free_work(float *const work[4], const float *a, const float *b,
     for (int i = 0; i < 4; ++i)
         if (work[i] != nullptr && work[i] != a && work[i] != b && work[i] != c) return work[i];
     return nullptr;
+}
+
+
+// ---------------------------------------------------------------------------
+// Runs as state machines: one run = the launches of one swfd_forward or
+// swfd_adjoint call. swfd_forward / swfd_adjoint drive one run; the batch
+// entry points drive several in lockstep, packing the two-step launches of
+// all runs at each tick into one batched kernel launch.
+// ---------------------------------------------------------------------------
+
+int check_forward_args(const swfd_grid *grid, const swfd_medium *medium,
+                       const swfd_receivers *rec, const swfd_forward_args *a)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
+    if (a == nullptr) return fail(SWFD_EINVAL, "forward args are NULL");
+    if (a->n_shots < 1 || a->n_steps < 0 || a->step_begin < 0)
+        return fail(SWFD_EINVAL, "forward: bad n_shots / n_steps / step_begin");
+    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
+        a->work1 == nullptr || a->work2 == nullptr)
+        return fail(SWFD_EINVAL, "forward: state or work buffers are NULL");
+    if (a->src_rows == nullptr || a->src_cols == nullptr || a->src_values == nullptr)
+        return fail(SWFD_EINVAL, "forward: point sources are required");
+    if (a->samples != nullptr && (rc = check_receivers(rec, false))) return rc;
+    if (a->hist != nullptr && (a->hist_every < 1 || a->hist_slots < 1))
+        return fail(SWFD_EINVAL, "forward: hist_every and hist_slots must be positive");
+    if (a->ckpt != nullptr && (a->ckpt_every < 2 || a->ckpt_slots < 2))
+        return fail(SWFD_EINVAL, "forward: ckpt_every must be >= 2 and ckpt_slots >= 2");
+    return SWFD_OK;
+}
+
+bool two_step_allowed(int ny, int nx, const float *work3)
+{
+    static const bool env = env_int("SWFD_TWO_STEP", 1) != 0;
+    const TuneEntry *t = find_tune(2, ny, nx);
+    return env && work3 != nullptr && (t == nullptr || t->depth != 1);
+}
+
+struct ForwardRun {
+    const swfd_grid *grid = nullptr;
+    const swfd_medium *medium = nullptr;
+    const swfd_receivers *rec = nullptr;
+    const swfd_forward_args *a = nullptr;
+    size_t field = 0, sr = 0;
+    float *work[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool two_step = false;
+    const float *p_old = nullptr, *p_cur = nullptr;
+    int k = 0;
+    // fields whose receiver samples are still to be taken (the last one or
+    // two outputs); the next launch, which reads them anyway, gathers them
+    const float *pend_field[2] = {nullptr, nullptr};
+    int pend_step[2] = {0, 0};
+    int n_pend = 0;
+
+    int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
+             const swfd_forward_args *args)
+    {
+        int rc;
+        if ((rc = check_forward_args(g, m, r, args))) return rc;
+        grid = g;
+        medium = m;
+        rec = r;
+        a = args;
+        field = (size_t)g->ny * g->nx * a->n_shots;
+        sr = (size_t)a->n_shots * (a->samples != nullptr ? r->n : 0);
+        work[0] = a->work0;
+        work[1] = a->work1;
+        work[2] = a->work2;
+        work[3] = a->work3;
+        two_step = two_step_allowed(g->ny, g->nx, a->work3);
+        p_old = a->init_old;
+        p_cur = a->init_u;
+        return SWFD_OK;
+    }
+    bool done() const { return k >= a->n_steps; }
+    bool next_is_pair() const { return two_step && k + 1 < a->n_steps; }
+    // output of step n: its history slot, else a work buffer chosen later
+    float *hist_out(int n) const
+    {
+        if (a->hist != nullptr && n >= a->hist_first && (n - a->hist_first) % a->hist_every == 0) {
+            const int slot = (n - a->hist_first) / a->hist_every;
+            if (slot < a->hist_slots) return a->hist + (size_t)slot * field;
+        }
+        return nullptr;
+    }
+    // checkpoint copy of the output of step n: the state before step jK is
+    // (output of jK-2, output of jK-1)
+    float *ckpt_out(int n) const
+    {
+        if (a->ckpt == nullptr) return nullptr;
+        const int K = a->ckpt_every;
+        const int j2 = n + 2, j1 = n + 1;
+        int slot = -1;
+        if (j2 % K == 0 && j2 / K >= 1) slot = 2 * (j2 / K - 1);
+        else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
+        return (slot >= 0 && slot < a->ckpt_slots) ? a->ckpt + (size_t)slot * field : nullptr;
+    }
+    // Parameters of the next launch (a pair when next_is_pair()); returns the
+    // number of fused gathers (pairs of shot and receiver).
+    int build(StepParams &p)
+    {
+        const int n = a->step_begin + k;
+        base_params(p, grid, medium, a->n_shots);
+        attach_receivers(p, a->samples != nullptr ? rec : nullptr);
+        p.f_old = p_old;
+        p.f_cur = p_cur;
+        p.src_rows = a->src_rows;
+        p.src_cols = a->src_cols;
+        p.src_values = a->src_values;
+        p.src_index = n;
+        int gathers = 0;
+        if (a->samples != nullptr && n_pend > 0) {
+            p.gat_field = pend_field[0];
+            p.gat_out = a->samples + (size_t)pend_step[0] * sr;
+            if (n_pend == 2) {
+                p.gat_field2 = pend_field[1];
+                p.gat_out2 = a->samples + (size_t)pend_step[1] * sr;
+            }
+            gathers = n_pend * a->n_shots * rec->n;
+        }
+        if (next_is_pair()) {
+            float *out_c = hist_out(n);
+            if (out_c == nullptr) out_c = free_work(work, p_old, p_cur);
+            float *out_d = hist_out(n + 1);
+            if (out_d == nullptr) out_d = free_work(work, p_old, p_cur, out_c);
+            p.f_out = out_c;
+            p.f_out_b = out_d;
+            p.f_out2 = ckpt_out(n);
+            p.f_out2_b = ckpt_out(n + 1);
+        } else {
+            float *out = hist_out(n);
+            if (out == nullptr) out = free_work(work, p_old, p_cur);
+            p.f_out = out;
+            p.f_out2 = ckpt_out(n);
+        }
+        return gathers;
+    }
+    void advance(const StepParams &p, bool pair)
+    {
+        const int n = a->step_begin + k;
+        if (pair) {
+            pend_field[0] = p.f_out;
+            pend_step[0] = n;
+            pend_field[1] = p.f_out_b;
+            pend_step[1] = n + 1;
+            n_pend = 2;
+            p_old = p.f_out;
+            p_cur = p.f_out_b;
+            k += 2;
+        } else {
+            pend_field[0] = p.f_out;
+            pend_step[0] = n;
+            n_pend = 1;
+            p_old = p_cur;
+            p_cur = p.f_out;
+            k += 1;
+        }
+    }
+    int finish(void *stream)
+    {
+        int rc;
+        if (a->samples != nullptr) {
+            for (int i = 0; i < n_pend; ++i)
+                if ((rc = swfd_gather(grid, rec, a->n_shots, pend_field[i],
+                                      a->samples + (size_t)pend_step[i] * sr, stream)))
+                    return rc;
+        }
+        return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field,
+                            as_stream(stream));
+    }
+};
+
+int check_adjoint_args(const swfd_grid *grid, const swfd_medium *medium,
+                       const swfd_receivers *rec, const swfd_adjoint_args *a)
+{
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
+    if (a == nullptr) return fail(SWFD_EINVAL, "adjoint args are NULL");
+    if (a->n_shots < 1 || a->step_lo < 0 || a->step_hi < a->step_lo)
+        return fail(SWFD_EINVAL, "adjoint: bad n_shots / step range");
+    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
+        a->work1 == nullptr || a->work2 == nullptr)
+        return fail(SWFD_EINVAL, "adjoint: state or work buffers are NULL");
+    if (a->residual == nullptr) return fail(SWFD_EINVAL, "adjoint: residual is NULL");
+    if ((rc = check_receivers(rec, true))) return rc;
+    if (a->hist != nullptr && (a->grad == nullptr || a->hist_every < 1 || a->hist_slots < 1))
+        return fail(SWFD_EINVAL, "adjoint: imaging needs grad, hist_every and hist_slots");
+    if (a->hist != nullptr && a->workspace == nullptr)
+        return fail(SWFD_EINVAL, "adjoint: imaging needs the workspace");
+    return SWFD_OK;
+}
+
+// Step m images lam' (after step m) against the forward field after step
+// m-1. Two-step launches image their own outputs (steps n and n-1); a single
+// step images the step before it (its input lam), and the last step is
+// imaged after the run. With two-step launches only the last step (when the
+// count is odd) runs alone, so no step is imaged twice.
+struct AdjointRun {
+    const swfd_grid *grid = nullptr;
+    const swfd_medium *medium = nullptr;
+    const swfd_receivers *rec = nullptr;
+    const swfd_adjoint_args *a = nullptr;
+    size_t field = 0, plane = 0, sr = 0;
+    float *work[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool two_step = false;
+    int n_part = 0;
+    bool partials_used = false;
+    const float *p_old = nullptr, *p_cur = nullptr;
+    int n = 0;  // next step to run (descending)
+
+    int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
+             const swfd_adjoint_args *args)
+    {
+        int rc;
+        if ((rc = check_adjoint_args(g, m, r, args))) return rc;
+        grid = g;
+        medium = m;
+        rec = r;
+        a = args;
+        field = (size_t)g->ny * g->nx * a->n_shots;
+        plane = (size_t)g->ny * g->nx;
+        sr = (size_t)a->n_shots * r->n;
+        work[0] = a->work0;
+        work[1] = a->work1;
+        work[2] = a->work2;
+        work[3] = a->work3;
+        two_step = two_step_allowed(g->ny, g->nx, a->work3);
+        n_part = two_step ? adj2_shape(g->ny, g->nx, a->n_shots).n_groups
+                          : image_groups(g->ny, g->nx, a->n_shots);
+        p_old = a->init_old;
+        p_cur = a->init_u;
+        n = a->step_hi - 1;
+        return SWFD_OK;
+    }
+    bool done() const { return n < a->step_lo; }
+    bool next_is_pair() const { return two_step && n - 1 >= a->step_lo; }
+    long image_slot(int m) const
+    {
+        if (a->hist == nullptr || m < 1) return -1;
+        const int j = m - 1;
+        if (j < a->hist_first || (j - a->hist_first) % a->hist_every != 0) return -1;
+        return (j - a->hist_first) / a->hist_every;
+    }
+    // Parameters of the next launch; `fill` is set when the imaging partials
+    // must be cleared before it (first imaging launch of the run).
+    int build(StepParams &p, bool &fill)
+    {
+        fill = false;
+        base_params(p, grid, medium, a->n_shots);
+        attach_receivers(p, rec);
+        p.f_old = p_old;
+        p.f_cur = p_cur;
+        p.resid = a->residual + (size_t)n * sr;
+        auto use_partials = [&]() {
+            if (!partials_used) fill = true;
+            partials_used = true;
+            p.g_part = a->workspace;
+            p.img_scale = a->image_scale;
+        };
+        if (next_is_pair()) {
+            float *out_c = free_work(work, p_old, p_cur);
+            float *out_d = free_work(work, p_old, p_cur, out_c);
+            if (out_c == nullptr || out_d == nullptr)
+                return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+            p.f_out = out_c;
+            p.f_out_b = out_d;
+            p.resid_b = a->residual + (size_t)(n - 1) * sr;
+            const long s1 = image_slot(n), s2 = image_slot(n - 1);
+            if (s1 >= a->hist_slots || s2 >= a->hist_slots)
+                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+            if (s1 >= 0) p.img = a->hist + (size_t)s1 * field;
+            if (s2 >= 0) p.img_b = a->hist + (size_t)s2 * field;
+            if (s1 >= 0 || s2 >= 0) use_partials();
+            return SWFD_OK;
+        }
+        float *out = free_work(work, p_old, p_cur);
+        if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+        p.f_out = out;
+        if (!two_step && n < a->step_hi - 1) {
+            const long slot = image_slot(n + 1);
+            if (slot >= a->hist_slots)
+                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+            if (slot >= 0) {
+                p.img = a->hist + (size_t)slot * field;
+                use_partials();
+            }
+        }
+        return SWFD_OK;
+    }
+    void advance(const StepParams &p, bool pair)
+    {
+        if (pair) {
+            p_old = p.f_out;
+            p_cur = p.f_out_b;
+            n -= 2;
+        } else {
+            p_old = p_cur;
+            p_cur = p.f_out;
+            n -= 1;
+        }
+    }
+    int launch_fill(hipStream_t st) const
+    {
+        const size_t nz = (size_t)n_part * plane;
+        hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st,
+                           a->workspace, nz, 0.0f);
+        SWFD_HIP_CHECK(hipGetLastError());
+        return SWFD_OK;
+    }
+    int finish(hipStream_t st)
+    {
+        int rc;
+        if (partials_used) {
+            hipLaunchKernelGGL(reduce_groups_kernel, dim3((unsigned)((plane + 255) / 256)),
+                               dim3(256), 0, st, plane, n_part, a->workspace, a->grad);
+            SWFD_HIP_CHECK(hipGetLastError());
+        }
+        // the last step, unless a two-step launch imaged it
+        const bool last_imaged = two_step && (a->step_hi - a->step_lo) % 2 == 0;
+        if (a->step_hi > a->step_lo && !last_imaged) {
+            const long slot = image_slot(a->step_lo);
+            if (slot >= a->hist_slots)
+                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+            if (slot >= 0 &&
+                (rc = launch_image(grid, a->n_shots, p_cur, a->hist + (size_t)slot * field,
+                                   a->grad, a->grad, a->image_scale, st)))
+                return rc;
+        }
+        return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+    }
+};
+
+// Batched tables: per tick, the StepParams of its entries and the first
+// workgroup of each entry (n + 1 ints), each block aligned to 256 bytes.
+constexpr size_t kTableAlign = 256;
+
+size_t align_up(size_t v) { return (v + kTableAlign - 1) / kTableAlign * kTableAlign; }
+
+struct TickTable {
+    std::vector<StepParams> entries;
+    std::vector<int> first;
+    size_t lds = 0;
+    unsigned flags = 0;
+    size_t bytes() const
+    {
+        return align_up(entries.size() * sizeof(StepParams)) +
+               align_up((entries.size() + 1) * sizeof(int));
+    }
+};
+
+// mode 0: count table bytes; 1: write the tables to `table` (blocking copy);
+// 2: launch (tables already in `table`). Modes 0 and 1 launch nothing.
+int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
+                  const swfd_receivers *recs, const swfd_forward_args *args, int mode,
+                  void *table, size_t table_bytes, size_t *bytes_out, void *stream)
+{
+    int rc;
+    if (n_runs < 1 || grids == nullptr || media == nullptr || args == nullptr)
+        return fail(SWFD_EINVAL, "forward batch: bad arguments");
+    std::vector<ForwardRun> runs((size_t)n_runs);
+    for (int i = 0; i < n_runs; ++i)
+        if ((rc = runs[i].init(&grids[i], &media[i], recs != nullptr ? &recs[i] : nullptr,
+                               &args[i])))
+            return rc;
+    hipStream_t st = as_stream(stream);
+    std::vector<char> host;
+    size_t off = 0;
+    for (;;) {
+        bool any = false;
+        TickTable t;
+        t.first.push_back(0);
+        std::vector<int> pair_runs;
+        for (int i = 0; i < n_runs; ++i) {
+            ForwardRun &r = runs[i];
+            if (r.done()) continue;
+            any = true;
+            StepParams p;
+            const int gathers = r.build(p);
+            if (r.next_is_pair()) {
+                const int blocks = pair_geometry(p, gathers);
+                t.flags |= pair_flags(p, gathers > 0);
+                t.entries.push_back(p);
+                t.first.push_back(t.first.back() + blocks);
+                pair_runs.push_back(i);
+            } else {
+                if (mode == 2 && (rc = launch_step(p, false, st, gathers))) return rc;
+                r.advance(p, false);
+            }
+        }
+        if (!any) break;
+        if (!t.entries.empty()) {
+            const size_t nb = t.bytes();
+            if (mode == 1) {
+                host.resize(off + nb);
+                std::memcpy(host.data() + off, t.entries.data(), t.entries.size() * sizeof(StepParams));
+                std::memcpy(host.data() + off + align_up(t.entries.size() * sizeof(StepParams)),
+                            t.first.data(), t.first.size() * sizeof(int));
+            } else if (mode == 2) {
+                if (off + nb > table_bytes)
+                    return fail(SWFD_EINVAL, "forward batch: table buffer too small");
+                const char *base = static_cast<const char *>(table) + off;
+                const StepParams *tab = reinterpret_cast<const StepParams *>(base);
+                const int *first = reinterpret_cast<const int *>(
+                    base + align_up(t.entries.size() * sizeof(StepParams)));
+                const int ne = (int)t.entries.size();
+                const dim3 grid((unsigned)t.first.back());
+                const dim3 block(kLanes * kWavesPerBlock);
+                launch_fwd2_variant(t.flags, grid, st, [&](auto, auto batched) {
+                    hipLaunchKernelGGL(batched, grid, block, 0, st, tab, first, ne);
+                });
+                SWFD_HIP_CHECK(hipGetLastError());
+            }
+            off += nb;
+            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j], true);
+        }
+    }
+    if (bytes_out != nullptr) *bytes_out = off;
+    if (mode == 1) {
+        if (off > table_bytes) return fail(SWFD_EINVAL, "forward batch: table buffer too small");
+        if (off > 0) SWFD_HIP_CHECK(hipMemcpy(table, host.data(), off, hipMemcpyHostToDevice));
+    }
+    if (mode == 2)
+        for (ForwardRun &r : runs)
+            if ((rc = r.finish(stream))) return rc;
+    return SWFD_OK;
+}
+
+int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
+                  const swfd_receivers *recs, const swfd_adjoint_args *args, int mode,
+                  void *table, size_t table_bytes, size_t *bytes_out, void *stream)
+{
+    int rc;
+    if (n_runs < 1 || grids == nullptr || media == nullptr || recs == nullptr || args == nullptr)
+        return fail(SWFD_EINVAL, "adjoint batch: bad arguments");
+    std::vector<AdjointRun> runs((size_t)n_runs);
+    for (int i = 0; i < n_runs; ++i)
+        if ((rc = runs[i].init(&grids[i], &media[i], &recs[i], &args[i]))) return rc;
+    hipStream_t st = as_stream(stream);
+    std::vector<char> host;
+    size_t off = 0;
+    for (;;) {
+        bool any = false;
+        TickTable t;
+        t.first.push_back(0);
+        std::vector<int> pair_runs;
+        for (int i = 0; i < n_runs; ++i) {
+            AdjointRun &r = runs[i];
+            if (r.done()) continue;
+            any = true;
+            StepParams p;
+            bool fill = false;
+            if ((rc = r.build(p, fill))) return rc;
+            if (mode == 2 && fill && (rc = r.launch_fill(st))) return rc;
+            if (r.next_is_pair()) {
+                size_t lds = 0;
+                const int blocks = adj_pair_geometry(p, lds);
+                // the batched launch runs every entry with the imaging variant
+                // when any entry images, so size the LDS for every entry
+                const size_t acc = (size_t)kWavesPerBlock * p.acc_rows * kLanes * sizeof(float);
+                if (acc > 64 * 1024)
+                    return fail(SWFD_EINVAL, "two-step adjoint: row segments too long");
+                if (lds > 0) t.flags |= kImage;
+                t.lds = std::max(t.lds, acc);
+                t.entries.push_back(p);
+                t.first.push_back(t.first.back() + blocks);
+                pair_runs.push_back(i);
+            } else {
+                if (mode == 2 && (rc = launch_step(p, true, st, 0))) return rc;
+                r.advance(p, false);
+            }
+        }
+        if (!any) break;
+        if (!t.entries.empty()) {
+            const size_t nb = t.bytes();
+            if (mode == 1) {
+                host.resize(off + nb);
+                std::memcpy(host.data() + off, t.entries.data(), t.entries.size() * sizeof(StepParams));
+                std::memcpy(host.data() + off + align_up(t.entries.size() * sizeof(StepParams)),
+                            t.first.data(), t.first.size() * sizeof(int));
+            } else if (mode == 2) {
+                if (off + nb > table_bytes)
+                    return fail(SWFD_EINVAL, "adjoint batch: table buffer too small");
+                const char *base = static_cast<const char *>(table) + off;
+                const StepParams *tab = reinterpret_cast<const StepParams *>(base);
+                const int *first = reinterpret_cast<const int *>(
+                    base + align_up(t.entries.size() * sizeof(StepParams)));
+                const int ne = (int)t.entries.size();
+                const dim3 grid((unsigned)t.first.back());
+                const dim3 block(kLanes * kWavesPerBlock);
+                if ((t.flags & kImage) != 0)
+                    hipLaunchKernelGGL((adj2_batch_kernel<kResid | kImage>), grid, block, t.lds, st,
+                                       tab, first, ne);
+                else
+                    hipLaunchKernelGGL((adj2_batch_kernel<kResid>), grid, block, 0, st, tab, first,
+                                       ne);
+                SWFD_HIP_CHECK(hipGetLastError());
+            }
+            off += nb;
+            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j], true);
+        }
+    }
+    if (bytes_out != nullptr) *bytes_out = off;
+    if (mode == 1) {
+        if (off > table_bytes) return fail(SWFD_EINVAL, "adjoint batch: table buffer too small");
+        if (off > 0) SWFD_HIP_CHECK(hipMemcpy(table, host.data(), off, hipMemcpyHostToDevice));
+    }
+    if (mode == 2)
+        for (AdjointRun &r : runs)
+            if ((rc = r.finish(st))) return rc;
+    return SWFD_OK;
 }
 
 }  // namespace
@@ -1975,236 +2583,100 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
                  const swfd_forward_args *a, void *stream)
 {
     int rc;
-    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
-    if (a == nullptr) return fail(SWFD_EINVAL, "forward args are NULL");
-    if (a->n_shots < 1 || a->n_steps < 0 || a->step_begin < 0)
-        return fail(SWFD_EINVAL, "forward: bad n_shots / n_steps / step_begin");
-    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
-        a->work1 == nullptr || a->work2 == nullptr)
-        return fail(SWFD_EINVAL, "forward: state or work buffers are NULL");
-    if (a->src_rows == nullptr || a->src_cols == nullptr || a->src_values == nullptr)
-        return fail(SWFD_EINVAL, "forward: point sources are required");
-    if (a->samples != nullptr && (rc = check_receivers(rec, false))) return rc;
-    if (a->hist != nullptr && (a->hist_every < 1 || a->hist_slots < 1))
-        return fail(SWFD_EINVAL, "forward: hist_every and hist_slots must be positive");
-    if (a->ckpt != nullptr && (a->ckpt_every < 2 || a->ckpt_slots < 2))
-        return fail(SWFD_EINVAL, "forward: ckpt_every must be >= 2 and ckpt_slots >= 2");
+    ForwardRun run;
+    if ((rc = run.init(grid, medium, rec, a))) return rc;
     hipStream_t st = as_stream(stream);
-    const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
-    const size_t sr = (size_t)a->n_shots * (a->samples != nullptr ? rec->n : 0);
-    float *const work[4] = {a->work0, a->work1, a->work2, a->work3};
-    static const bool two_step_env = env_int("SWFD_TWO_STEP", 1) != 0;
-    const bool two_step = two_step_env && a->work3 != nullptr;
-    // output of step n: its history slot, else a work buffer chosen later
-    auto hist_out = [&](int n) -> float * {
-        if (a->hist != nullptr && n >= a->hist_first && (n - a->hist_first) % a->hist_every == 0) {
-            const int slot = (n - a->hist_first) / a->hist_every;
-            if (slot < a->hist_slots) return a->hist + (size_t)slot * field;
-        }
-        return nullptr;
-    };
-    // checkpoint copy of the output of step n: the state before step jK is
-    // (output of jK-2, output of jK-1)
-    auto ckpt_out = [&](int n) -> float * {
-        if (a->ckpt == nullptr) return nullptr;
-        const int K = a->ckpt_every;
-        const int j2 = n + 2, j1 = n + 1;
-        int slot = -1;
-        if (j2 % K == 0 && j2 / K >= 1) slot = 2 * (j2 / K - 1);
-        else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
-        return (slot >= 0 && slot < a->ckpt_slots) ? a->ckpt + (size_t)slot * field : nullptr;
-    };
-    // fields whose receiver samples are still to be taken (the last one or two
-    // outputs); they are gathered by the next launch, which reads them anyway
-    const float *pend_field[2] = {nullptr, nullptr};
-    int pend_step[2] = {0, 0};
-    int n_pend = 0;
-    auto attach_gathers = [&](StepParams &p) -> int {
-        if (a->samples == nullptr || n_pend == 0) return 0;
-        p.gat_field = pend_field[0];
-        p.gat_out = a->samples + (size_t)pend_step[0] * sr;
-        if (n_pend == 2) {
-            p.gat_field2 = pend_field[1];
-            p.gat_out2 = a->samples + (size_t)pend_step[1] * sr;
-        }
-        return n_pend * a->n_shots * rec->n;
-    };
-    const float *p_old = a->init_old;
-    const float *p_cur = a->init_u;
-    int k = 0;
-    while (k < a->n_steps) {
-        const int n = a->step_begin + k;
+    while (!run.done()) {
         StepParams p;
-        base_params(p, grid, medium, a->n_shots);
-        attach_receivers(p, a->samples != nullptr ? rec : nullptr);
-        p.f_old = p_old;
-        p.f_cur = p_cur;
-        p.src_rows = a->src_rows;
-        p.src_cols = a->src_cols;
-        p.src_values = a->src_values;
-        p.src_index = n;
-        const int gather_count = attach_gathers(p);
-        if (two_step && k + 1 < a->n_steps) {
-            float *out_c = hist_out(n);
-            if (out_c == nullptr) out_c = free_work(work, p_old, p_cur);
-            float *out_d = hist_out(n + 1);
-            if (out_d == nullptr) out_d = free_work(work, p_old, p_cur, out_c);
-            p.f_out = out_c;
-            p.f_out_b = out_d;
-            p.f_out2 = ckpt_out(n);
-            p.f_out2_b = ckpt_out(n + 1);
-            if ((rc = launch_pair(p, st, gather_count))) return rc;
-            pend_field[0] = out_c;
-            pend_step[0] = n;
-            pend_field[1] = out_d;
-            pend_step[1] = n + 1;
-            n_pend = 2;
-            p_old = out_c;
-            p_cur = out_d;
-            k += 2;
-        } else {
-            float *out = hist_out(n);
-            if (out == nullptr) out = free_work(work, p_old, p_cur);
-            p.f_out = out;
-            p.f_out2 = ckpt_out(n);
-            if ((rc = launch_step(p, false, st, gather_count))) return rc;
-            pend_field[0] = out;
-            pend_step[0] = n;
-            n_pend = 1;
-            p_old = p_cur;
-            p_cur = out;
-            k += 1;
-        }
+        const int gathers = run.build(p);
+        const bool pair = run.next_is_pair();
+        if ((rc = pair ? launch_pair(p, st, gathers) : launch_step(p, false, st, gathers)))
+            return rc;
+        run.advance(p, pair);
     }
-    if (a->samples != nullptr) {
-        for (int i = 0; i < n_pend; ++i)
-            if ((rc = swfd_gather(grid, rec, a->n_shots, pend_field[i],
-                                  a->samples + (size_t)pend_step[i] * sr, stream)))
-                return rc;
-    }
-    return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+    return run.finish(stream);
 }
 
 int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
                  const swfd_adjoint_args *a, void *stream)
 {
     int rc;
-    if ((rc = check_grid(grid)) || (rc = check_medium(medium))) return rc;
-    if (a == nullptr) return fail(SWFD_EINVAL, "adjoint args are NULL");
-    if (a->n_shots < 1 || a->step_lo < 0 || a->step_hi < a->step_lo)
-        return fail(SWFD_EINVAL, "adjoint: bad n_shots / step range");
-    if (a->init_old == nullptr || a->init_u == nullptr || a->work0 == nullptr ||
-        a->work1 == nullptr || a->work2 == nullptr)
-        return fail(SWFD_EINVAL, "adjoint: state or work buffers are NULL");
-    if (a->residual == nullptr) return fail(SWFD_EINVAL, "adjoint: residual is NULL");
-    if ((rc = check_receivers(rec, true))) return rc;
-    if (a->hist != nullptr && (a->grad == nullptr || a->hist_every < 1 || a->hist_slots < 1))
-        return fail(SWFD_EINVAL, "adjoint: imaging needs grad, hist_every and hist_slots");
-    if (a->hist != nullptr && a->workspace == nullptr)
-        return fail(SWFD_EINVAL, "adjoint: imaging needs the workspace");
+    AdjointRun run;
+    if ((rc = run.init(grid, medium, rec, a))) return rc;
     hipStream_t st = as_stream(stream);
-    const size_t field = (size_t)grid->ny * grid->nx * a->n_shots;
-    const size_t plane = (size_t)grid->ny * grid->nx;
-    static const bool two_step_env = env_int("SWFD_TWO_STEP", 1) != 0;
-    const bool two_step = two_step_env && a->work3 != nullptr;
-    const int n_part = two_step ? adj2_shape(grid->ny, grid->nx, a->n_shots).n_groups
-                                : image_groups(grid->ny, grid->nx, a->n_shots);
-    bool partials_used = false;
-    const size_t sr = (size_t)a->n_shots * rec->n;
-    float *const work[4] = {a->work0, a->work1, a->work2, a->work3};
-    const float *p_old = a->init_old;
-    const float *p_cur = a->init_u;
-    // Step m images lam' (after step m) against the forward field after step
-    // m-1. The kernel of step n performs the imaging of step n+1, whose lam' is
-    // its input u; the imaging of step_lo follows the loop.
-    auto image_slot = [&](int m) -> long {
-        if (a->hist == nullptr || m < 1) return -1;
-        const int j = m - 1;
-        if (j < a->hist_first || (j - a->hist_first) % a->hist_every != 0) return -1;
-        return (j - a->hist_first) / a->hist_every;
-    };
-    auto use_partials = [&]() -> int {
-        if (!partials_used) {
-            const size_t nz = (size_t)n_part * plane;
-            hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st,
-                               a->workspace, nz, 0.0f);
-            SWFD_HIP_CHECK(hipGetLastError());
-            partials_used = true;
-        }
-        return SWFD_OK;
-    };
-    // Two-step launches image their own outputs (steps n and n-1); a single
-    // step images the step before it, and the last step is imaged after the
-    // loop. With two-step launches only the last step (when the count is odd)
-    // runs alone, so no step is imaged twice.
-    int n = a->step_hi - 1;
-    while (n >= a->step_lo) {
+    while (!run.done()) {
         StepParams p;
-        base_params(p, grid, medium, a->n_shots);
-        attach_receivers(p, rec);
-        p.f_old = p_old;
-        p.f_cur = p_cur;
-        p.resid = a->residual + (size_t)n * sr;
-        if (two_step && n - 1 >= a->step_lo) {
-            float *out_c = free_work(work, p_old, p_cur);
-            float *out_d = free_work(work, p_old, p_cur, out_c);
-            if (out_c == nullptr || out_d == nullptr)
-                return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
-            p.f_out = out_c;
-            p.f_out_b = out_d;
-            p.resid_b = a->residual + (size_t)(n - 1) * sr;
-            const long s1 = image_slot(n), s2 = image_slot(n - 1);
-            if (s1 >= a->hist_slots || s2 >= a->hist_slots)
-                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
-            if (s1 >= 0) p.img = a->hist + (size_t)s1 * field;
-            if (s2 >= 0) p.img_b = a->hist + (size_t)s2 * field;
-            if (s1 >= 0 || s2 >= 0) {
-                if ((rc = use_partials())) return rc;
-                p.g_part = a->workspace;
-                p.img_scale = a->image_scale;
-            }
-            if ((rc = launch_adj_pair(p, st))) return rc;
-            p_old = out_c;
-            p_cur = out_d;
-            n -= 2;
-            continue;
-        }
-        float *out = free_work(work, p_old, p_cur);
-        if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
-        p.f_out = out;
-        if (!two_step && n < a->step_hi - 1) {
-            const long slot = image_slot(n + 1);
-            if (slot >= a->hist_slots)
-                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
-            if (slot >= 0) {
-                if ((rc = use_partials())) return rc;
-                p.img = a->hist + (size_t)slot * field;
-                p.g_part = a->workspace;
-                p.img_scale = a->image_scale;
-            }
-        }
-        if ((rc = launch_step(p, true, st, 0))) return rc;
-        p_old = p_cur;
-        p_cur = out;
-        n -= 1;
+        bool fill = false;
+        if ((rc = run.build(p, fill))) return rc;
+        if (fill && (rc = run.launch_fill(st))) return rc;
+        const bool pair = run.next_is_pair();
+        if ((rc = pair ? launch_adj_pair(p, st) : launch_step(p, true, st, 0))) return rc;
+        run.advance(p, pair);
     }
-    if (partials_used) {
-        hipLaunchKernelGGL(reduce_groups_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256),
-                           0, st, plane, n_part, a->workspace, a->grad);
-        SWFD_HIP_CHECK(hipGetLastError());
-    }
-    // the last step, unless a two-step launch imaged it
-    const bool last_imaged = two_step && (a->step_hi - a->step_lo) % 2 == 0;
-    if (a->step_hi > a->step_lo && !last_imaged) {
-        const long slot = image_slot(a->step_lo);
-        if (slot >= a->hist_slots) return fail(SWFD_EINVAL, "adjoint: history slot out of range");
-        if (slot >= 0 &&
-            (rc = launch_image(grid, a->n_shots, p_cur, a->hist + (size_t)slot * field, a->grad,
-                               a->grad, a->image_scale, st)))
-            return rc;
-    }
-    return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
+    return run.finish(st);
 }
+
+int swfd_forward_batch_table_bytes(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                                   const swfd_receivers *recs, const swfd_forward_args *args,
+                                   size_t *bytes)
+{
+    return forward_batch(n, grids, media, recs, args, 0, nullptr, 0, bytes, nullptr);
+}
+
+int swfd_forward_batch_prepare(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                               const swfd_receivers *recs, const swfd_forward_args *args,
+                               void *table, size_t table_bytes)
+{
+    return forward_batch(n, grids, media, recs, args, 1, table, table_bytes, nullptr, nullptr);
+}
+
+int swfd_forward_batch(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                       const swfd_receivers *recs, const swfd_forward_args *args,
+                       const void *table, size_t table_bytes, void *stream)
+{
+    return forward_batch(n, grids, media, recs, args, 2, const_cast<void *>(table), table_bytes,
+                         nullptr, stream);
+}
+
+int swfd_adjoint_batch_table_bytes(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                                   const swfd_receivers *recs, const swfd_adjoint_args *args,
+                                   size_t *bytes)
+{
+    return adjoint_batch(n, grids, media, recs, args, 0, nullptr, 0, bytes, nullptr);
+}
+
+int swfd_adjoint_batch_prepare(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                               const swfd_receivers *recs, const swfd_adjoint_args *args,
+                               void *table, size_t table_bytes)
+{
+    return adjoint_batch(n, grids, media, recs, args, 1, table, table_bytes, nullptr, nullptr);
+}
+
+int swfd_adjoint_batch(int32_t n, const swfd_grid *grids, const swfd_medium *media,
+                       const swfd_receivers *recs, const swfd_adjoint_args *args,
+                       const void *table, size_t table_bytes, void *stream)
+{
+    return adjoint_batch(n, grids, media, recs, args, 2, const_cast<void *>(table), table_bytes,
+                         nullptr, stream);
+}
+
+int swfd_tune(int32_t kind, int32_t ny, int32_t nx, int32_t seg_rows, int32_t param,
+              int32_t depth)
+{
+    if (kind < 0 || kind > 2 || ny < 3 || nx < 3)
+        return fail(SWFD_EINVAL, "tune: kind must be 0..2 and the grid at least 3 x 3");
+    for (TuneEntry &t : g_tune)
+        if (t.kind == kind && t.ny == ny && t.nx == nx) {
+            t.seg_rows = seg_rows;
+            t.param = param;
+            t.depth = depth;
+            return SWFD_OK;
+        }
+    g_tune.push_back(TuneEntry{kind, ny, nx, seg_rows, param, depth});
+    return SWFD_OK;
+}
+
+void swfd_tune_clear(void) { g_tune.clear(); }
 
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots)
 {

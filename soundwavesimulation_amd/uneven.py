@@ -1,0 +1,176 @@
+"""Uneven batch of FWI gradient problems on one GPU.
+
+``python -m soundwavesimulation_amd.uneven`` draws 24 independent problems --
+grid from {256^2, 384^2, 512^2, 768^2} at 0.5 mm, a ring of 16-128 elements
+(every element fires), 1500-5000 steps, a seeded ellipse phantom each (c
+1400-1650 m/s) and a 300 kHz Ricker -- simulates their observed data, tunes
+the launch shapes of every grid size on the device (``autotune.tune``), and
+computes all 24 checkpointed gradients (from a homogeneous 1500 m/s start)
+with ``batch.UnevenGradients``: at every tick one batched launch advances
+every problem, and the whole sequence is captured once as a graph and
+replayed.
+
+Reported (one JSON line): throughput in cell-updates per second over the
+graph replays (2 ny nx N S per problem, summed), and per-problem latency
+percentiles from an eager run with an event after each problem's last
+launch. The draw starts at seed 2 and moves to the next seed while the
+problems' device memory (checkpointed gradient: 4 + 2(n_seg - 1) + K + 1
+fields per shot plus traces) exceeds the budget (``--budget-gb``).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .array import RingArray
+from .autotune import tune
+from .batch import GradientProblem, UnevenGradients
+from .experiment import axis_centers, stable_dt
+from .inversion.fwi import default_checkpoint_every, segment_plan
+from .phantoms import ellipse_phantom
+from .sources import ricker
+from .torch_backend import TorchFDTD2D, TorchRingSampler, TorchScalarWave2D, simulate_shots_batch
+
+SIZES = (256, 384, 512, 768)
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def estimate_bytes(h, n_el, n_steps):
+    K = default_checkpoint_every(n_steps)
+    _, n_ckpt, n_hist = segment_plan(n_steps, K)
+    fields = 4 + n_ckpt + n_hist
+    return fields * n_el * h * h * 4 + 3 * n_steps * n_el * n_el * 4
+
+
+def draw(seed, n=24):
+    rng = np.random.default_rng(seed)
+    H = rng.choice(SIZES, n)
+    R = rng.integers(16, 129, n)
+    N = rng.integers(1500, 5001, n)
+    return [(int(h), int(r), int(s)) for h, r, s in zip(H, R, N)]
+
+
+def draw_within(budget_bytes, seed=2, n=24):
+    while True:
+        probs = draw(seed, n)
+        if sum(estimate_bytes(*p) for p in probs) <= budget_bytes:
+            return seed, probs
+        seed += 1
+
+
+def build(spec, index, seed, device, spacing=0.5e-3, f_peak=300e3):
+    h, n_el, n_steps = spec
+    x = axis_centers(h, spacing)
+    radius = 0.43 * h * spacing
+    dt = stable_dt(1650.0, spacing, spacing, cfl=0.3)
+    pulse = torch.as_tensor(ricker(dt, f_peak), dtype=torch.float32)
+    array = RingArray(n_el, radius, x, x)
+    sampler = TorchRingSampler(array, device=device)
+    c_true = ellipse_phantom(h, h, spacing, 8, 1000 * seed + index, radius)
+    truth = TorchFDTD2D(TorchScalarWave2D(c_true, spacing, device=device), dt)
+    observed = torch.empty((n_el, n_el, n_steps), device=device)
+    per = 64
+    for s0 in range(0, n_el, per):
+        sl = slice(s0, min(n_el, s0 + per))
+        tr, _ = simulate_shots_batch(truth, sampler, pulse.to(device), array.rows[sl],
+                                     array.cols[sl], n_steps)
+        observed[sl] = tr
+    del truth
+    solver = TorchFDTD2D(TorchScalarWave2D(np.full((h, h), 1500.0), spacing, device=device), dt)
+    return GradientProblem(solver, sampler, array, pulse, n_steps, observed)
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--problems", type=int, default=24)
+    p.add_argument("--budget-gb", type=float, default=265.0)
+    p.add_argument("--replays", type=int, default=2)
+    p.add_argument("--scale", type=float, default=1.0,
+                   help="multiply step counts (a quick reduced run: e.g. 0.05)")
+    p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--out", default=None)
+    a = p.parse_args(argv)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    seed, specs = draw_within(a.budget_gb * 1e9, n=a.problems)
+    if a.scale != 1.0:
+        specs = [(h, r, max(8, int(round(s * a.scale)))) for h, r, s in specs]
+    est = sum(estimate_bytes(*s) for s in specs)
+    log(f"seed {seed}: {len(specs)} problems, about {est / 1e9:.0f} GB")
+    tuned = {}
+    if not a.no_tune:
+        for h in sorted({s[0] for s in specs}):
+            shots = max(r for hh, r, _ in specs if hh == h)
+            tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log)
+    t0 = time.perf_counter()
+    problems = [build(s, i, seed, dev) for i, s in enumerate(specs)]
+    torch.cuda.synchronize()
+    log(f"built problems and observed data in {time.perf_counter() - t0:.1f} s")
+    ug = UnevenGradients(problems)
+    cells = ug.cells
+    # eager run with per-problem events
+    start = torch.cuda.Event(enable_timing=True)
+    events = {i: torch.cuda.Event(enable_timing=True) for i in range(len(problems))}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    ug.run(events)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    lat = np.array([start.elapsed_time(events[i]) / 1e3 for i in range(len(problems))])
+    log(f"eager: {t_eager:.2f} s, {cells / t_eager / 1e9:.1f} Gcell/s")
+    # graph capture and replays
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        ug.run()
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    t0 = time.perf_counter()
+    with torch.cuda.graph(graph):
+        ug.run()
+    t_capture = time.perf_counter() - t0
+    times = []
+    for r in range(a.replays):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.replay()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"replay {r}: {times[-1]:.2f} s, {cells / times[-1] / 1e9:.1f} Gcell/s")
+    t_best = min(times)
+    report = {
+        "workload": f"uneven_{len(specs)}_fwi_gradients",
+        "seed": seed,
+        "problems": [{"grid": h, "shots": r, "steps": s} for h, r, s in specs],
+        "cell_updates": cells,
+        "graph_seconds": round(t_best, 3),
+        "gcell_s_graph": round(cells / t_best / 1e9, 2),
+        "eager_seconds": round(t_eager, 3),
+        "gcell_s_eager": round(cells / t_eager / 1e9, 2),
+        "capture_seconds": round(t_capture, 3),
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
+                      "p99": round(float(np.percentile(lat, 99)), 3),
+                      "min": round(float(lat.min()), 3), "max": round(float(lat.max()), 3)},
+        "tuned": {str(k): v for k, v in tuned.items()},
+        "memory_estimate_gb": round(est / 1e9, 1),
+    }
+    line = json.dumps(report)
+    print(line, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

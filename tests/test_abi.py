@@ -28,3 +28,43 @@ def test_host_only_entry_points():
     assert lib.swfd_version().decode().startswith("swfd")
     assert lib.swfd_medium_floats(100, 50) == 2 * 5000 + 2 * 100 + 2 * 50 + 8
     assert lib.swfd_residual_workspace_floats(130, 3, 65) == 3 * 2 * 3
+
+
+def test_operator_library_exports_every_declared_function():
+    from soundwavesimulation_amd.surrogate import _native as no_native
+
+    text = open(os.path.join(ROOT, "include", "swno.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = sorted(set(re.findall(r"\b(swno_[a-z_0-9]+)\s*\(", text)))
+    assert declared
+    lib = no_native.load_library()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(no_native.EXPORTS) == declared
+    # host-only sizes
+    g = no_native.GemmArgs(M=10, N=20, K=300, batch=3, k_split=4)
+    assert lib.swno_gemm_workspace_floats(g) == 3 * 4 * 10 * 20
+    assert lib.swno_col_sum_workspace_floats(10000, 7) == 3 * 7
+
+
+def test_batch_table_sizes_are_host_only():
+    """Batch table sizing validates and counts without touching a device."""
+    import ctypes
+
+    lib = _native.load_library()
+    grid = _native.Grid(ny=64, nx=64, inv_dx2=1.0, inv_dy2=1.0, center=-4.0, dt2=1e-3)
+    med = _native.Medium(c2=1, c2dt2=1, mur=1)  # never dereferenced on the host
+    args = _native.ForwardArgs(n_shots=2, step_begin=0, n_steps=7, init_old=8, init_u=8,
+                               work0=16, work1=24, work2=32, work3=40, src_rows=8, src_cols=8,
+                               src_values=8)
+    nbytes = ctypes.c_size_t(0)
+    n = 2
+    rc = lib.swfd_forward_batch_table_bytes(n, (_native.Grid * n)(grid, grid),
+                                            (_native.Medium * n)(med, med),
+                                            (_native.Receivers * n)(),
+                                            (_native.ForwardArgs * n)(args, args),
+                                            ctypes.byref(nbytes))
+    assert rc == 0
+    # 7 steps = 3 two-step ticks (the 7th step runs alone) with 2 entries each
+    assert nbytes.value > 0 and nbytes.value % 256 == 0
+    lib.swfd_tune_clear()
