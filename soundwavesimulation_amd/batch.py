@@ -106,11 +106,15 @@ class _Prepared:
 
 
 class BatchedForward(_Prepared):
-    """Lockstep forward runs; ``runs`` = list of (solver, sampler or None, kwargs)."""
+    """Lockstep forward runs; ``runs`` = list of (solver, sampler or None, kwargs).
+
+    The prepared tables hold raw device pointers, so the object keeps every
+    tensor of ``runs`` alive."""
 
     kind = "forward"
 
     def __init__(self, runs):
+        self._runs = list(runs)
         solvers = [r[0] for r in runs]
         samplers = [r[1] for r in runs]
         args = [_forward_args(r[0], **r[2]) for r in runs]
@@ -118,11 +122,13 @@ class BatchedForward(_Prepared):
 
 
 class BatchedAdjoint(_Prepared):
-    """Lockstep adjoint runs; ``runs`` = list of (solver, sampler, kwargs)."""
+    """Lockstep adjoint runs; ``runs`` = list of (solver, sampler, kwargs) (kept
+    alive, as for ``BatchedForward``)."""
 
     kind = "adjoint"
 
     def __init__(self, runs):
+        self._runs = list(runs)
         solvers = [r[0] for r in runs]
         samplers = [r[1] for r in runs]
         args = [_adjoint_args(**r[2]) for r in runs]

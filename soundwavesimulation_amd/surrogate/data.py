@@ -20,6 +20,7 @@ import time
 import numpy as np
 import torch
 
+from .. import _native
 from ..array import RingArray
 from ..experiment import axis_centers, stable_dt
 from ..inversion.fwi import CheckpointedGradient, _i32, residual_and_misfit
@@ -102,6 +103,92 @@ class FirstGradientGenerator:
         return torch.cat([gm[..., None], coords], dim=-1)
 
 
+class BatchedFirstGradients:
+    """``batch`` phantoms at a time through batched launches.
+
+    All phantoms share the homogeneous start, so each segment of the reverse
+    pass recomputes the start's forward field once and the adjoints of all
+    phantoms (their own residuals, states and gradients) run as one batched
+    launch per tick against it; the observed data of all phantoms are one
+    batched forward run. Results equal ``FirstGradientGenerator`` per phantom.
+    """
+
+    def __init__(self, geometry, device, batch):
+        from ..batch import BatchedAdjoint, BatchedForward
+
+        self.base = FirstGradientGenerator(geometry, device)
+        g, b = geometry, self.base
+        self.g, self.device, self.P = geometry, device, int(batch)
+        S, n, N = g.n_elements, g.n, g.n_steps
+        shape = (S, n, n)
+        e = lambda *sh: torch.empty(sh, device=device, dtype=_F32)  # noqa: E731
+        self.truth = [TorchFDTD2D(TorchScalarWave2D(np.full((n, n), g.c0), g.spacing,
+                                                    device=device), g.dt)
+                      for _ in range(self.P)]
+        self.obs_work = [[e(*shape) for _ in range(4)] for _ in range(self.P)]
+        self.obs = [e(N, S, S) for _ in range(self.P)]
+        rows = _i32(b.rows_np, device)
+        cols = _i32(b.cols_np, device)
+        values = _source_values_tensor(b.pulse, N, device)
+        self.observe = BatchedForward([
+            (self.truth[p], b.sampler, dict(n_shots=S, rows=rows, cols=cols, values=values,
+                                            n_steps=N, init=(w[0], w[1]), work=w,
+                                            samples=self.obs[p]))
+            for p, w in enumerate(self.obs_work)]).prepare()
+        self.residual = [e(N, S, S) for _ in range(self.P)]
+        self.lam = [[e(*shape) for _ in range(4)] for _ in range(self.P)]
+        self.grads = e(self.P, n, n)
+        ws = int(_native.lib().swfd_adjoint_workspace_floats(n, n, S))
+        self.ws = [e(ws) for _ in range(self.P)]
+        eng = b.engine
+        self.adjoint = []
+        for j in range(eng.n_seg - 1, -1, -1):
+            n0 = j * eng.K
+            n1 = min(n0 + eng.K, N)
+            self.adjoint.append(BatchedAdjoint([
+                (b.solver0, b.sampler, dict(
+                    n_shots=S, residual=self.residual[p], step_hi=n1, step_lo=n0,
+                    state=(self.lam[p][0], self.lam[p][1]), work=self.lam[p], hist=eng.hist,
+                    hist_first=n0 - 2, hist_every=1, image_scale=b.solver0.dt2,
+                    grad=self.grads[p], workspace=self.ws[p]))
+                for p in range(self.P)]).prepare())
+        self.eye = torch.eye(S, device=device, dtype=torch.bool)
+
+    def first_gradients(self, speeds):
+        """Gradients (P, n, n) w.r.t. slowness squared for P speed maps."""
+        from ..torch_backend import forward_run
+
+        b, g = self.base, self.g
+        for p, c in enumerate(speeds):
+            self.truth[p].set_c(c)
+            self.truth[p]._sync_medium()
+            self.obs_work[p][0].zero_()
+            self.obs_work[p][1].zero_()
+        self.observe.launch()
+        for p in range(self.P):
+            r = torch.sub(b.samples0, self.obs[p], out=self.residual[p])
+            r.masked_fill_(self.eye.unsqueeze(0), 0.0)  # transmitter's own channel muted
+            self.lam[p][0].zero_()
+            self.lam[p][1].zero_()
+        self.grads.zero_()
+        eng = b.engine
+        for i, j in enumerate(range(eng.n_seg - 1, -1, -1)):
+            n0 = j * eng.K
+            n1 = min(n0 + eng.K, g.n_steps)
+            if j == 0:
+                eng.hist[0].zero_()
+                eng.hist[1].zero_()
+            else:
+                eng.hist[0].copy_(eng.ckpt[2 * (j - 1)])
+                eng.hist[1].copy_(eng.ckpt[2 * (j - 1) + 1])
+            if n1 - 1 - n0 > 0:
+                forward_run(b.solver0, eng.n_shots, eng.rows, eng.cols, eng.values, n1 - 1 - n0,
+                            init=(eng.hist[0], eng.hist[1]), hist=eng.hist, hist_first=n0 - 2,
+                            hist_every=1, step_begin=n0, work=eng.work, final=False)
+            self.adjoint[i].launch()
+        return -self.grads / (b.m0 * b.m0)
+
+
 def target_from_speed(c, c0=1500.0, scale=100.0):
     return (c - c0) / scale
 
@@ -110,14 +197,36 @@ def speed_from_target(t, c0=1500.0, scale=100.0):
     return t * scale + c0
 
 
-def generate(geometry, device, n_samples, seed0=0, log=None, log_every=30.0):
-    """Inputs (n_samples, n, n, 3), targets (n_samples, n, n) on the device, seconds."""
-    gen = FirstGradientGenerator(geometry, device)
+def generate(geometry, device, n_samples, seed0=0, log=None, log_every=30.0, batch=32):
+    """Inputs (n_samples, n, n, 3), targets (n_samples, n, n) on the device, seconds.
+
+    ``batch`` phantoms go through ``BatchedFirstGradients`` together (1: one
+    at a time through ``FirstGradientGenerator``)."""
     n = geometry.n
     inputs = torch.empty((n_samples, n, n, 3), device=device, dtype=_F32)
     targets = torch.empty((n_samples, n, n), device=device, dtype=_F32)
     t0 = time.perf_counter()
     last = t0
+    if batch > 1:
+        bg = BatchedFirstGradients(geometry, device, batch)
+        gen = bg.base
+        for i0 in range(0, n_samples, batch):
+            cs = [geometry.phantom(seed0 + i) for i in range(i0, min(n_samples, i0 + batch))]
+            cs += [cs[-1]] * (batch - len(cs))  # pad the last batch
+            grads = bg.first_gradients(cs)
+            for k in range(min(batch, n_samples - i0)):
+                inputs[i0 + k] = gen.make_input(grads[k])
+                targets[i0 + k] = torch.as_tensor(target_from_speed(cs[k]), dtype=_F32,
+                                                  device=device)
+            now = time.perf_counter()
+            if log is not None and now - last > log_every:
+                torch.cuda.synchronize()
+                log(f"generated {min(n_samples, i0 + batch)}/{n_samples} pairs in "
+                    f"{now - t0:.1f} s")
+                last = now
+        torch.cuda.synchronize()
+        return inputs, targets, time.perf_counter() - t0, gen
+    gen = FirstGradientGenerator(geometry, device)
     for i in range(n_samples):
         c = geometry.phantom(seed0 + i)
         gm, _ = gen.first_gradient(c)

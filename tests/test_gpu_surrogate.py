@@ -182,3 +182,22 @@ def test_adamw_training_reduces_loss():
         op.adamw_step(3e-3, weight_decay=0.0)
     print(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.2 * losses[0]
+
+
+def test_batched_first_gradients_equal_one_at_a_time():
+    from soundwavesimulation_amd.surrogate.data import (BatchedFirstGradients,
+                                                        FirstGradientGenerator, Geometry)
+
+    geo = Geometry(n=64, n_elements=8, radius_m=25e-3, n_steps=300, f_peak_hz=200e3)
+    print("batched first gradients, 3 phantoms", flush=True)
+    cs = [geo.phantom(s) for s in range(3)]
+    bg = BatchedFirstGradients(geo, DEV, 3)
+    gb = bg.first_gradients(cs).clone()
+    gen = FirstGradientGenerator(geo, DEV)
+    for k, c in enumerate(cs):
+        g1, _ = gen.first_gradient(c)
+        torch.cuda.synchronize()
+        assert float(g1.abs().max()) > 0.0
+        err = float((gb[k] - g1).norm() / g1.norm())
+        print(f"  phantom {k}: rel {err:.1e}")
+        assert err <= 1e-6
