@@ -250,7 +250,10 @@ int swfd_adjoint_batch(int32_t n, const swfd_grid *grids, const swfd_medium *med
  *   kind 0: two-step forward; seg_rows = rows per march, param = target
  *           work items per CU;
  *   kind 1: two-step adjoint; seg_rows (<= 32), param = shot groups (<= 8);
- *   kind 2: temporal depth: depth 1 = single-step kernels, 2 = two-step.
+ *   kind 2: temporal depth and form: depth 1 = single-step kernels,
+ *           2 = two-step marches in 64-column blocks, 3 = two-step marches
+ *           in full-width workgroups (rows of up to 1024 columns; the
+ *           default where they fit; batched launches always use blocks).
  * Values <= 0 keep the defaults. swfd_tune_clear forgets every entry. The
  * table is process-wide host state: set it before running (not while another
  * host thread launches). */

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "swfd.h"
@@ -103,6 +104,11 @@ struct StepParams {
     // step, shots per wavefront, rows of the imaging accumulator
     const float *resid_b, *img_b;
     int shots_per_wave, acc_rows;
+    // full-width launches: wavefronts per workgroup, column offset of wave 0,
+    // time levels per launch and each level's outputs (NULL: not stored)
+    int wide_waves, wide_off;
+    int levels;
+    float *lv_out[4], *lv_out2[4];
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -871,6 +877,251 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
             for (int i = 0; i < kPf; ++i)
                 row(y + i, qb[i], qq[i], qa[i], qk[i], qha[i], qhk[i], qm[i]);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Several forward steps per launch, full-width workgroups
+// ---------------------------------------------------------------------------
+//
+// For grids up to 1024 columns one workgroup holds a whole row: wavefront w
+// covers columns 64 w - off .. 64 w - off + 63 (off = 1 when nx = 64 k + 1,
+// so that column nx - 1 never sits on lane 0 and its Mur update finds its
+// west neighbour in the same wavefront; otherwise 0 and rows are 256-byte
+// aligned). KL time levels are pipelined down a row segment of one shot:
+// at iteration y, level j (1..KL) computes row y - j + 1 from the rows of
+// level j - 1 (level 0 = the input field u, level -1 = u_old) that are
+// already in registers, so the inputs are read once and only the levels
+// asked for are written. The wavefronts march in lockstep, one barrier per
+// iteration, and pass the values next to their block edges (the newest row
+// of every level but the last) to their neighbours through LDS,
+// double-buffered by iteration parity: nothing is loaded or recomputed
+// twice across columns. Rows within KL - 1 of a segment's ends are
+// recomputed by the neighbouring segment. Every cell gets the arithmetic of
+// the single-step kernel, so the results are the same bits.
+//
+// Stores are raw buffer stores; lanes outside the grid use an offset past
+// the buffer's end, which the hardware drops (planes are < 2 GiB).
+
+constexpr int kMaxWideWaves = 16;
+constexpr int kMaxLevels = 4;
+constexpr int kDropOffset = (int)0x80000000u;
+
+struct alignas(16) Edge4 {
+    float v[kMaxLevels];
+};
+// [iteration parity][wavefront slot: 0 and nw + 1 are zero pads][west, east]
+typedef Edge4 EdgeRow[kMaxWideWaves + 2][2];
+
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+// Rows loaded ahead of the march; three, the period of the row rotation,
+// so that an unrolled group needs no register moves.
+constexpr int kPfW = 3;
+
+// Which levels a launch stores: kStoreAny = those with a non-NULL output
+// (checked per level), kStoreLastTwo = the last two, kStoreAll = every level.
+enum : int { kStoreAny = 0, kStoreLastTwo = 1, kStoreAll = 2 };
+
+// XE: the wavefront holds column 0 or nx - 1 (Mur side updates); YE: the
+// segment's levels reach rows 0 / 1 or ny - 1 (Mur row updates).
+template <int KL, int SM, bool XE, bool YE, unsigned F>
+__device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wave, int grp,
+                                          int seg, EdgeRow *ex)
+{
+    const int W = p.nx, H = p.ny;
+    const int x = wave * kLanes - p.wide_off + lane;
+    const bool owned = (x >= 0) & (x < W);
+    const int xo = min(max(x, 0), W - 1) * 4;
+    const int so = owned ? x * 4 : kDropOffset;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_begin = max(r0 - (KL - 1), 0);
+    const int y_end = r1 + KL - 2;  // level KL reaches row r1 - 1
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int row_bytes = W * 4;
+    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * row_bytes; };
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2dt2, plane_bytes);
+    const float *mur = p.mur;
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
+    const int mo = (x == 0) ? 0 : H * 4;
+    auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
+    float kb_x = 0.0f, kt_x = 0.0f;
+    if constexpr (YE) {
+        kb_x = mur[2 * H + xo / 4];
+        kt_x = mur[2 * H + W + xo / 4];
+    }
+    // LDS slots the neighbour values come from: lane 0 reads the east edge
+    // of the wavefront to the west, lane 63 the west edge of the one to the
+    // east (pads beyond the grid); other lanes read their own wavefront's
+    // slots and ignore the values (distinct addresses for the two reads).
+    const int w_slot = lane == 0 ? wave : wave + 1;
+    const int w_side = lane == 0 ? 1 : 0;
+    const int e_slot = lane == kLanes - 1 ? wave + 2 : wave + 1;
+    const int e_side = lane == kLanes - 1 ? 0 : 1;
+    const bool edge_lane = (lane == 0) | (lane == kLanes - 1);
+    const int my_side = lane == kLanes - 1 ? 1 : 0;
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        __amdgpu_buffer_rsrc_t ro[KL], ro2[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            // integer arithmetic: the pointer may be NULL (level not stored)
+            ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
+                              plane_bytes);
+            ro2[j] = make_rsrc((const float *)((uintptr_t)p.lv_out2[j] + soff * sizeof(float)),
+                               plane_bytes);
+        }
+        const int sr = p.src_rows[s], sc = p.src_cols[s];
+        float sv[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) sv[j] = p.src_values[p.src_index + j];
+        // lv[j][0..2]: level j's newest row and the two before it; level 0
+        // is the input field (its newest row at iteration y is y + 1)
+        float lv[KL + 1][3];
+        float kh[KL], mh[KL];  // coefficient rows y, y - 1, ...
+#pragma unroll
+        for (int j = 0; j <= KL; ++j) lv[j][0] = lv[j][1] = lv[j][2] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KL; ++j) kh[j] = mh[j] = 0.0f;
+        lv[0][0] = bload(rb, xo, row_off(y_begin));
+        lv[0][1] = bload(rb, xo, row_off(y_begin - 1));
+        float qb[kPfW], qa[kPfW], qk[kPfW], qm[kPfW];
+#pragma unroll
+        for (int i = 0; i < kPfW; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            qb[i] = bload(rb, xo, row_off(y_begin + 1 + i));
+            qa[i] = bload(ra, xo, row_off(y_begin + i));
+            qk[i] = bload(rk, xo, row_off(y_begin + i));
+            qm[i] = 0.0f;
+            if constexpr (XE) qm[i] = bload(rm, mo, mrow(y_begin + i));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (edge_lane) {
+            Edge4 e;
+            e.v[0] = lv[0][0];
+#pragma unroll
+            for (int j = 1; j < kMaxLevels; ++j) e.v[j] = 0.0f;
+            ex[y_begin & 1][wave + 1][my_side] = e;
+        }
+        __syncthreads();
+        auto store_row = [&](int j, int rho, float v) {
+            const bool stored = SM == kStoreAll || (SM == kStoreLastTwo && j >= KL - 2) ||
+                                (SM == kStoreAny && p.lv_out[j] != nullptr);
+            if (stored) bstore(ro[j], v, so, rho * row_bytes);
+            if constexpr ((F & kOut2) != 0)
+                if (p.lv_out2[j] != nullptr) bstore(ro2[j], v, so, rho * row_bytes);
+        };
+        auto iteration = [&](int y, float &sb, float &sa, float &sk, float &sm) {
+#pragma unroll
+            for (int j = 0; j <= KL; ++j) {
+                lv[j][2] = lv[j][1];
+                lv[j][1] = lv[j][0];
+            }
+#pragma unroll
+            for (int j = KL - 1; j >= 1; --j) {
+                kh[j] = kh[j - 1];
+                mh[j] = mh[j - 1];
+            }
+            float b1 = sb, a0 = sa, k0 = sk, m0 = sm;
+            asm volatile("" : "+v"(b1), "+v"(a0), "+v"(k0), "+v"(m0));
+            __builtin_amdgcn_sched_barrier(0);
+            lv[0][0] = b1;
+            kh[0] = k0;
+            mh[0] = m0;
+            const int yf = min(y + kPfW, y_end);
+            sb = bload(rb, xo, row_off(yf + 1));
+            sa = bload(ra, xo, row_off(yf));
+            sk = bload(rk, xo, row_off(yf));
+            if constexpr (XE) sm = bload(rm, mo, mrow(yf));
+            const Edge4 *hw = &ex[y & 1][w_slot][w_side];
+            const Edge4 *he = &ex[y & 1][e_slot][e_side];
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                const int rho = y - j + 1;
+                const float u = lv[j - 1][1], un = lv[j - 1][0], us = lv[j - 1][2];
+                const float old = j == 1 ? a0 : lv[j - 2][2];
+                // west and east neighbour values read per level into registers
+                // that the lane shifts overwrite in place
+                const float uw = from_west(u, hw->v[j - 1]), ue = from_east(u, he->v[j - 1]);
+                float v = __fmaf_rn(kh[j - 1], lap5(uw, ue, us, un, u, p.idx2, p.idy2, p.cen),
+                                    2.0f * u - old);
+                if (__builtin_expect(rho == sr, 0)) v = (x == sc) ? v + sv[j - 1] : v;
+                if constexpr (XE) {
+                    // rows of segments away from the top and bottom are interior
+                    if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
+                }
+                if constexpr (YE) {
+                    if (rho == H - 1) v = mur_row(mur, W, H, x, kt_x, u, us, lv[j][1], 2);
+                    if (rho == 1) {
+                        const float v0 = mur_row(mur, W, H, x, kb_x, us, u, v, 0);
+                        lv[j][1] = v0;
+                        if (r0 == 0) store_row(j - 1, 0, v0);
+                    }
+                }
+                lv[j][0] = v;
+                if ((rho >= r0) & (rho < r1) & (rho > 0)) store_row(j - 1, rho, v);
+            }
+            if (edge_lane) {
+                Edge4 e;
+#pragma unroll
+                for (int j = 0; j < kMaxLevels; ++j) e.v[j] = j < KL ? lv[j][0] : 0.0f;
+                ex[(y + 1) & 1][wave + 1][my_side] = e;
+            }
+            __syncthreads();
+        };
+        for (int y = y_begin; y <= y_end; y += kPfW) {
+#pragma unroll
+            for (int i = 0; i < kPfW; ++i) iteration(y + i, qb[i], qa[i], qk[i], qm[i]);
+        }
+    }
+}
+
+// At most 64 VGPRs (eight wavefronts per SIMD): two full-width workgroups
+// per CU, so one marches while the other waits at its barrier.
+template <int KL, int SM, unsigned F>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+fwdm_kernel(StepParams p)
+{
+    __shared__ EdgeRow ex[2];
+    const int b = (int)blockIdx.x;
+    if constexpr ((F & kGather) != 0) {
+        if (b >= p.n_stencil_blocks) {
+            gather_pairs(p, (b - p.n_stencil_blocks) * (int)blockDim.x + (int)threadIdx.x);
+            return;
+        }
+    }
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
+    if (wblk >= p.n_work) return;  // uniform over the workgroup
+    const int t = (int)threadIdx.x;
+    if (t < 2 * 2 * kMaxLevels) {  // zero pads at both ends, both parities
+        const int par = t / (2 * kMaxLevels), side = (t / kMaxLevels) & 1, j = t % kMaxLevels;
+        ex[par][0][side].v[j] = 0.0f;
+        ex[par][p.wide_waves + 1][side].v[j] = 0.0f;
+    }
+    const int lane = t & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(t / kLanes);
+    // shot group fastest: consecutive workgroups share the coefficient rows
+    const int grp = wblk % p.n_groups;
+    const int seg = wblk / p.n_groups;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
+    const bool ye = (r0 <= KL) || (r1 + KL - 2 >= p.ny - 1);
+    if (xe) {
+        if (ye) fwdm_wave<KL, SM, true, true, F>(p, lane, wave, grp, seg, ex);
+        else fwdm_wave<KL, SM, true, false, F>(p, lane, wave, grp, seg, ex);
+    } else {
+        if (ye) fwdm_wave<KL, SM, false, true, F>(p, lane, wave, grp, seg, ex);
+        else fwdm_wave<KL, SM, false, false, F>(p, lane, wave, grp, seg, ex);
     }
 }
 
@@ -1749,8 +2000,139 @@ void launch_fwd2_variant(unsigned f, dim3 grid, hipStream_t st, K &&launch)
     (void)st;
 }
 
-// Two forward steps in one launch: outputs p.f_out (first step) and
-// p.f_out_b (second); p.f_out2 / p.f_out2_b optional copies.
+// Whether a row of nx columns fits one full-width workgroup.
+bool wide_fits(int nx)
+{
+    const int off = (nx % kLanes == 1) ? 1 : 0;
+    return (nx + off + kLanes - 1) / kLanes <= kMaxWideWaves;
+}
+
+// Steps per launch and form of the forward runs of a grid. A tuned entry of
+// kind 2 gives the steps per launch (depth 1..4) and the form (param 1 =
+// full width; more than two steps need full width); otherwise full width
+// with SWFD_FWD_LEVELS (default 4) steps where a row fits, else two-step
+// 64-column blocks. No fourth work buffer or SWFD_TWO_STEP=0: single steps.
+struct ForwardPlan {
+    int levels;
+    bool wide;
+};
+
+ForwardPlan forward_plan(int ny, int nx, const float *work3)
+{
+    static const int env_levels = std::min(std::max(env_int("SWFD_FWD_LEVELS", 4), 1), kMaxLevels);
+    static const bool env_two = env_int("SWFD_TWO_STEP", 1) != 0;
+    static const bool env_wide = env_int("SWFD_WIDE", 1) != 0;
+    ForwardPlan f{1, false};
+    if (work3 == nullptr || !env_two) return f;
+    const bool fits = wide_fits(nx);
+    const TuneEntry *t = find_tune(2, ny, nx);
+    if (t != nullptr && t->depth > 0) {
+        f.levels = std::min(t->depth, kMaxLevels);
+        f.wide = fits && f.levels >= 2 && (t->param == 1 || f.levels > 2);
+        if (f.levels > 2 && !fits) f.levels = 2;
+        return f;
+    }
+    f.wide = fits && env_wide && env_levels >= 2;
+    f.levels = f.wide ? env_levels : std::min(env_levels, 2);
+    return f;
+}
+
+// Geometry of a full-width forward launch; returns its workgroup count
+// (gather workgroups included) and the threads per workgroup.
+int wide_geometry(StepParams &p, int gather_pairs_count, int &threads)
+{
+    static const int seg_env = std::max(2, env_int("SWFD_WIDE_SEG_ROWS", 64));
+    static const int spg_env = std::max(1, env_int("SWFD_WIDE_SHOTS_PER_GROUP", 1));
+    const TuneEntry *t = find_tune(0, p.ny, p.nx);
+    const int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
+    p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
+    threads = p.wide_waves * kLanes;
+    p.seg = seg_rows;
+    p.n_seg = std::max(1, p.ny / seg_rows);
+    p.shots_per_group = std::min(spg_env, p.n_shots);
+    p.n_groups = (p.n_shots + p.shots_per_group - 1) / p.shots_per_group;
+    p.n_work = p.n_seg * p.n_groups;
+    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    p.n_stencil_blocks = p.chunk * kXcds;
+    const int gather_blocks =
+        gather_pairs_count > 0 ? (gather_pairs_count + threads - 1) / threads : 0;
+    return p.n_stencil_blocks + gather_blocks;
+}
+
+// Kernel variants: store pattern x (point source, checkpoint copies, fused
+// gather). Copies take the per-level checks; gathers come with two levels
+// only (receiver samples are read from the stored fields).
+template <int KL>
+int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
+                       hipStream_t stream)
+{
+#define SWFD_FWDM(SMV, FV) hipLaunchKernelGGL((fwdm_kernel<KL, SMV, FV>), grid, block, 0, stream, p)
+    if ((f & kOut2) != 0) {
+        if ((f & kGather) != 0) {
+            if constexpr (KL == 2) SWFD_FWDM(kStoreAny, kPoint | kOut2 | kGather);
+            else return fail(SWFD_EINVAL, "multi-step launch: gathers need two levels");
+        } else {
+            SWFD_FWDM(kStoreAny, kPoint | kOut2);
+        }
+        return SWFD_OK;
+    }
+    if ((f & kGather) != 0) {
+        if constexpr (KL == 2) {
+            switch (sm) {
+                case kStoreAll: SWFD_FWDM(kStoreAll, kPoint | kGather); break;
+                case kStoreLastTwo: SWFD_FWDM(kStoreLastTwo, kPoint | kGather); break;
+                default: SWFD_FWDM(kStoreAny, kPoint | kGather); break;
+            }
+            return SWFD_OK;
+        } else {
+            return fail(SWFD_EINVAL, "multi-step launch: gathers need two levels");
+        }
+    }
+    switch (sm) {
+        case kStoreAll: SWFD_FWDM(kStoreAll, kPoint); break;
+        case kStoreLastTwo: SWFD_FWDM(kStoreLastTwo, kPoint); break;
+        default: SWFD_FWDM(kStoreAny, kPoint); break;
+    }
+#undef SWFD_FWDM
+    return SWFD_OK;
+}
+
+// p.levels (2..4) forward steps in one full-width launch; level j's outputs
+// p.lv_out[j] / p.lv_out2[j] (either may be NULL; the last two levels must be
+// stored somewhere by the caller's plan).
+int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
+{
+    if (p.src_rows == nullptr || p.levels < 2 || p.levels > kMaxLevels || !wide_fits(p.nx))
+        return fail(SWFD_EINVAL, "multi-step launch: needs point sources, 2-4 levels, nx <= 1023");
+    if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
+        return fail(SWFD_EINVAL, "multi-step launch: a field plane must be under 2 GiB");
+    int threads = 0;
+    const dim3 grid(wide_geometry(p, gather_pairs_count, threads));
+    const dim3 block(threads);
+    unsigned f = kPoint;
+    bool all = true, last_two = true;
+    for (int j = 0; j < p.levels; ++j) {
+        if (p.lv_out2[j] != nullptr) f |= kOut2;
+        const bool st = p.lv_out[j] != nullptr;
+        all = all && st;
+        last_two = last_two && (st == (j >= p.levels - 2));
+    }
+    if (gather_pairs_count > 0) f |= kGather;
+    const int sm = all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny);
+    int rc;
+    switch (p.levels) {
+        case 2: rc = launch_fwdm_levels<2>(p, sm, f, grid, block, stream); break;
+        case 3: rc = launch_fwdm_levels<3>(p, sm, f, grid, block, stream); break;
+        default: rc = launch_fwdm_levels<4>(p, sm, f, grid, block, stream); break;
+    }
+    if (rc) return rc;
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+// Two forward steps in one 64-column-block launch: outputs p.f_out (first
+// step) and p.f_out_b (second); p.f_out2 / p.f_out2_b optional copies.
 int launch_pair(StepParams &p, hipStream_t stream, int gather_pairs_count)
 {
     if (p.src_rows == nullptr || p.f_out == nullptr || p.f_out_b == nullptr)
@@ -1976,7 +2358,8 @@ struct ForwardRun {
     const swfd_forward_args *a = nullptr;
     size_t field = 0, sr = 0;
     float *work[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool two_step = false;
+    int max_levels = 1;  // steps per launch
+    bool wide = false;   // full-width launches (else 64-column blocks, two steps)
     const float *p_old = nullptr, *p_cur = nullptr;
     int k = 0;
     // fields whose receiver samples are still to be taken (the last one or
@@ -1985,8 +2368,9 @@ struct ForwardRun {
     int pend_step[2] = {0, 0};
     int n_pend = 0;
 
+    // batched: the lockstep tables hold two-step block launches only
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
-             const swfd_forward_args *args)
+             const swfd_forward_args *args, bool batched = false)
     {
         int rc;
         if ((rc = check_forward_args(g, m, r, args))) return rc;
@@ -2000,13 +2384,19 @@ struct ForwardRun {
         work[1] = a->work1;
         work[2] = a->work2;
         work[3] = a->work3;
-        two_step = two_step_allowed(g->ny, g->nx, a->work3);
+        const ForwardPlan plan = forward_plan(g->ny, g->nx, a->work3);
+        max_levels = plan.levels;
+        wide = plan.wide && !batched;
+        // receiver samples are gathered from stored fields: two levels at most
+        if (batched || !wide || a->samples != nullptr) max_levels = std::min(max_levels, 2);
         p_old = a->init_old;
         p_cur = a->init_u;
         return SWFD_OK;
     }
     bool done() const { return k >= a->n_steps; }
-    bool next_is_pair() const { return two_step && k + 1 < a->n_steps; }
+    int next_levels() const { return std::min(max_levels, a->n_steps - k); }
+    bool next_is_pair() const { return next_levels() >= 2; }
+    bool next_is_wide() const { return wide && next_levels() >= 2; }
     // output of step n: its history slot, else a work buffer chosen later
     float *hist_out(int n) const
     {
@@ -2028,8 +2418,8 @@ struct ForwardRun {
         else if (j1 % K == 0 && j1 / K >= 1) slot = 2 * (j1 / K - 1) + 1;
         return (slot >= 0 && slot < a->ckpt_slots) ? a->ckpt + (size_t)slot * field : nullptr;
     }
-    // Parameters of the next launch (a pair when next_is_pair()); returns the
-    // number of fused gathers (pairs of shot and receiver).
+    // Parameters of the next launch (next_levels() steps); returns the number
+    // of fused gathers (pairs of shot and receiver).
     int build(StepParams &p)
     {
         const int n = a->step_begin + k;
@@ -2051,43 +2441,49 @@ struct ForwardRun {
             }
             gathers = n_pend * a->n_shots * rec->n;
         }
-        if (next_is_pair()) {
-            float *out_c = hist_out(n);
-            if (out_c == nullptr) out_c = free_work(work, p_old, p_cur);
-            float *out_d = hist_out(n + 1);
-            if (out_d == nullptr) out_d = free_work(work, p_old, p_cur, out_c);
-            p.f_out = out_c;
-            p.f_out_b = out_d;
-            p.f_out2 = ckpt_out(n);
-            p.f_out2_b = ckpt_out(n + 1);
-        } else {
-            float *out = hist_out(n);
-            if (out == nullptr) out = free_work(work, p_old, p_cur);
-            p.f_out = out;
-            p.f_out2 = ckpt_out(n);
+        const int L = next_levels();
+        p.levels = L;
+        for (int j = 0; j < kMaxLevels; ++j) p.lv_out[j] = p.lv_out2[j] = nullptr;
+        // the last two levels become the next state: a history slot or a
+        // work buffer other than the current state
+        const float *taken = nullptr;
+        for (int j = 0; j < L; ++j) {
+            float *out = hist_out(n + j);
+            if (out == nullptr && j >= L - 2) {
+                out = free_work(work, p_old, p_cur, taken);
+                taken = out;
+            }
+            p.lv_out[j] = out;
+            p.lv_out2[j] = ckpt_out(n + j);
+        }
+        p.f_out = p.lv_out[0];
+        p.f_out2 = p.lv_out2[0];
+        if (L >= 2) {
+            p.f_out_b = p.lv_out[1];
+            p.f_out2_b = p.lv_out2[1];
         }
         return gathers;
     }
-    void advance(const StepParams &p, bool pair)
+    void advance(const StepParams &p)
     {
         const int n = a->step_begin + k;
-        if (pair) {
-            pend_field[0] = p.f_out;
-            pend_step[0] = n;
-            pend_field[1] = p.f_out_b;
-            pend_step[1] = n + 1;
+        const int L = p.levels;
+        if (L >= 2) {
+            pend_field[0] = p.lv_out[L - 2];
+            pend_step[0] = n + L - 2;
+            pend_field[1] = p.lv_out[L - 1];
+            pend_step[1] = n + L - 1;
             n_pend = 2;
-            p_old = p.f_out;
-            p_cur = p.f_out_b;
-            k += 2;
+            p_old = p.lv_out[L - 2];
+            p_cur = p.lv_out[L - 1];
         } else {
             pend_field[0] = p.f_out;
             pend_step[0] = n;
             n_pend = 1;
             p_old = p_cur;
             p_cur = p.f_out;
-            k += 1;
         }
+        k += L;
     }
     int finish(void *stream)
     {
@@ -2293,7 +2689,7 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
     std::vector<ForwardRun> runs((size_t)n_runs);
     for (int i = 0; i < n_runs; ++i)
         if ((rc = runs[i].init(&grids[i], &media[i], recs != nullptr ? &recs[i] : nullptr,
-                               &args[i])))
+                               &args[i], true)))
             return rc;
     hipStream_t st = as_stream(stream);
     std::vector<char> host;
@@ -2317,7 +2713,7 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 pair_runs.push_back(i);
             } else {
                 if (mode == 2 && (rc = launch_step(p, false, st, gathers))) return rc;
-                r.advance(p, false);
+                r.advance(p);
             }
         }
         if (!any) break;
@@ -2344,7 +2740,7 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 SWFD_HIP_CHECK(hipGetLastError());
             }
             off += nb;
-            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j], true);
+            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j]);
         }
     }
     if (bytes_out != nullptr) *bytes_out = off;
@@ -2589,10 +2985,11 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
     while (!run.done()) {
         StepParams p;
         const int gathers = run.build(p);
-        const bool pair = run.next_is_pair();
-        if ((rc = pair ? launch_pair(p, st, gathers) : launch_step(p, false, st, gathers)))
-            return rc;
-        run.advance(p, pair);
+        if (run.next_is_wide()) rc = launch_multi(p, st, gathers);
+        else if (run.next_is_pair()) rc = launch_pair(p, st, gathers);
+        else rc = launch_step(p, false, st, gathers);
+        if (rc) return rc;
+        run.advance(p);
     }
     return run.finish(stream);
 }

@@ -1,10 +1,12 @@
-"""Two-step forward and adjoint launches against single-step launches.
+"""Multi-step forward and two-step adjoint launches against single-step launches.
 
-``swfd_forward`` advances two steps per launch when it is given a fourth work
-buffer. The two-step kernel evaluates the same float32 expressions in the same
-order as the single-step kernel (stencil, point source, Mur edges and corners),
-so samples, history, checkpoints and the final state must be identical
-(``torch.equal``), not merely close. The shapes exercise column blocks that
+``swfd_forward`` advances several steps per launch when it is given a fourth
+work buffer: two per launch in 64-column blocks, or two to four per launch in
+full-width workgroups (two when receiver samples are taken). These kernels
+evaluate the same float32 expressions in the same order as the single-step
+kernel (stencil, point source, Mur edges and corners), so samples, history,
+checkpoints and the final state must be identical (``torch.equal``), not
+merely close. The shapes exercise column blocks that
 end on the right boundary (nx = 63, 125), row segments with a merged tail,
 grids smaller than one segment, sources on and next to the boundary, odd step
 counts (a trailing single step) and a non-zero first step.
@@ -69,15 +71,41 @@ def _pulse(n, seed):
     return torch.randn(n, generator=g, dtype=F32).to(DEV)
 
 
+@pytest.fixture(params=["wide4", "wide3", "wide2", "blocks"])
+def form(request):
+    """Launch form: full-width workgroups with 4 steps per launch (the default
+    where a row fits), 3 or 2, or two-step 64-column blocks (forced through
+    the tuning table for the test's grid)."""
+    from soundwavesimulation_amd import _native
+
+    lib = _native.lib()
+    yield request.param
+    lib.swfd_tune_clear()
+
+
+def _use_form(form, ny, nx):
+    from soundwavesimulation_amd import _native
+
+    lib = _native.lib()
+    lib.swfd_tune_clear()
+    if form == "blocks":
+        _native.check(lib.swfd_tune(2, ny, nx, 0, 0, 2))
+    elif form in ("wide3", "wide2"):
+        _native.check(lib.swfd_tune(2, ny, nx, 0, 1, int(form[-1])))
+
+
 @pytest.mark.parametrize("ny,nx,n_steps", [
     (64, 64, 41),     # one row segment, odd step count
     (200, 63, 30),    # column W-1 on the last lane of the first block; merged tail segment
     (257, 125, 24),   # column W-1 on the last lane of the second block
     (130, 190, 19),   # boundary-adjacent sources, three column blocks
     (5, 7, 9),        # smaller than one segment and one column block
+    (70, 129, 22),    # nx = 64 k + 1: the full-width rows start one column left
+    (40, 1000, 12),   # sixteen wavefronts per row
 ])
-def test_two_step_forward_is_bitwise_single_step(ny, nx, n_steps):
-    print(f"two-step forward {ny}x{nx}, {n_steps} steps", flush=True)
+def test_two_step_forward_is_bitwise_single_step(ny, nx, n_steps, form):
+    print(f"two-step forward ({form}) {ny}x{nx}, {n_steps} steps", flush=True)
+    _use_form(form, ny, nx)
     solver, sp = _medium(ny, nx, seed=ny * 1000 + nx)
     rows = [1, ny - 2, ny // 2, 0, ny - 1, 1]
     cols = [1, nx - 2, nx // 2, nx // 3, 2, nx - 1]
@@ -94,11 +122,16 @@ def test_two_step_forward_is_bitwise_single_step(ny, nx, n_steps):
     assert bool(torch.isfinite(two["cur"]).all())
     assert float(two["cur"].abs().max()) > 0.0
     _assert_identical(one, two, f"{ny}x{nx}")
+    if sampler is not None:  # without samples: full-width launches take all their levels
+        one = _run(solver, None, rows_t, cols_t, values, n_steps, 3)
+        two = _run(solver, None, rows_t, cols_t, values, n_steps, 4)
+        _assert_identical(one, two, f"{ny}x{nx} without samples")
 
 
-def test_two_step_history_checkpoints_and_offset_start():
+def test_two_step_history_checkpoints_and_offset_start(form):
     ny, nx, n_steps, begin = 150, 140, 23, 4
-    print(f"two-step forward with history/checkpoints {ny}x{nx}", flush=True)
+    print(f"two-step forward ({form}) with history/checkpoints {ny}x{nx}", flush=True)
+    _use_form(form, ny, nx)
     solver, sp = _medium(ny, nx, seed=7)
     x, y = axis_centers(nx, sp), axis_centers(ny, sp)
     array = RingArray(8, 0.4 * min(nx, ny) * sp, x, y)
@@ -115,6 +148,11 @@ def test_two_step_history_checkpoints_and_offset_start():
     two = _run(solver, sampler, rows_t, cols_t, values, n_steps, 4, **kw)
     assert float(two["hist"].abs().sum()) > 0.0 and float(two["ckpt"].abs().sum()) > 0.0
     _assert_identical(one, two, "history/checkpoints")
+    # without samples (all levels per launch), every step kept in the history
+    kw.update(hist_slots=n_steps, hist_first=begin, hist_every=1)
+    one = _run(solver, None, rows_t, cols_t, values, n_steps, 3, **kw)
+    two = _run(solver, None, rows_t, cols_t, values, n_steps, 4, **kw)
+    _assert_identical(one, two, "history/checkpoints without samples")
 
 
 def _adjoint(solver, sampler, residual, step_hi, step_lo, n_work, *, init, hist=None,
