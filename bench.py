@@ -120,16 +120,20 @@ def build_problem(a, dev):
 def time_adjoint_kernel(problem, n_steps=20):
     """Average launch time of the fused adjoint kernel (residual + imaging).
 
-    The run advances two adjoint steps per launch, so ``n_steps`` (even) steps
-    are ``n_steps / 2`` launches; the fill and reduction of the partial images
-    around them are included (two small kernels, well under 1 % here).
+    The run advances ``L`` adjoint steps per launch (``swfd_steps_per_launch``),
+    so ``n_steps`` (a multiple of L) steps are ``n_steps / L`` launches; the
+    fill and reduction of the partial images around them are included (two
+    small kernels, well under 1 % here).
     """
+    import soundwavesimulation_amd._native as native
     from soundwavesimulation_amd.torch_backend import adjoint_run
 
     S = problem.parallel_shots
     engine = problem._engine(S)
-    n_steps = min(n_steps, engine.K) // 2 * 2  # imaging reads history slots up to n_steps
-    n_launch = n_steps // 2 if len(engine.lam) > 3 else n_steps
+    ny, nx = problem.model.shape
+    L = int(native.lib().swfd_steps_per_launch(1, ny, nx, 0)) if len(engine.lam) > 3 else 1
+    n_steps = min(n_steps, engine.K) // L * L  # imaging reads history slots up to n_steps
+    n_launch = n_steps // L
     dev = problem.model.device
     R = problem.sampler.n_elements
     residual = torch.randn((n_steps, S, R), device=dev) * 1e-3
@@ -337,15 +341,22 @@ def main():
     # write lam_next), steps_per_launch steps per launch
     adj_bytes = 20.0 * units * steps_per_launch
     achieved = adj_bytes / t_adj / 1e9
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes of this kernel (same grid and steps per launch), per shot
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "traffic_adjoint_pair_1024x1024.json")
-    if os.path.exists(tpath) and n == 1024 and steps_per_launch == 2:
+    tpath = os.path.join(HERE, "profiles", f"traffic_adjoint_{n}x{n}.json")
+    if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch_per_shot")
-        if traffic is not None:
-            traffic = traffic * S
-    kernel = ("adj2_kernel<residual+imaging> (two adjoint steps per launch)"
-              if steps_per_launch == 2 else "step_kernel<adjoint, residual+imaging>")
+            tr = json.load(f)
+        if tr.get("steps_per_launch") == steps_per_launch:
+            traffic = tr["bytes_per_launch_per_shot"] * S
+    if steps_per_launch > 2 or (steps_per_launch == 2 and native.lib().swfd_lap_history(n, n)):
+        kernel = (f"adjm_kernel<{steps_per_launch}, imaging> (full-width, "
+                  f"{steps_per_launch} adjoint steps per launch)")
+    elif steps_per_launch == 2:
+        kernel = "adj2_kernel<residual+imaging> (two adjoint steps per launch)"
+    else:
+        kernel = "step_kernel<adjoint, residual+imaging>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel, "launch_ms": round(t_adj * 1e3, 4),

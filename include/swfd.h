@@ -106,7 +106,13 @@ typedef struct swfd_step_args {
  *   hist (optional): output of step n is stored in slot (n - hist_first) /
  *   hist_every when n >= hist_first and aligned; hist_slots bounds the slot.
  *   ckpt (optional): the state before step j*ckpt_every (j >= 1) is stored as
- *   ckpt slots 2(j-1) (older) and 2(j-1)+1; ckpt_slots bounds the slot. */
+ *   ckpt slots 2(j-1) (older) and 2(j-1)+1; ckpt_slots bounds the slot.
+ *   hist_lap = 1 (no counterpart in the reference; for the adjoint's
+ *   imaging): the slot of step n receives the 5-point Laplacian of the
+ *   output of step n (zero outside the grid) instead of the output, for
+ *   n = step_begin - 1 (the Laplacian of init_u) up to step_begin + n_steps
+ *   - 2 (the last step's output gets none); needs work3, full-width launches
+ *   (swfd_lap_history) and no samples or checkpoints. */
 typedef struct swfd_forward_args {
     int32_t n_shots;
     int32_t step_begin, n_steps;
@@ -120,6 +126,7 @@ typedef struct swfd_forward_args {
     int32_t hist_first, hist_every, hist_slots;
     float *ckpt;
     int32_t ckpt_every, ckpt_slots;
+    int32_t hist_lap; /* 1: history slots receive Laplacians (see above) */
 } swfd_forward_args;
 
 /* A run of adjoint steps n = step_hi - 1 down to step_lo with residual
@@ -144,7 +151,18 @@ typedef struct swfd_adjoint_args {
     float image_scale;
     float *grad;
     float *workspace; /* swfd_adjoint_workspace_floats() floats, needed with hist */
+    int32_t hist_lap; /* 1: slots hold Lap(field): the imaging adds lam' * slot */
 } swfd_adjoint_args;
+
+/* 1 when runs on an ny x nx grid (with work3) support Laplacian histories
+ * (hist_lap), else 0 (no counterpart in the reference). */
+int32_t swfd_lap_history(int32_t ny, int32_t nx);
+
+/* Steps per launch that swfd_forward (kind 0; samples != 0: with receiver
+ * samples) or swfd_adjoint (kind 1) takes on an ny x nx grid given work3,
+ * under the current tuning table and environment; 0 for bad arguments (no
+ * counterpart in the reference; for profiling). */
+int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samples);
 
 /* Thread-local message of the last failure. */
 const char *swfd_last_error(void);
@@ -247,14 +265,19 @@ int swfd_adjoint_batch(int32_t n, const swfd_grid *grids, const swfd_medium *med
                        const void *table, size_t table_bytes, void *stream);
 
 /* Launch-shape tuning for one grid size (no counterpart in the reference):
- *   kind 0: two-step forward; seg_rows = rows per march, param = target
- *           work items per CU;
- *   kind 1: two-step adjoint; seg_rows (<= 32), param = shot groups (<= 8);
- *   kind 2: temporal depth and form: depth 1 = single-step kernels,
- *           2 = two-step marches in 64-column blocks, 3 = two-step marches
- *           in full-width workgroups (rows of up to 1024 columns; the
- *           default where they fit; batched launches always use blocks).
- * Values <= 0 keep the defaults. swfd_tune_clear forgets every entry. The
+ *   kind 0: multi-step forward; seg_rows = rows per march, param = target
+ *           work items per CU (64-column blocks);
+ *   kind 1: multi-step adjoint; seg_rows = rows per march, param = shot
+ *           groups (<= 8);
+ *   kind 2: forward steps per launch and form: depth 1 = single-step
+ *           kernels (adjoint too), 2..4 = steps per launch; param 1 =
+ *           full-width workgroups (rows of up to 1023 columns; required for
+ *           more than two steps), 0 = 64-column blocks (two steps; with
+ *           depth 2 the adjoint uses blocks too);
+ *   kind 3: adjoint steps per launch and form, as kind 2.
+ * Without entries: full width where a row fits, 4 forward steps per launch
+ * (2 when receiver samples are taken), 4 adjoint steps; batched launches
+ * always use two-step blocks. Values <= 0 keep the defaults. swfd_tune_clear forgets every entry. The
  * table is process-wide host state: set it before running (not while another
  * host thread launches). */
 int swfd_tune(int32_t kind, int32_t ny, int32_t nx, int32_t seg_rows, int32_t param,

@@ -54,7 +54,8 @@ class ForwardArgs(ctypes.Structure):
                 ("work2", _fp), ("work3", _fp), ("final_old", _fp), ("final_u", _fp), ("src_rows", _fp),
                 ("src_cols", _fp), ("src_values", _fp), ("samples", _fp), ("hist", _fp),
                 ("hist_first", _c_int), ("hist_every", _c_int), ("hist_slots", _c_int),
-                ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int)]
+                ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int),
+                ("hist_lap", _c_int)]
 
 
 class AdjointArgs(ctypes.Structure):
@@ -63,7 +64,7 @@ class AdjointArgs(ctypes.Structure):
                 ("work2", _fp), ("work3", _fp), ("final_old", _fp), ("final_u", _fp), ("residual", _fp),
                 ("hist", _fp), ("hist_first", _c_int), ("hist_every", _c_int),
                 ("hist_slots", _c_int), ("image_scale", _c_float), ("grad", _fp),
-                ("workspace", _fp)]
+                ("workspace", _fp), ("hist_lap", _c_int)]
 
 
 #: every symbol include/swfd.h declares
@@ -74,6 +75,7 @@ EXPORTS = (
     "swfd_residual_workspace_floats", "swfd_residual", "swfd_forward_batch_table_bytes",
     "swfd_forward_batch_prepare", "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
     "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
+    "swfd_lap_history", "swfd_steps_per_launch",
 )
 
 _lib = None
@@ -123,6 +125,10 @@ def load_library(path=LIBRARY):
     lib.swfd_tune.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]
     lib.swfd_tune_clear.restype = None
     lib.swfd_tune_clear.argtypes = []
+    lib.swfd_lap_history.restype = _c_int
+    lib.swfd_lap_history.argtypes = [_c_int, _c_int]
+    lib.swfd_steps_per_launch.restype = _c_int
+    lib.swfd_steps_per_launch.argtypes = [_c_int, _c_int, _c_int, _c_int]
     for name in ("swfd_prepare_medium", "swfd_step", "swfd_gather", "swfd_scatter",
                  "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
                  "swfd_forward_batch_table_bytes", "swfd_forward_batch_prepare",

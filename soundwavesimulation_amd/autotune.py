@@ -4,16 +4,19 @@ For a grid (ny, nx) and a shot count it times, on the device, short runs of
 the forward and adjoint kernels under candidate shapes and installs the
 fastest through ``swfd_tune`` (``include/swfd.h``):
 
-* temporal depth: single-step kernels (depth 1) or two-step marches (depth 2);
-* forward marches: rows per march (the tile is 64 columns by that many rows)
-  and the target number of work items per CU (which sets the shots per
-  wavefront);
-* adjoint marches: rows per march (<= 32, bounded by the imaging accumulator
-  in LDS) and the number of shot groups (<= 8 partial images).
+* single-step kernels for both (kind 2, depth 1);
+* forward: two-step marches in 64-column blocks (rows per march x target
+  work items per CU), or full-width workgroups with 2-4 steps per launch
+  (rows per march);
+* adjoint: two-step blocks (rows per march <= 32 x shot groups), or
+  full-width workgroups with 2-4 steps per launch (rows per march; the
+  segment's image lives in LDS).
 
-Results are cached per device name, grid and shot count in a JSON file
-(``SWFD_TUNE_CACHE``, default ``~/.cache/swfd_tune.json``), so a later
-process applies them without timing again.
+``batched=True`` keeps to the forms the batched lockstep launches use
+(single steps or two-step blocks). Results are cached per device name, grid,
+shot count and mode in a JSON file (``SWFD_TUNE_CACHE``, default
+``~/.cache/swfd_tune.json``), so a later process applies them without timing
+again.
 """
 
 from __future__ import annotations
@@ -34,6 +37,9 @@ FORWARD_SEG_ROWS = (8, 16, 32, 64, 128)
 FORWARD_WAVES_PER_CU = (128, 256)
 ADJOINT_SEG_ROWS = (8, 16, 32)
 ADJOINT_GROUPS = (4, 8)
+WIDE_LEVELS = (2, 3, 4)
+WIDE_FORWARD_SEG_ROWS = (32, 64, 128)
+WIDE_ADJOINT_SEG_ROWS = (16, 32)
 
 
 def _cache_path():
@@ -41,15 +47,22 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "swfd_tune.json"))
 
 
-def _key(device, ny, nx, n_shots):
-    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}"
+def _key(device, ny, nx, n_shots, batched):
+    mode = "batched" if batched else "single"
+    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}:{mode}"
 
 
 def apply(cfg, ny, nx):
+    """Install a configuration returned by ``tune``."""
     lib = _native.lib()
-    _native.check(lib.swfd_tune(0, ny, nx, cfg["fwd_seg"], cfg["fwd_wpc"], 0))
-    _native.check(lib.swfd_tune(1, ny, nx, cfg["adj_seg"], cfg["adj_groups"], 0))
-    _native.check(lib.swfd_tune(2, ny, nx, 0, 0, cfg["depth"]))
+    f, a = cfg["forward"], cfg["adjoint"]
+    _native.check(lib.swfd_tune(0, ny, nx, f["seg"], f.get("wpc", 0), 0))
+    _native.check(lib.swfd_tune(1, ny, nx, a["seg"], a.get("groups", 0), 0))
+    if cfg["depth1"]:
+        _native.check(lib.swfd_tune(2, ny, nx, 0, 0, 1))
+        return
+    _native.check(lib.swfd_tune(2, ny, nx, 0, f["wide"], f["levels"]))
+    _native.check(lib.swfd_tune(3, ny, nx, 0, a["wide"], a["levels"]))
 
 
 def _time(fn, reps=2):
@@ -64,10 +77,11 @@ def _time(fn, reps=2):
     return a.elapsed_time(b) / 1e3 / reps
 
 
-def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, log=None):
+def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, log=None,
+         batched=False):
     """Time candidates for (ny, nx, n_shots); install and return the fastest."""
     path = _cache_path()
-    key = _key(device, ny, nx, n_shots)
+    key = _key(device, ny, nx, n_shots, batched)
     cache = {}
     if use_cache and os.path.exists(path):
         with open(path) as f:
@@ -85,6 +99,7 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
     S = n_shots
     rows = torch.as_tensor(np.resize(array.rows, S).astype(np.int32), device=device)
     cols = torch.as_tensor(np.resize(array.cols, S).astype(np.int32), device=device)
+    n_steps = max(12, n_steps // 12 * 12)  # whole launches for 1-4 steps per launch
     vals = torch.zeros(n_steps + 2, device=device)
     vals[:4] = 1.0
     shape = (S, ny, nx)
@@ -102,35 +117,59 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
                     hist=hist, hist_first=-2, hist_every=1, image_scale=1e-15, grad=grad,
                     workspace=ws)
 
-    results = []
+    # entries of this grid only (other grids keep theirs); kind 3 depth 0 =
+    # no adjoint entry
+    lib.swfd_tune(3, ny, nx, 0, 0, 0)
     lib.swfd_tune(2, ny, nx, 0, 0, 1)
     t1 = _time(fwd) + _time(adj)
-    results.append(("depth1", t1))
+    # forward candidates
+    fwd_best = None
     lib.swfd_tune(2, ny, nx, 0, 0, 2)
-    best_f = None
     for seg in FORWARD_SEG_ROWS:
         if seg > ny:
             continue
         for wpc in FORWARD_WAVES_PER_CU:
             lib.swfd_tune(0, ny, nx, seg, wpc, 0)
             t = _time(fwd)
-            if best_f is None or t < best_f[0]:
-                best_f = (t, seg, wpc)
-    lib.swfd_tune(0, ny, nx, best_f[1], best_f[2], 0)
-    best_a = None
+            if fwd_best is None or t < fwd_best[0]:
+                fwd_best = (t, {"wide": 0, "levels": 2, "seg": seg, "wpc": wpc})
+    wide = not batched and bool(lib.swfd_lap_history(ny, nx))  # full-width launches fit
+    if wide:
+        for levels in WIDE_LEVELS:
+            lib.swfd_tune(2, ny, nx, 0, 1, levels)
+            for seg in WIDE_FORWARD_SEG_ROWS:
+                if seg > ny:
+                    continue
+                lib.swfd_tune(0, ny, nx, seg, 0, 0)
+                t = _time(fwd)
+                if t < fwd_best[0]:
+                    fwd_best = (t, {"wide": 1, "levels": levels, "seg": seg, "wpc": 0})
+    # adjoint candidates
+    adj_best = None
+    lib.swfd_tune(2, ny, nx, 0, 0, 2)
+    lib.swfd_tune(3, ny, nx, 0, 0, 2)
     for seg in ADJOINT_SEG_ROWS:
         if seg > ny:
             continue
         for groups in ADJOINT_GROUPS:
             lib.swfd_tune(1, ny, nx, seg, groups, 0)
             t = _time(adj)
-            if best_a is None or t < best_a[0]:
-                best_a = (t, seg, groups)
-    t2 = best_f[0] + best_a[0]
-    cfg = {"depth": 2 if t2 <= t1 else 1, "fwd_seg": best_f[1], "fwd_wpc": best_f[2],
-           "adj_seg": best_a[1], "adj_groups": best_a[2],
-           "ms_per_step_depth1": round(t1 / n_steps * 1e3, 4),
-           "ms_per_step_depth2": round(t2 / n_steps * 1e3, 4)}
+            if adj_best is None or t < adj_best[0]:
+                adj_best = (t, {"wide": 0, "levels": 2, "seg": seg, "groups": groups})
+    if wide:
+        for levels in WIDE_LEVELS:
+            lib.swfd_tune(3, ny, nx, 0, 1, levels)
+            for seg in WIDE_ADJOINT_SEG_ROWS:
+                if seg > ny:
+                    continue
+                lib.swfd_tune(1, ny, nx, seg, 8, 0)
+                t = _time(adj)
+                if t < adj_best[0]:
+                    adj_best = (t, {"wide": 1, "levels": levels, "seg": seg, "groups": 8})
+    t2 = fwd_best[0] + adj_best[0]
+    cfg = {"depth1": t1 < t2, "forward": fwd_best[1], "adjoint": adj_best[1],
+           "ms_per_step_single": round(t1 / n_steps * 1e3, 4),
+           "ms_per_step_multi": round(t2 / n_steps * 1e3, 4)}
     apply(cfg, ny, nx)
     if log is not None:
         log(f"tuned {ny}x{nx} x {S} shots: {cfg}")

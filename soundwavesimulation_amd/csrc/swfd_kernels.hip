@@ -109,6 +109,11 @@ struct StepParams {
     int wide_waves, wide_off;
     int levels;
     float *lv_out[4], *lv_out2[4];
+    // Laplacian history: level j stores the Laplacian of its input field
+    float *lv_lap[4];
+    int hist_lap;
+    // multi-step adjoint launches: each level's imaging field and residual
+    const float *lv_in[4], *lv_resid[4];
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -285,6 +290,7 @@ enum : unsigned {
     kImage = 8u,
     kOut2 = 16u,
     kGather = 32u,
+    kLap = 64u,  // multi-step forward: store each level's input Laplacian
 };
 
 // Neighbour exchange inside a wavefront through DPP wavefront shifts: lane i
@@ -907,8 +913,10 @@ constexpr int kMaxWideWaves = 16;
 constexpr int kMaxLevels = 4;
 constexpr int kDropOffset = (int)0x80000000u;
 
+// Edge values of one wavefront side: the newest row of each level, then (for
+// Laplacian histories) the row before it.
 struct alignas(16) Edge4 {
-    float v[kMaxLevels];
+    float v[2 * kMaxLevels];
 };
 // [iteration parity][wavefront slot: 0 and nw + 1 are zero pads][west, east]
 typedef Edge4 EdgeRow[kMaxWideWaves + 2][2];
@@ -932,9 +940,17 @@ template <int KL, int SM, bool XE, bool YE, unsigned F>
 __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wave, int grp,
                                           int seg, EdgeRow *ex)
 {
+    constexpr bool LAP = (F & kLap) != 0;
     const int W = p.nx, H = p.ny;
     const int x = wave * kLanes - p.wide_off + lane;
     const bool owned = (x >= 0) & (x < W);
+    // Laplacian histories need values outside the grid to be zero: lanes
+    // beyond the side columns and rows beyond the top and bottom (they only
+    // feed boundary cells, which the Mur updates replace, so the fields do
+    // not change)
+    auto in_grid = [&](int row) -> bool {
+        return (!XE || owned) && (!YE || ((row >= 0) & (row < H)));
+    };
     const int xo = min(max(x, 0), W - 1) * 4;
     const int so = owned ? x * 4 : kDropOffset;
     int r0, r1;
@@ -971,7 +987,7 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
         const size_t soff = (size_t)s * plane;
         const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
         const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
-        __amdgpu_buffer_rsrc_t ro[KL], ro2[KL];
+        __amdgpu_buffer_rsrc_t ro[KL], ro2[KL], rl[KL];
 #pragma unroll
         for (int j = 0; j < KL; ++j) {
             // integer arithmetic: the pointer may be NULL (level not stored)
@@ -979,6 +995,8 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                               plane_bytes);
             ro2[j] = make_rsrc((const float *)((uintptr_t)p.lv_out2[j] + soff * sizeof(float)),
                                plane_bytes);
+            rl[j] = make_rsrc((const float *)((uintptr_t)p.lv_lap[j] + soff * sizeof(float)),
+                              plane_bytes);
         }
         const int sr = p.src_rows[s], sc = p.src_cols[s];
         float sv[KL];
@@ -994,6 +1012,10 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
         for (int j = 0; j < KL; ++j) kh[j] = mh[j] = 0.0f;
         lv[0][0] = bload(rb, xo, row_off(y_begin));
         lv[0][1] = bload(rb, xo, row_off(y_begin - 1));
+        if constexpr (LAP) {
+            lv[0][0] = in_grid(y_begin) ? lv[0][0] : 0.0f;
+            lv[0][1] = in_grid(y_begin - 1) ? lv[0][1] : 0.0f;
+        }
         float qb[kPfW], qa[kPfW], qk[kPfW], qm[kPfW];
 #pragma unroll
         for (int i = 0; i < kPfW; ++i) {
@@ -1007,9 +1029,10 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
         __builtin_amdgcn_sched_barrier(0);
         if (edge_lane) {
             Edge4 e;
-            e.v[0] = lv[0][0];
 #pragma unroll
-            for (int j = 1; j < kMaxLevels; ++j) e.v[j] = 0.0f;
+            for (int j = 0; j < 2 * kMaxLevels; ++j) e.v[j] = 0.0f;
+            e.v[0] = lv[0][0];
+            e.v[kMaxLevels] = lv[0][1];
             ex[y_begin & 1][wave + 1][my_side] = e;
         }
         __syncthreads();
@@ -1034,7 +1057,7 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
             float b1 = sb, a0 = sa, k0 = sk, m0 = sm;
             asm volatile("" : "+v"(b1), "+v"(a0), "+v"(k0), "+v"(m0));
             __builtin_amdgcn_sched_barrier(0);
-            lv[0][0] = b1;
+            lv[0][0] = LAP && !in_grid(y + 1) ? 0.0f : b1;
             kh[0] = k0;
             mh[0] = m0;
             const int yf = min(y + kPfW, y_end);
@@ -1052,8 +1075,8 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                 // west and east neighbour values read per level into registers
                 // that the lane shifts overwrite in place
                 const float uw = from_west(u, hw->v[j - 1]), ue = from_east(u, he->v[j - 1]);
-                float v = __fmaf_rn(kh[j - 1], lap5(uw, ue, us, un, u, p.idx2, p.idy2, p.cen),
-                                    2.0f * u - old);
+                const float lp = lap5(uw, ue, us, un, u, p.idx2, p.idy2, p.cen);
+                float v = __fmaf_rn(kh[j - 1], lp, 2.0f * u - old);
                 if (__builtin_expect(rho == sr, 0)) v = (x == sc) ? v + sv[j - 1] : v;
                 if constexpr (XE) {
                     // rows of segments away from the top and bottom are interior
@@ -1062,18 +1085,38 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                 if constexpr (YE) {
                     if (rho == H - 1) v = mur_row(mur, W, H, x, kt_x, u, us, lv[j][1], 2);
                     if (rho == 1) {
-                        const float v0 = mur_row(mur, W, H, x, kb_x, us, u, v, 0);
+                        float v0 = mur_row(mur, W, H, x, kb_x, us, u, v, 0);
+                        if constexpr (LAP) v0 = owned ? v0 : 0.0f;
                         lv[j][1] = v0;
                         if (r0 == 0) store_row(j - 1, 0, v0);
+                        if constexpr (LAP) {
+                            // row 0 of the input is final only now (and its edge
+                            // values came as the previous row): its Laplacian
+                            const float z = us;  // input row 0
+                            const float zw = from_west(z, hw->v[kMaxLevels + j - 1]);
+                            const float ze = from_east(z, he->v[kMaxLevels + j - 1]);
+                            const float l0 = lap5(zw, ze, 0.0f, u, z, p.idx2, p.idy2, p.cen);
+                            if ((r0 == 0) & (p.lv_lap[j - 1] != nullptr)) bstore(rl[j - 1], l0, so, 0);
+                        }
                     }
                 }
+                if constexpr (LAP) v = in_grid(rho) ? v : 0.0f;
                 lv[j][0] = v;
                 if ((rho >= r0) & (rho < r1) & (rho > 0)) store_row(j - 1, rho, v);
+                if constexpr (LAP) {
+                    if ((p.lv_lap[j - 1] != nullptr) & (rho >= r0) & (rho < r1) & (rho > 0))
+                        bstore(rl[j - 1], lp, so, rho * row_bytes);
+                }
             }
             if (edge_lane) {
                 Edge4 e;
 #pragma unroll
                 for (int j = 0; j < kMaxLevels; ++j) e.v[j] = j < KL ? lv[j][0] : 0.0f;
+                if constexpr (LAP && YE) {
+#pragma unroll
+                    for (int j = 0; j < kMaxLevels; ++j)
+                        e.v[kMaxLevels + j] = j < KL ? lv[j][1] : 0.0f;
+                }
                 ex[(y + 1) & 1][wave + 1][my_side] = e;
             }
             __syncthreads();
@@ -1102,8 +1145,9 @@ fwdm_kernel(StepParams p)
     const int wblk = (b % kXcds) * p.chunk + b / kXcds;
     if (wblk >= p.n_work) return;  // uniform over the workgroup
     const int t = (int)threadIdx.x;
-    if (t < 2 * 2 * kMaxLevels) {  // zero pads at both ends, both parities
-        const int par = t / (2 * kMaxLevels), side = (t / kMaxLevels) & 1, j = t % kMaxLevels;
+    if (t < 2 * 2 * 2 * kMaxLevels) {  // zero pads at both ends, both parities
+        const int par = t / (4 * kMaxLevels), side = (t / (2 * kMaxLevels)) & 1,
+                  j = t % (2 * kMaxLevels);
         ex[par][0][side].v[j] = 0.0f;
         ex[par][p.wide_waves + 1][side].v[j] = 0.0f;
     }
@@ -1574,6 +1618,304 @@ __global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table
     adj2_block<F>(table[e], b - first[e]);
 }
 
+// ---------------------------------------------------------------------------
+// Several adjoint steps per launch, full-width workgroups
+// ---------------------------------------------------------------------------
+//
+// Steps n, n-1, ..., n-KL+1 of the reverse run in one launch, with the
+// full-width workgroups and the level pipeline of the forward kernel above:
+// level j (1..KL) computes lam' = 2 lam - lam_old + dt^2 Lap(c^2 lam) +
+// residual, then the forward Mur update, at row y - j + 1 of iteration y.
+// Each level keeps lam and P = c^2 lam of its three newest rows; the edge
+// lanes exchange P (and the imaging fields) through LDS. The arithmetic per
+// cell and step is the single-step kernel's, so the states are the same bits.
+//
+// Imaging: level j (step n - j + 1) is imaged against the forward field of
+// its step, lv_in[j - 1] (zero outside the grid), one row late so that row 0
+// is final: g += scale sum_s lam_j Lap(F_j). A workgroup takes one row
+// segment of one shot group and marches the group's shots one after the
+// other, adding each cell's terms into an LDS image of its segment (each lane
+// owns its column, so the order is fixed); at the end it adds the segment
+// into the group's partial image (at most kMaxImageGroups), which are summed
+// in group order after the run: the gradient is deterministic.
+
+constexpr int kPfA = 2;  // rows loaded ahead in the adjoint march
+
+// (P of level j - 1, imaging field of level j) for j = 1..KL
+struct alignas(16) AdjEdge {
+    float2 v[kMaxLevels];
+};
+typedef AdjEdge AdjEdgeRow[kMaxWideWaves + 2][2];
+
+extern __shared__ float adjm_acc[];
+
+// IMG: 0 no imaging, 1 imaging against fields, 2 against their Laplacians.
+template <int KL, bool XE, bool YE, int IMG>
+__device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wave, int grp,
+                                          int seg, AdjEdgeRow *ex)
+{
+    const int W = p.nx, H = p.ny;
+    const int x = wave * kLanes - p.wide_off + lane;
+    const bool owned = (x >= 0) & (x < W);
+    const int xo = min(max(x, 0), W - 1) * 4;
+    const int so = owned ? x * 4 : kDropOffset;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_begin = max(r0 - (KL - 1), 0);
+    // level KL reaches row r1 - 1; imaging one row late needs one more
+    const int y_end = r1 + KL - 2 + (IMG != 0 ? 1 : 0);
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int row_bytes = W * 4;
+    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * row_bytes; };
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
+    const __amdgpu_buffer_rsrc_t re = make_rsrc((const float *)p.cell_map, plane_bytes);
+    const float *mur = p.mur;
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
+    const int mo = (x == 0) ? 0 : H * 4;
+    auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
+    const float dt2 = p.dt2;
+    float kb_x = 0.0f, kt_x = 0.0f;
+    if constexpr (YE) {
+        kb_x = mur[2 * H + xo / 4];
+        kt_x = mur[2 * H + W + xo / 4];
+    }
+    const int w_slot = lane == 0 ? wave : wave + 1;
+    const int w_side = lane == 0 ? 1 : 0;
+    const int e_slot = lane == kLanes - 1 ? wave + 2 : wave + 1;
+    const int e_side = lane == kLanes - 1 ? 0 : 1;
+    const bool edge_lane = (lane == 0) | (lane == kLanes - 1);
+    const int my_side = lane == kLanes - 1 ? 1 : 0;
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    float *acc = adjm_acc;
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        __amdgpu_buffer_rsrc_t ro[KL], rf[KL];
+        const float *res[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            // integer arithmetic: the pointers may be NULL (not stored / not imaged)
+            ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
+                              plane_bytes);
+            // a level without imaging field reads (and ignores) the state instead
+            rf[j] = make_rsrc((p.lv_in[j] != nullptr ? p.lv_in[j] : p.f_cur) + soff, plane_bytes);
+            res[j] = p.lv_resid[j] + (size_t)s * p.n_rec;
+        }
+        // residual injected at a cell: sum over its scatter entries, in order
+        auto injected = [&](const float *rs, int e) -> float {
+            float a = 0.0f;
+            for (int k = p.entry_start[e]; k < p.entry_start[e + 1]; ++k)
+                a += rs[p.pair_rec[k]] * p.pair_w[k];
+            return a;
+        };
+        // lam of each level's newest three rows (level 0 = the input state,
+        // newest row y + 1 at iteration y); P = c^2 lam is formed where it is
+        // needed (the same product each time, so the same bits), which keeps
+        // the register budget down
+        float lv[KL + 1][3];
+        // c^2 of rows y + 1, y, ..., y - KL; scatter entry and Mur side
+        // coefficient of rows y, ..., y - KL + 1; imaging field of level j at
+        // rows y - j + 1, y - j, y - j - 1
+        float kh[KL + 2], mh[KL], fh[KL + 1][3];
+        int eh[KL];
+#pragma unroll
+        for (int j = 0; j <= KL; ++j)
+            lv[j][0] = lv[j][1] = lv[j][2] = fh[j][0] = fh[j][1] = fh[j][2] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KL + 2; ++j) kh[j] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            mh[j] = 0.0f;
+            eh[j] = -1;
+        }
+        lv[0][0] = bload(rb, xo, row_off(y_begin));
+        lv[0][1] = bload(rb, xo, row_off(y_begin - 1));
+        kh[0] = bload(rk, xo, row_off(y_begin));
+        kh[1] = bload(rk, xo, row_off(y_begin - 1));
+        float qb[kPfA], qa[kPfA], qk[kPfA], qe[kPfA], qm[kPfA], qf[kPfA][KL];
+        auto fill = [&](int i, int y) {
+            qb[i] = bload(rb, xo, row_off(y + 1));
+            qa[i] = bload(ra, xo, row_off(y));
+            qk[i] = bload(rk, xo, row_off(y + 1));
+            qe[i] = bload(re, xo, row_off(y));
+            qm[i] = 0.0f;
+            if constexpr (XE) qm[i] = bload(rm, mo, mrow(y));
+#pragma unroll
+            for (int j = 0; j < KL; ++j) {
+                // level j + 1: newest field row (y - j), or the Laplacian of the
+                // row it images (y - j - 1)
+                qf[i][j] = 0.0f;
+                if constexpr (IMG == 1) qf[i][j] = bload(rf[j], xo, row_off(y - j));
+                if constexpr (IMG == 2) qf[i][j] = bload(rf[j], xo, row_off(y - j - 1));
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < kPfA; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            fill(i, y_begin + i);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (edge_lane) {
+            AdjEdge e;
+            e.v[0] = make_float2(kh[0] * lv[0][0], 0.0f);
+#pragma unroll
+            for (int j = 1; j < kMaxLevels; ++j) e.v[j] = make_float2(0.0f, 0.0f);
+            ex[y_begin & 1][wave + 1][my_side] = e;
+        }
+        __syncthreads();
+        auto iteration = [&](int y, int i) {
+#pragma unroll
+            for (int j = 0; j <= KL; ++j) {
+                lv[j][2] = lv[j][1];
+                lv[j][1] = lv[j][0];
+                fh[j][2] = fh[j][1];
+                fh[j][1] = fh[j][0];
+            }
+#pragma unroll
+            for (int j = KL + 1; j >= 1; --j) kh[j] = kh[j - 1];
+#pragma unroll
+            for (int j = KL - 1; j >= 1; --j) {
+                mh[j] = mh[j - 1];
+                eh[j] = eh[j - 1];
+            }
+            float b1 = qb[i], a0 = qa[i], k1 = qk[i], e0 = qe[i], m0 = qm[i];
+            float f[KL];
+#pragma unroll
+            for (int j = 0; j < KL; ++j) f[j] = qf[i][j];
+            asm volatile("" : "+v"(b1), "+v"(a0), "+v"(k1), "+v"(e0), "+v"(m0));
+#pragma unroll
+            for (int j = 0; j < KL; ++j) asm volatile("" : "+v"(f[j]));
+            __builtin_amdgcn_sched_barrier(0);
+            fill(i, min(y + kPfA, y_end));
+            lv[0][0] = b1;
+            kh[0] = k1;
+            eh[0] = owned ? __float_as_int(e0) : -1;
+            mh[0] = m0;
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                if constexpr (IMG == 1) {
+                    const int fr = y - j + 1;  // row of level j's newest imaging field
+                    fh[j][0] = (owned & (fr >= 0) & (fr < H)) ? f[j - 1] : 0.0f;
+                } else {
+                    fh[j][0] = f[j - 1];
+                }
+            }
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                const int rho = y - j + 1;
+                const AdjEdge *hw = &ex[y & 1][w_slot][w_side];
+                const AdjEdge *he = &ex[y & 1][e_slot][e_side];
+                const float u = lv[j - 1][1];
+                const float old = j == 1 ? a0 : lv[j - 2][2];
+                // P of the input level at rows rho - 1, rho, rho + 1
+                const float Ps = kh[j + 1] * lv[j - 1][2];
+                const float P0 = kh[j] * u;
+                const float Pn = kh[j - 1] * lv[j - 1][0];
+                const float pw = from_west(P0, hw->v[j - 1].x), pe = from_east(P0, he->v[j - 1].x);
+                float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, p.idx2, p.idy2, p.cen),
+                                    2.0f * u - old);
+                const int e = eh[j - 1];
+                if (e >= 0) v += injected(res[j - 1], e);
+                if constexpr (XE) {
+                    if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
+                }
+                if constexpr (YE) {
+                    if (rho == H - 1) v = mur_row(mur, W, H, x, kt_x, u, lv[j - 1][2], lv[j][1], 2);
+                    if (rho == 1) {
+                        const float v0 = mur_row(mur, W, H, x, kb_x, lv[j - 1][2], u, v, 0);
+                        lv[j][1] = v0;
+                        if ((j >= KL - 1) & (r0 == 0)) bstore(ro[j - 1], v0, so, 0);
+                    }
+                }
+                lv[j][0] = v;
+                if ((j >= KL - 1) & (rho >= r0) & (rho < r1) & (rho > 0))
+                    bstore(ro[j - 1], v, so, rho * row_bytes);
+                if constexpr (IMG != 0) {
+                    const int R = rho - 1;
+                    if ((p.lv_in[j - 1] != nullptr) & (R >= r0) & (R < r1)) {
+                        float t;
+                        if constexpr (IMG == 1) {
+                            const float F0 = fh[j][1];
+                            const float fw = from_west(F0, hw->v[j - 1].y),
+                                        fe = from_east(F0, he->v[j - 1].y);
+                            t = lv[j][1] * lap5(fw, fe, fh[j][2], fh[j][0], F0, p.idx2, p.idy2,
+                                                p.cen);
+                        } else {
+                            t = lv[j][1] * fh[j][0];  // the Laplacian of row R
+                        }
+                        if (owned) acc[(R - r0) * W + x] += t;
+                    }
+                }
+            }
+            if (edge_lane) {
+                AdjEdge e;
+#pragma unroll
+                for (int j = 0; j < kMaxLevels; ++j)
+                    e.v[j] = j < KL ? make_float2(kh[j] * lv[j][0], IMG == 1 ? fh[j + 1][0] : 0.0f)
+                                    : make_float2(0.0f, 0.0f);
+                ex[(y + 1) & 1][wave + 1][my_side] = e;
+            }
+            __syncthreads();
+        };
+        for (int y = y_begin; y <= y_end; y += kPfA) {
+#pragma unroll
+            for (int i = 0; i < kPfA; ++i) iteration(y + i, i);
+        }
+    }
+}
+
+// KL adjoint steps in one launch; workgroup = (row segment, shot group).
+template <int KL, int IMG>
+__global__ void __launch_bounds__(1024) adjm_kernel(StepParams p)
+{
+    __shared__ AdjEdgeRow ex[2];
+    const int b = (int)blockIdx.x;
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
+    if (wblk >= p.n_work) return;  // uniform over the workgroup
+    const int t = (int)threadIdx.x;
+    if (t < 2 * 2 * kMaxLevels) {  // zero pads at both ends, both parities
+        const int par = t / (2 * kMaxLevels), side = (t / kMaxLevels) & 1, j = t % kMaxLevels;
+        ex[par][0][side].v[j] = make_float2(0.0f, 0.0f);
+        ex[par][p.wide_waves + 1][side].v[j] = make_float2(0.0f, 0.0f);
+    }
+    const int lane = t & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(t / kLanes);
+    const int grp = wblk % p.n_groups;
+    const int seg = wblk / p.n_groups;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int W = p.nx;
+    const int n_acc = (r1 - r0) * W;
+    if constexpr (IMG != 0) {
+        for (int i = t; i < n_acc; i += (int)blockDim.x) adjm_acc[i] = 0.0f;
+    }
+    // (the first barrier of the march orders these writes before any use)
+    const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
+    const bool ye = (r0 <= KL) || (r1 + KL - 2 >= p.ny - 1);
+    if (xe) {
+        if (ye) adjm_wave<KL, true, true, IMG>(p, lane, wave, grp, seg, ex);
+        else adjm_wave<KL, true, false, IMG>(p, lane, wave, grp, seg, ex);
+    } else {
+        if (ye) adjm_wave<KL, false, true, IMG>(p, lane, wave, grp, seg, ex);
+        else adjm_wave<KL, false, false, IMG>(p, lane, wave, grp, seg, ex);
+    }
+    if constexpr (IMG != 0) {
+        __syncthreads();
+        // partial[grp] += scale * segment image, per cell
+        const int x = wave * kLanes - p.wide_off + lane;
+        if ((x >= 0) & (x < W)) {
+            gf *gp = gstore_ptr(p.g_part) + (size_t)grp * ((size_t)p.ny * W);
+            for (int i = 0; i < r1 - r0; ++i) {
+                gf *cell = gp + (size_t)(r0 + i) * W + x;
+                *cell = __fmaf_rn(p.img_scale, adjm_acc[i * W + x], *cell);
+            }
+        }
+    }
+}
+
 // g[i] += sum over groups of g_part[k][i], in group order.
 __global__ void __launch_bounds__(256) reduce_groups_kernel(size_t plane, int n_groups,
                                                             const float *g_part, float *g)
@@ -1608,7 +1950,8 @@ __global__ void __launch_bounds__(256) image_kernel(StepParams p)
         const float fe = x < W - 1 ? f[i + 1] : 0.0f;
         const float fs = y > 0 ? f[i - W] : 0.0f;
         const float fn = y < H - 1 ? f[i + W] : 0.0f;
-        const float lf = lap5(fw, fe, fs, fn, f[i], p.idx2, p.idy2, p.cen);
+        // a Laplacian history holds lf itself
+        const float lf = p.hist_lap ? f[i] : lap5(fw, fe, fs, fn, f[i], p.idx2, p.idy2, p.cen);
         acc = __fmaf_rn(p.f_cur[soff + i], lf, acc);
     }
     p.g_out[i] = p.g_in[i] + p.img_scale * acc;
@@ -2068,7 +2411,16 @@ int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
                        hipStream_t stream)
 {
 #define SWFD_FWDM(SMV, FV) hipLaunchKernelGGL((fwdm_kernel<KL, SMV, FV>), grid, block, 0, stream, p)
-    if ((f & kOut2) != 0) {
+    if ((f & kLap) != 0) {
+        // Laplacian histories: the fields of the last two levels are stored
+        if (f != (kPoint | kLap) || sm != kStoreLastTwo)
+            return fail(SWFD_EINVAL, "Laplacian history: no copies, gathers or field history");
+        SWFD_FWDM(kStoreLastTwo, kPoint | kLap);
+        return SWFD_OK;
+    }
+    if constexpr (KL == 1) {
+        return fail(SWFD_EINVAL, "multi-step launch: one level only with Laplacian histories");
+    } else if ((f & kOut2) != 0) {
         if ((f & kGather) != 0) {
             if constexpr (KL == 2) SWFD_FWDM(kStoreAny, kPoint | kOut2 | kGather);
             else return fail(SWFD_EINVAL, "multi-step launch: gathers need two levels");
@@ -2089,10 +2441,12 @@ int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
             return fail(SWFD_EINVAL, "multi-step launch: gathers need two levels");
         }
     }
-    switch (sm) {
-        case kStoreAll: SWFD_FWDM(kStoreAll, kPoint); break;
-        case kStoreLastTwo: SWFD_FWDM(kStoreLastTwo, kPoint); break;
-        default: SWFD_FWDM(kStoreAny, kPoint); break;
+    if constexpr (KL >= 2) {
+        switch (sm) {
+            case kStoreAll: SWFD_FWDM(kStoreAll, kPoint); break;
+            case kStoreLastTwo: SWFD_FWDM(kStoreLastTwo, kPoint); break;
+            default: SWFD_FWDM(kStoreAny, kPoint); break;
+        }
     }
 #undef SWFD_FWDM
     return SWFD_OK;
@@ -2103,7 +2457,9 @@ int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
 // stored somewhere by the caller's plan).
 int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
 {
-    if (p.src_rows == nullptr || p.levels < 2 || p.levels > kMaxLevels || !wide_fits(p.nx))
+    const int min_levels = p.hist_lap ? 1 : 2;
+    if (p.src_rows == nullptr || p.levels < min_levels || p.levels > kMaxLevels ||
+        !wide_fits(p.nx))
         return fail(SWFD_EINVAL, "multi-step launch: needs point sources, 2-4 levels, nx <= 1023");
     if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
         return fail(SWFD_EINVAL, "multi-step launch: a field plane must be under 2 GiB");
@@ -2119,9 +2475,12 @@ int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
         last_two = last_two && (st == (j >= p.levels - 2));
     }
     if (gather_pairs_count > 0) f |= kGather;
-    const int sm = all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny);
+    if (p.hist_lap) f |= kLap;
+    const int sm = (p.hist_lap && last_two) ? kStoreLastTwo
+                   : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny));
     int rc;
     switch (p.levels) {
+        case 1: rc = launch_fwdm_levels<1>(p, sm, f, grid, block, stream); break;
         case 2: rc = launch_fwdm_levels<2>(p, sm, f, grid, block, stream); break;
         case 3: rc = launch_fwdm_levels<3>(p, sm, f, grid, block, stream); break;
         default: rc = launch_fwdm_levels<4>(p, sm, f, grid, block, stream); break;
@@ -2218,6 +2577,101 @@ int launch_adj_pair(StepParams &p, hipStream_t stream)
     return SWFD_OK;
 }
 
+// Largest LDS image of a row segment a full-width adjoint workgroup may use
+// (one such workgroup per CU).
+constexpr size_t kMaxAdjwLds = 128 * 1024;
+
+// Shape of a full-width multi-step adjoint launch: row segments of
+// SWFD_ADJW_SEG_ROWS rows (default 32, fewer when the segment's LDS image
+// would not fit) and at most kMaxImageGroups shot groups, one partial image
+// each.
+struct AdjwShape {
+    int n_seg, seg, n_groups, shots_per_group, acc_rows;
+};
+
+AdjwShape adjw_shape(int ny, int nx, int n_shots)
+{
+    static const int seg_env = std::max(2, env_int("SWFD_ADJW_SEG_ROWS", 32));
+    static const int groups_env = env_int("SWFD_ADJW_GROUPS", kMaxImageGroups);
+    const TuneEntry *t = find_tune(1, ny, nx);
+    const int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    const int groups_param = t != nullptr && t->param > 0 ? t->param : groups_env;
+    AdjwShape a{};
+    a.seg = seg_rows;
+    a.n_seg = std::max(1, ny / seg_rows);
+    const int max_rows = std::max(1, (int)(kMaxAdjwLds / (sizeof(float) * (size_t)nx)));
+    a.n_seg = std::max(a.n_seg, (ny + max_rows - 1) / max_rows);
+    a.acc_rows = (ny + a.n_seg - 1) / a.n_seg;
+    const int groups = std::max(1, std::min(std::min(groups_param, kMaxImageGroups), n_shots));
+    a.shots_per_group = (n_shots + groups - 1) / groups;
+    a.n_groups = (n_shots + a.shots_per_group - 1) / a.shots_per_group;
+    return a;
+}
+
+// p.levels (2..4) adjoint steps in one full-width launch: outputs of the
+// last two levels in p.lv_out[levels - 2], p.lv_out[levels - 1]; residuals
+// p.lv_resid[j]; imaging fields p.lv_in[j] (NULL: level not imaged).
+int launch_adj_multi(StepParams &p, hipStream_t stream)
+{
+    const int L = p.levels;
+    if (L < 2 || L > kMaxLevels || !wide_fits(p.nx) || p.cell_map == nullptr)
+        return fail(SWFD_EINVAL, "multi-step adjoint: 2-4 levels, nx <= 1023, scatter table");
+    if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
+        return fail(SWFD_EINVAL, "multi-step adjoint: a field plane must be under 2 GiB");
+    bool image = false;
+    for (int j = 0; j < L; ++j) {
+        if (p.lv_resid[j] == nullptr) return fail(SWFD_EINVAL, "multi-step adjoint: residual");
+        image = image || p.lv_in[j] != nullptr;
+    }
+    if (p.lv_out[L - 2] == nullptr || p.lv_out[L - 1] == nullptr)
+        return fail(SWFD_EINVAL, "multi-step adjoint needs the last two outputs");
+    if (image && p.g_part == nullptr)
+        return fail(SWFD_EINVAL, "multi-step adjoint imaging needs the partial images");
+    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots);
+    p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
+    p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
+    p.seg = a.seg;
+    p.n_seg = a.n_seg;
+    p.n_groups = a.n_groups;
+    p.shots_per_group = a.shots_per_group;
+    p.acc_rows = a.acc_rows;
+    p.n_work = p.n_seg * p.n_groups;
+    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    p.n_stencil_blocks = p.chunk * kXcds;
+    const size_t lds = image ? (size_t)a.acc_rows * p.nx * sizeof(float) : 0;
+    if (lds > kMaxAdjwLds)
+        return fail(SWFD_EINVAL, "multi-step adjoint: row segments too long for the LDS image");
+    const dim3 grid((unsigned)p.n_stencil_blocks);
+    const dim3 block((unsigned)(p.wide_waves * kLanes));
+    // dynamic LDS beyond 64 KiB has to be allowed per kernel (done once)
+#define SWFD_ADJM_IMG(KLV, IV)                                                          \
+    do {                                                                                 \
+        static bool big = false;                                                         \
+        if (!big) {                                                                      \
+            SWFD_HIP_CHECK(hipFuncSetAttribute(                                          \
+                reinterpret_cast<const void *>(&adjm_kernel<KLV, IV>),                    \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxAdjwLds));           \
+            big = true;                                                                  \
+        }                                                                                \
+        hipLaunchKernelGGL((adjm_kernel<KLV, IV>), grid, block, lds, stream, p);         \
+    } while (0)
+#define SWFD_ADJM(KLV)                                                                  \
+    do {                                                                                 \
+        if (!image) hipLaunchKernelGGL((adjm_kernel<KLV, 0>), grid, block, 0, stream, p); \
+        else if (p.hist_lap) SWFD_ADJM_IMG(KLV, 2);                                     \
+        else SWFD_ADJM_IMG(KLV, 1);                                                      \
+    } while (0)
+    switch (L) {
+        case 2: SWFD_ADJM(2); break;
+        case 3: SWFD_ADJM(3); break;
+        default: SWFD_ADJM(4); break;
+    }
+#undef SWFD_ADJM
+#undef SWFD_ADJM_IMG
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
 // Number of imaging partial slices a step of this shape uses.
 int image_groups(int ny, int nx, int n_shots)
 {
@@ -2226,10 +2680,12 @@ int image_groups(int ny, int nx, int n_shots)
 
 // Imaging of the field `lam` (S, ny, nx) against `img` into g_out.
 int launch_image(const swfd_grid *grid, int n_shots, const float *lam, const float *img,
-                 const float *g_in, float *g_out, float scale, hipStream_t stream)
+                 const float *g_in, float *g_out, float scale, hipStream_t stream,
+                 bool lap = false)
 {
     StepParams q;
     base_params(q, grid, nullptr, n_shots);
+    q.hist_lap = lap ? 1 : 0;
     q.f_cur = lam;
     q.img = img;
     q.g_in = g_in;
@@ -2360,6 +2816,7 @@ struct ForwardRun {
     float *work[4] = {nullptr, nullptr, nullptr, nullptr};
     int max_levels = 1;  // steps per launch
     bool wide = false;   // full-width launches (else 64-column blocks, two steps)
+    bool lap = false;    // history slots receive Laplacians of the fields
     const float *p_old = nullptr, *p_cur = nullptr;
     int k = 0;
     // fields whose receiver samples are still to be taken (the last one or
@@ -2389,6 +2846,12 @@ struct ForwardRun {
         wide = plan.wide && !batched;
         // receiver samples are gathered from stored fields: two levels at most
         if (batched || !wide || a->samples != nullptr) max_levels = std::min(max_levels, 2);
+        lap = a->hist_lap != 0;
+        if (lap && (!wide || max_levels < 2))
+            return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step launches "
+                                     "(see swfd_lap_history) and work3");
+        if (lap && (a->hist == nullptr || a->samples != nullptr || a->ckpt != nullptr))
+            return fail(SWFD_EINVAL, "Laplacian history: needs hist, no samples or checkpoints");
         p_old = a->init_old;
         p_cur = a->init_u;
         return SWFD_OK;
@@ -2396,7 +2859,7 @@ struct ForwardRun {
     bool done() const { return k >= a->n_steps; }
     int next_levels() const { return std::min(max_levels, a->n_steps - k); }
     bool next_is_pair() const { return next_levels() >= 2; }
-    bool next_is_wide() const { return wide && next_levels() >= 2; }
+    bool next_is_wide() const { return wide && (next_levels() >= 2 || lap); }
     // output of step n: its history slot, else a work buffer chosen later
     float *hist_out(int n) const
     {
@@ -2443,10 +2906,26 @@ struct ForwardRun {
         }
         const int L = next_levels();
         p.levels = L;
-        for (int j = 0; j < kMaxLevels; ++j) p.lv_out[j] = p.lv_out2[j] = nullptr;
+        for (int j = 0; j < kMaxLevels; ++j) p.lv_out[j] = p.lv_out2[j] = p.lv_lap[j] = nullptr;
         // the last two levels become the next state: a history slot or a
         // work buffer other than the current state
         const float *taken = nullptr;
+        if (lap) {
+            // level j's input is the output of step n + j - 1: its Laplacian
+            // goes to that step's slot; the fields stay in work buffers
+            p.hist_lap = 1;
+            for (int j = 0; j < L; ++j) {
+                p.lv_lap[j] = hist_out(n + j - 1);
+                if (j >= L - 2) {
+                    p.lv_out[j] = free_work(work, p_old, p_cur, taken);
+                    taken = p.lv_out[j];
+                }
+            }
+            p.f_out = p.lv_out[0];
+            if (L >= 2) p.f_out_b = p.lv_out[1];
+            if (L == 1) p.f_out = p.lv_out[0];
+            return gathers;
+        }
         for (int j = 0; j < L; ++j) {
             float *out = hist_out(n + j);
             if (out == nullptr && j >= L - 2) {
@@ -2524,6 +3003,33 @@ int check_adjoint_args(const swfd_grid *grid, const swfd_medium *medium,
 // step images the step before it (its input lam), and the last step is
 // imaged after the run. With two-step launches only the last step (when the
 // count is odd) runs alone, so no step is imaged twice.
+// Steps per launch and form of the adjoint runs of a grid: a tuned kind-3
+// entry (depth = steps per launch, param 1 = full width), else blocks when
+// the kind-2 entry asks for two-step blocks, else full-width launches of
+// SWFD_ADJ_LEVELS (default 4) steps where a row fits and the scatter table is
+// there. No fourth work buffer, SWFD_TWO_STEP=0 or a kind-2 depth of 1:
+// single steps.
+ForwardPlan adjoint_plan(int ny, int nx, const float *work3, const swfd_receivers *rec)
+{
+    static const int env_levels = std::min(std::max(env_int("SWFD_ADJ_LEVELS", 4), 1), kMaxLevels);
+    static const bool env_wide = env_int("SWFD_WIDE", 1) != 0;
+    ForwardPlan f{1, false};
+    if (!two_step_allowed(ny, nx, work3)) return f;
+    const bool fits = wide_fits(nx) && rec != nullptr && rec->cell_map != nullptr;
+    const TuneEntry *t3 = find_tune(3, ny, nx);
+    if (t3 != nullptr && t3->depth >= 1) {
+        f.levels = std::min(t3->depth, kMaxLevels);
+        f.wide = fits && f.levels >= 2 && (t3->param == 1 || f.levels > 2);
+        if (!f.wide) f.levels = std::min(f.levels, 2);
+        return f;
+    }
+    const TuneEntry *t = find_tune(2, ny, nx);
+    if (t != nullptr && t->depth == 2 && t->param != 1) return ForwardPlan{2, false};
+    f.wide = fits && env_wide && env_levels >= 2;
+    f.levels = f.wide ? env_levels : std::min(env_levels, 2);
+    return f;
+}
+
 struct AdjointRun {
     const swfd_grid *grid = nullptr;
     const swfd_medium *medium = nullptr;
@@ -2531,14 +3037,18 @@ struct AdjointRun {
     const swfd_adjoint_args *a = nullptr;
     size_t field = 0, plane = 0, sr = 0;
     float *work[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool two_step = false;
+    int max_levels = 1;  // steps per launch
+    bool wide = false;   // full-width launches (else 64-column blocks, two steps)
+    bool last_multi = false;  // the last launch imaged its own outputs
+    bool lap = false;         // history slots hold Laplacians of the fields
     int n_part = 0;
     bool partials_used = false;
     const float *p_old = nullptr, *p_cur = nullptr;
     int n = 0;  // next step to run (descending)
 
+    // batched: the lockstep tables hold two-step block launches only
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
-             const swfd_adjoint_args *args)
+             const swfd_adjoint_args *args, bool batched = false)
     {
         int rc;
         if ((rc = check_adjoint_args(g, m, r, args))) return rc;
@@ -2553,16 +3063,28 @@ struct AdjointRun {
         work[1] = a->work1;
         work[2] = a->work2;
         work[3] = a->work3;
-        two_step = two_step_allowed(g->ny, g->nx, a->work3);
-        n_part = two_step ? adj2_shape(g->ny, g->nx, a->n_shots).n_groups
-                          : image_groups(g->ny, g->nx, a->n_shots);
+        const ForwardPlan plan = adjoint_plan(g->ny, g->nx, a->work3, r);
+        max_levels = plan.levels;
+        wide = plan.wide && !batched;
+        if (!wide) max_levels = std::min(max_levels, 2);
+        if (max_levels >= 2)
+            n_part = wide ? adjw_shape(g->ny, g->nx, a->n_shots).n_groups
+                          : adj2_shape(g->ny, g->nx, a->n_shots).n_groups;
+        else
+            n_part = image_groups(g->ny, g->nx, a->n_shots);
+        lap = a->hist_lap != 0;
+        if (lap && (!wide || max_levels < 2))
+            return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step adjoint "
+                                     "launches (see swfd_lap_history) and work3");
         p_old = a->init_old;
         p_cur = a->init_u;
         n = a->step_hi - 1;
         return SWFD_OK;
     }
     bool done() const { return n < a->step_lo; }
-    bool next_is_pair() const { return two_step && n - 1 >= a->step_lo; }
+    int next_levels() const { return std::min(max_levels, n - a->step_lo + 1); }
+    bool next_is_pair() const { return next_levels() >= 2; }
+    bool next_is_wide() const { return wide && next_levels() >= 2; }
     long image_slot(int m) const
     {
         if (a->hist == nullptr || m < 1) return -1;
@@ -2580,32 +3102,47 @@ struct AdjointRun {
         p.f_old = p_old;
         p.f_cur = p_cur;
         p.resid = a->residual + (size_t)n * sr;
+        p.hist_lap = lap ? 1 : 0;
         auto use_partials = [&]() {
             if (!partials_used) fill = true;
             partials_used = true;
             p.g_part = a->workspace;
             p.img_scale = a->image_scale;
         };
-        if (next_is_pair()) {
+        const int L = next_levels();
+        p.levels = L;
+        if (L >= 2) {
+            // level j is step n - j, imaged against the forward field before it
             float *out_c = free_work(work, p_old, p_cur);
             float *out_d = free_work(work, p_old, p_cur, out_c);
             if (out_c == nullptr || out_d == nullptr)
                 return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+            bool any = false;
+            for (int j = 0; j < L; ++j) {
+                p.lv_resid[j] = a->residual + (size_t)(n - j) * sr;
+                const long slot = image_slot(n - j);
+                if (slot >= a->hist_slots)
+                    return fail(SWFD_EINVAL, "adjoint: history slot out of range");
+                if (slot >= 0) {
+                    p.lv_in[j] = a->hist + (size_t)slot * field;
+                    any = true;
+                }
+            }
+            p.lv_out[L - 2] = out_c;
+            p.lv_out[L - 1] = out_d;
+            // two-step block launches
             p.f_out = out_c;
             p.f_out_b = out_d;
             p.resid_b = a->residual + (size_t)(n - 1) * sr;
-            const long s1 = image_slot(n), s2 = image_slot(n - 1);
-            if (s1 >= a->hist_slots || s2 >= a->hist_slots)
-                return fail(SWFD_EINVAL, "adjoint: history slot out of range");
-            if (s1 >= 0) p.img = a->hist + (size_t)s1 * field;
-            if (s2 >= 0) p.img_b = a->hist + (size_t)s2 * field;
-            if (s1 >= 0 || s2 >= 0) use_partials();
+            p.img = p.lv_in[0];
+            p.img_b = p.lv_in[1];
+            if (any) use_partials();
             return SWFD_OK;
         }
         float *out = free_work(work, p_old, p_cur);
         if (out == nullptr) return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
         p.f_out = out;
-        if (!two_step && n < a->step_hi - 1) {
+        if (max_levels < 2 && n < a->step_hi - 1) {
             const long slot = image_slot(n + 1);
             if (slot >= a->hist_slots)
                 return fail(SWFD_EINVAL, "adjoint: history slot out of range");
@@ -2616,17 +3153,18 @@ struct AdjointRun {
         }
         return SWFD_OK;
     }
-    void advance(const StepParams &p, bool pair)
+    void advance(const StepParams &p)
     {
-        if (pair) {
-            p_old = p.f_out;
-            p_cur = p.f_out_b;
-            n -= 2;
+        const int L = p.levels;
+        if (L >= 2) {
+            p_old = p.lv_out[L - 2];
+            p_cur = p.lv_out[L - 1];
         } else {
             p_old = p_cur;
             p_cur = p.f_out;
-            n -= 1;
         }
+        last_multi = L >= 2;
+        n -= L;
     }
     int launch_fill(hipStream_t st) const
     {
@@ -2644,15 +3182,14 @@ struct AdjointRun {
                                dim3(256), 0, st, plane, n_part, a->workspace, a->grad);
             SWFD_HIP_CHECK(hipGetLastError());
         }
-        // the last step, unless a two-step launch imaged it
-        const bool last_imaged = two_step && (a->step_hi - a->step_lo) % 2 == 0;
-        if (a->step_hi > a->step_lo && !last_imaged) {
+        // the last step, unless a multi-step launch imaged it
+        if (a->step_hi > a->step_lo && !last_multi) {
             const long slot = image_slot(a->step_lo);
             if (slot >= a->hist_slots)
                 return fail(SWFD_EINVAL, "adjoint: history slot out of range");
             if (slot >= 0 &&
                 (rc = launch_image(grid, a->n_shots, p_cur, a->hist + (size_t)slot * field,
-                                   a->grad, a->grad, a->image_scale, st)))
+                                   a->grad, a->grad, a->image_scale, st, lap)))
                 return rc;
         }
         return settle_state(p_old, p_cur, a->final_old, a->final_u, work, field, st);
@@ -2763,7 +3300,7 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
         return fail(SWFD_EINVAL, "adjoint batch: bad arguments");
     std::vector<AdjointRun> runs((size_t)n_runs);
     for (int i = 0; i < n_runs; ++i)
-        if ((rc = runs[i].init(&grids[i], &media[i], &recs[i], &args[i]))) return rc;
+        if ((rc = runs[i].init(&grids[i], &media[i], &recs[i], &args[i], true))) return rc;
     hipStream_t st = as_stream(stream);
     std::vector<char> host;
     size_t off = 0;
@@ -2795,7 +3332,7 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 pair_runs.push_back(i);
             } else {
                 if (mode == 2 && (rc = launch_step(p, true, st, 0))) return rc;
-                r.advance(p, false);
+                r.advance(p);
             }
         }
         if (!any) break;
@@ -2825,7 +3362,7 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 SWFD_HIP_CHECK(hipGetLastError());
             }
             off += nb;
-            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j], true);
+            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j]);
         }
     }
     if (bytes_out != nullptr) *bytes_out = off;
@@ -3006,9 +3543,11 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         bool fill = false;
         if ((rc = run.build(p, fill))) return rc;
         if (fill && (rc = run.launch_fill(st))) return rc;
-        const bool pair = run.next_is_pair();
-        if ((rc = pair ? launch_adj_pair(p, st) : launch_step(p, true, st, 0))) return rc;
-        run.advance(p, pair);
+        if (run.next_is_wide()) rc = launch_adj_multi(p, st);
+        else if (run.next_is_pair()) rc = launch_adj_pair(p, st);
+        else rc = launch_step(p, true, st, 0);
+        if (rc) return rc;
+        run.advance(p);
     }
     return run.finish(st);
 }
@@ -3060,8 +3599,8 @@ int swfd_adjoint_batch(int32_t n, const swfd_grid *grids, const swfd_medium *med
 int swfd_tune(int32_t kind, int32_t ny, int32_t nx, int32_t seg_rows, int32_t param,
               int32_t depth)
 {
-    if (kind < 0 || kind > 2 || ny < 3 || nx < 3)
-        return fail(SWFD_EINVAL, "tune: kind must be 0..2 and the grid at least 3 x 3");
+    if (kind < 0 || kind > 3 || ny < 3 || nx < 3)
+        return fail(SWFD_EINVAL, "tune: kind must be 0..3 and the grid at least 3 x 3");
     for (TuneEntry &t : g_tune)
         if (t.kind == kind && t.ny == ny && t.nx == nx) {
             t.seg_rows = seg_rows;
@@ -3074,6 +3613,33 @@ int swfd_tune(int32_t kind, int32_t ny, int32_t nx, int32_t seg_rows, int32_t pa
 }
 
 void swfd_tune_clear(void) { g_tune.clear(); }
+
+int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samples)
+{
+    if ((kind != 0 && kind != 1) || ny < 3 || nx < 3) return 0;
+    static const int32_t any_map = 0;
+    swfd_receivers rec{};
+    rec.cell_map = &any_map;
+    static const float any_buffer = 0.0f;
+    float *w3 = const_cast<float *>(&any_buffer);
+    if (kind == 1) return adjoint_plan(ny, nx, w3, &rec).levels;
+    const ForwardPlan f = forward_plan(ny, nx, w3);
+    int levels = f.levels;
+    if (!f.wide || samples) levels = std::min(levels, 2);
+    return levels;
+}
+
+int32_t swfd_lap_history(int32_t ny, int32_t nx)
+{
+    if (ny < 3 || nx < 3) return 0;
+    static const int32_t any_map = 0;
+    swfd_receivers rec{};
+    rec.cell_map = &any_map;  // the adjoint's full-width launches need a scatter table
+    static const float any_buffer = 0.0f;
+    float *w3 = const_cast<float *>(&any_buffer);
+    const ForwardPlan f = forward_plan(ny, nx, w3), a = adjoint_plan(ny, nx, w3, &rec);
+    return (f.wide && f.levels >= 2 && a.wide && a.levels >= 2) ? 1 : 0;
+}
 
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots)
 {

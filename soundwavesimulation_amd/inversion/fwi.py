@@ -18,6 +18,7 @@ The reference's approximate options (``store_every > 1``, reduced-precision
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import torch
@@ -139,12 +140,25 @@ class CheckpointedGradient:
                     final=False)
         return self.samples
 
+    def lap_history(self):
+        """Whether the recomputation stores Laplacians of the fields (the
+        adjoint then images with one product per cell; same arithmetic, so the
+        same gradient). Opt-in with SWFD_LAP_HISTORY=1: on MI355X at 1024^2 the
+        recomputation's two extra field writes per launch cost more than the
+        adjoint saves."""
+        if os.environ.get("SWFD_LAP_HISTORY", "0") != "1":
+            return False
+        ny, nx = self.solver.model.shape
+        return bool(_native.lib().swfd_lap_history(ny, nx))
+
     def backward(self, residual_nsr, grad, image_scale):
         """Accumulate the imaging condition of every step into ``grad`` (ny, nx)."""
         K, N = self.K, self.n_steps
         for t in self.lam[:2]:
             t.zero_()
         state = (self.lam[0], self.lam[1])
+        if self.lap_history():
+            return self._backward_lap(residual_nsr, grad, image_scale, state)
         for j in range(self.n_seg - 1, -1, -1):
             n0 = j * K
             n1 = min(n0 + K, N)
@@ -164,6 +178,29 @@ class CheckpointedGradient:
                         state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
                         hist_every=1, image_scale=image_scale, grad=grad,
                         workspace=self.workspace)
+        return grad
+
+    def _backward_lap(self, residual_nsr, grad, image_scale, state):
+        # Each segment recomputes one step more than the field history needs,
+        # so that the Laplacian of its last field is taken too; the
+        # checkpoints (or zeros) are read in place.
+        K, N = self.K, self.n_steps
+        for j in range(self.n_seg - 1, -1, -1):
+            n0 = j * K
+            n1 = min(n0 + K, N)
+            if j == 0:
+                self.work[0].zero_()
+                self.work[1].zero_()
+                init = (self.work[0], self.work[1])
+            else:
+                init = (self.ckpt[2 * (j - 1)], self.ckpt[2 * (j - 1) + 1])
+            forward_run(self.solver, self.n_shots, self.rows, self.cols, self.values, n1 - n0,
+                        init=init, hist=self.hist, hist_first=n0 - 2, hist_every=1,
+                        step_begin=n0, work=self.work, final=False, hist_lap=True)
+            adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
+                        state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
+                        hist_every=1, image_scale=image_scale, grad=grad,
+                        workspace=self.workspace, hist_lap=True)
         return grad
 
 

@@ -458,13 +458,15 @@ def _source_values_tensor(pulse, n_steps, device, source_values=None):
 
 def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=None,
                 samples=None, sampler=None, hist=None, hist_first=0, hist_every=1, ckpt=None,
-                ckpt_every=0, step_begin=0, work=None, final=True):
+                ckpt_every=0, step_begin=0, work=None, final=True, hist_lap=False):
     """Run forward steps in one native call; returns the final ``(u_old, u)``.
 
     ``init`` is the state before ``step_begin`` (zeros if None); ``samples``
     (N, S, R), ``hist`` (slots, S, ny, nx) and ``ckpt`` (slots, S, ny, nx) are
     optional outputs (see ``swfd_forward_args``). ``work`` holds three or four
-    scratch fields; with four the run advances two steps per launch.
+    scratch fields; with four the run advances several steps per launch.
+    ``hist_lap``: the history receives the Laplacians of the fields (of
+    ``init`` and every output but the last) for ``adjoint_run(hist_lap=True)``.
     """
     model = solver.model
     dev = model.device
@@ -488,7 +490,7 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
         hist=_native.ptr(hist), hist_first=hist_first, hist_every=hist_every,
         hist_slots=0 if hist is None else hist.shape[0],
         ckpt=_native.ptr(ckpt), ckpt_every=ckpt_every,
-        ckpt_slots=0 if ckpt is None else ckpt.shape[0])
+        ckpt_slots=0 if ckpt is None else ckpt.shape[0], hist_lap=1 if hist_lap else 0)
     rec = None
     if samples is not None:
         if sampler is None:
@@ -509,13 +511,14 @@ def adjoint_workspace(solver, n_shots):
 
 def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, state, work,
                 hist=None, hist_first=0, hist_every=1, image_scale=0.0, grad=None,
-                workspace=None):
+                workspace=None, hist_lap=False):
     """Adjoint steps ``step_hi-1 .. step_lo`` in one native call.
 
     ``state`` = (lam_old, lam) before the run; the state after the run is
     written back into ``state``'s tensors. ``work`` holds three or four
     scratch fields (may include the state tensors); with four the run takes
-    two steps per launch.
+    several steps per launch. ``hist_lap``: the history holds Laplacians
+    (``forward_run(hist_lap=True)``).
     """
     dev = solver.model.device
     solver._sync_medium()
@@ -530,7 +533,7 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
         residual=_native.ptr(residual_nsr), hist=_native.ptr(hist), hist_first=hist_first,
         hist_every=hist_every, hist_slots=0 if hist is None else hist.shape[0],
         image_scale=float(image_scale), grad=_native.ptr(grad),
-        workspace=_native.ptr(workspace))
+        workspace=_native.ptr(workspace), hist_lap=1 if hist_lap else 0)
     _native.check(_native.lib().swfd_adjoint(solver._grid, solver._medium, sampler._rec, args,
                                              _native.stream_of(dev)))
 

@@ -77,7 +77,20 @@ def test_batched_forward_equals_separate_runs():
         assert torch.equal(w2[0], w[0]) and torch.equal(w2[1], w[1])
 
 
-def test_batched_adjoint_equals_separate_runs():
+@pytest.fixture
+def adjoint_blocks():
+    """Separate adjoint runs in the two-step block form, the form the batched
+    launches use (full-width runs sum the gradient in another order)."""
+    from soundwavesimulation_amd import _native
+
+    lib = _native.lib()
+    for ny, nx, _, _ in SPECS:
+        _native.check(lib.swfd_tune(3, ny, nx, 0, 0, 2))
+    yield
+    lib.swfd_tune_clear()
+
+
+def test_batched_adjoint_equals_separate_runs(adjoint_blocks):
     print("batched adjoint, 3 problems", flush=True)
     runs, refs = [], []
     for i, (ny, nx, n_el, N) in enumerate(SPECS):

@@ -1,4 +1,4 @@
-"""Multi-step forward and two-step adjoint launches against single-step launches.
+"""Multi-step forward and adjoint launches against single-step launches.
 
 ``swfd_forward`` advances several steps per launch when it is given a fourth
 work buffer: two per launch in 64-column blocks, or two to four per launch in
@@ -94,6 +94,28 @@ def _use_form(form, ny, nx):
         _native.check(lib.swfd_tune(2, ny, nx, 0, 1, int(form[-1])))
 
 
+@pytest.fixture(params=["wide2", "wide3", "wide4", "blocks"])
+def adj_form(request):
+    """Adjoint launch form: full-width workgroups with 2 (the default), 3 or 4
+    steps per launch, or two-step 64-column blocks."""
+    from soundwavesimulation_amd import _native
+
+    lib = _native.lib()
+    yield request.param
+    lib.swfd_tune_clear()
+
+
+def _use_adjoint_form(form, ny, nx):
+    from soundwavesimulation_amd import _native
+
+    lib = _native.lib()
+    lib.swfd_tune_clear()
+    if form == "blocks":
+        _native.check(lib.swfd_tune(3, ny, nx, 0, 0, 2))
+    else:
+        _native.check(lib.swfd_tune(3, ny, nx, 0, 1, int(form[-1])))
+
+
 @pytest.mark.parametrize("ny,nx,n_steps", [
     (64, 64, 41),     # one row segment, odd step count
     (200, 63, 30),    # column W-1 on the last lane of the first block; merged tail segment
@@ -176,10 +198,14 @@ def _adjoint(solver, sampler, residual, step_hi, step_lo, n_work, *, init, hist=
     (257, 125, 10, 2),  # every other step has no imaging field
     (130, 190, 7, 1),
     (5, 7, 6, 1),
+    (70, 129, 11, 1),   # nx = 64 k + 1
+    (40, 1000, 9, 1),   # sixteen wavefronts per row
 ])
-def test_two_step_adjoint_matches_single_step(ny, nx, n_steps, hist_every):
+def test_two_step_adjoint_matches_single_step(ny, nx, n_steps, hist_every, adj_form):
     """Adjoint state bit for bit; the gradient differs only by summation order."""
-    print(f"two-step adjoint {ny}x{nx}, {n_steps} steps, hist_every {hist_every}", flush=True)
+    print(f"multi-step adjoint ({adj_form}) {ny}x{nx}, {n_steps} steps, hist_every {hist_every}",
+          flush=True)
+    _use_adjoint_form(adj_form, ny, nx)
     solver, sp = _medium(ny, nx, seed=ny + 3 * nx)
     S = 6
     if min(ny, nx) >= 40:
@@ -210,3 +236,78 @@ def test_two_step_adjoint_matches_single_step(ny, nx, n_steps, hist_every):
     o3, u3, _ = _adjoint(solver, sampler, residual, step_hi, step_lo, 4, init=init)
     o4, u4, _ = _adjoint(solver, sampler, residual, step_hi, step_lo, 3, init=init)
     assert torch.equal(o3, o4) and torch.equal(u3, u4)
+
+
+def _lap64(f, sp):
+    """Float64 5-point Laplacian with zero outside the grid (last two axes)."""
+    f = f.astype(np.float64)
+    pad = np.zeros(f.shape[:-2] + (f.shape[-2] + 2, f.shape[-1] + 2))
+    pad[..., 1:-1, 1:-1] = f
+    inv = 1.0 / (sp * sp)
+    return inv * (pad[..., 1:-1, :-2] + pad[..., 1:-1, 2:] + pad[..., :-2, 1:-1]
+                  + pad[..., 2:, 1:-1] - 4.0 * f)
+
+
+@pytest.mark.parametrize("ny,nx,K", [(64, 64, 10), (70, 129, 9), (130, 190, 11), (40, 1000, 7),
+                                     (5, 7, 6)])
+def test_laplacian_history_matches_field_history(ny, nx, K, adj_form):
+    """Laplacian histories (forward_run/adjoint_run hist_lap=True): the slots
+    hold the zero-padded Laplacians of the field history, and the adjoint run
+    that images with them gives the same state bits and gradient as the
+    single-step adjoint imaging against the fields."""
+    from soundwavesimulation_amd import _native
+
+    if adj_form == "blocks":
+        pytest.skip("Laplacian histories need full-width adjoint launches")
+    if not _native.lib().swfd_lap_history(ny, nx):
+        pytest.skip("no full-width launches for this grid")
+    print(f"Laplacian history ({adj_form}) {ny}x{nx}, segment of {K} steps", flush=True)
+    _use_adjoint_form(adj_form, ny, nx)
+    solver, sp = _medium(ny, nx, seed=5 * ny + nx)
+    S, n0 = 5, 3
+    n1 = n0 + K
+    x, y = axis_centers(nx, sp), axis_centers(ny, sp)
+    big = min(ny, nx) >= 40
+    array = RingArray(8 if big else 3, (0.35 if big else 0.3) * min(ny, nx) * sp, x, y)
+    sampler = TorchRingSampler(array, device=DEV)
+    rows_t = torch.tensor([1, ny - 2, ny // 2, 0, ny - 1], dtype=torch.int32, device=DEV)
+    cols_t = torch.tensor([1, nx - 2, nx // 2, nx - 1, 0], dtype=torch.int32, device=DEV)
+    values = _pulse(n1 + 1, seed=K)
+    g = torch.Generator().manual_seed(ny + nx + K)
+    init = tuple(torch.randn((S, ny, nx), generator=g, dtype=F32).to(DEV) for _ in range(2))
+    shape = (S, ny, nx)
+    # field history (single steps): slot k holds the output of step n0 - 2 + k
+    hist_f = torch.zeros((K + 1, *shape), device=DEV)
+    hist_f[0].copy_(init[0])
+    hist_f[1].copy_(init[1])
+    work3 = [torch.empty(shape, device=DEV) for _ in range(3)]
+    forward_run(solver, S, rows_t, cols_t, values, K - 1, init=init, hist=hist_f,
+                hist_first=n0 - 2, step_begin=n0, work=work3, final=False)
+    # Laplacian history: one step more, slots 1..K
+    hist_l = torch.full((K + 1, *shape), float("nan"), device=DEV)
+    work4 = [torch.empty(shape, device=DEV) for _ in range(4)]
+    forward_run(solver, S, rows_t, cols_t, values, K, init=init, hist=hist_l,
+                hist_first=n0 - 2, step_begin=n0, work=work4, final=False, hist_lap=True)
+    torch.cuda.synchronize()
+    ref = _lap64(hist_f[1:].cpu().numpy(), sp)
+    got = hist_l[1:].cpu().numpy()
+    scale = float(np.abs(ref).max())
+    err = float(np.abs(got - ref).max())
+    print(f"  Laplacian slots: max |diff| {err:.3e} of {scale:.3e}")
+    assert np.isfinite(got).all() and err <= 1e-5 * scale
+    # adjoint over [n0, n1) imaging with either history
+    R = sampler.n_elements
+    residual = torch.randn((n1 + 1, S, R), generator=g, dtype=F32).to(DEV)
+    lam0 = tuple(torch.randn(shape, generator=g, dtype=F32).to(DEV) for _ in range(2))
+    kw = dict(init=lam0, hist_first=n0 - 2)
+    o1, u1, g1 = _adjoint(solver, sampler, residual, n1, n0, 3, hist=hist_f, **kw)
+    st = (lam0[0].clone(), lam0[1].clone())
+    work = [st[0], st[1], torch.empty(shape, device=DEV), torch.empty(shape, device=DEV)]
+    g2 = torch.zeros((ny, nx), device=DEV)
+    adjoint_run(solver, sampler, S, residual, n1, n0, state=st, work=work, hist=hist_l,
+                hist_first=n0 - 2, image_scale=1e-3, grad=g2, hist_lap=True)
+    torch.cuda.synchronize()
+    assert torch.equal(st[0], o1) and torch.equal(st[1], u1)
+    rel = float((g2 - g1).norm() / g1.norm())
+    print(f"  gradient rel-L2 {rel:.2e}")
+    assert float(g1.abs().max()) > 0.0 and rel <= 1e-6
