@@ -162,16 +162,19 @@ class CheckpointedGradient:
         for j in range(self.n_seg - 1, -1, -1):
             n0 = j * K
             n1 = min(n0 + K, N)
+            # history slot 1 holds the field before step n0 (imaged by step n0);
+            # the older state of the pair is read where it is
             if j == 0:
                 self.hist[0].zero_()
                 self.hist[1].zero_()
+                init = (self.hist[0], self.hist[1])
             else:
-                self.hist[0].copy_(self.ckpt[2 * (j - 1)])
                 self.hist[1].copy_(self.ckpt[2 * (j - 1) + 1])
+                init = (self.ckpt[2 * (j - 1)], self.hist[1])
             n_re = n1 - 1 - n0
             if n_re > 0:
                 forward_run(self.solver, self.n_shots, self.rows, self.cols, self.values, n_re,
-                            init=(self.hist[0], self.hist[1]), hist=self.hist,
+                            init=init, hist=self.hist,
                             hist_first=n0 - 2, hist_every=1, step_begin=n0, work=self.work,
                             final=False)
             adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
