@@ -98,7 +98,10 @@ typedef struct swfd_step_args {
  * :736-770, forward_store(_batch) invert_ring.py:395-455).
  *   init_old/init_u: state before step_begin (read only).
  *   work0..2: scratch fields; work3 (optional) a fourth one: with it the run
- *   advances two steps per launch (same results, fewer field reads).
+ *   advances several steps per launch (same results, fewer field reads);
+ *   with samples, two more (work4, work5) let full-width runs take more than
+ *   two steps per launch (the intermediate steps' fields are kept there for
+ *   the gather).
  *   final_old/final_u (optional) receive the state after the last step (they
  *   may be work0/work1 or the init buffers).
  *   src_values: per global step, length >= step_begin + n_steps.
@@ -127,6 +130,7 @@ typedef struct swfd_forward_args {
     float *ckpt;
     int32_t ckpt_every, ckpt_slots;
     int32_t hist_lap; /* 1: history slots receive Laplacians (see above) */
+    float *work4, *work5; /* optional: with samples, up to 4 steps per launch */
 } swfd_forward_args;
 
 /* A run of adjoint steps n = step_hi - 1 down to step_lo with residual

@@ -55,7 +55,7 @@ class ForwardArgs(ctypes.Structure):
                 ("src_cols", _fp), ("src_values", _fp), ("samples", _fp), ("hist", _fp),
                 ("hist_first", _c_int), ("hist_every", _c_int), ("hist_slots", _c_int),
                 ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int),
-                ("hist_lap", _c_int)]
+                ("hist_lap", _c_int), ("work4", _fp), ("work5", _fp)]
 
 
 class AdjointArgs(ctypes.Structure):

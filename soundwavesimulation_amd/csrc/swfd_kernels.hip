@@ -114,6 +114,8 @@ struct StepParams {
     int hist_lap;
     // multi-step adjoint launches: each level's imaging field and residual
     const float *lv_in[4], *lv_resid[4];
+    // receiver samples of each level of a multi-step forward launch (S, R)
+    float *lv_samples[4];
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -331,6 +333,13 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int s
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 __device__ __forceinline__ gf *gstore_ptr(float *p) { return (gf *)p; }
+
+// 32-bit load at a lane byte offset from a wavefront-uniform base (a global
+// load with the base in SGPRs and a 32-bit lane offset).
+__device__ __forceinline__ float gload_off(const float *base, int voff)
+{
+    return *(const gcf *)((const char SWFD_GLOBAL *)base + (unsigned)voff);
+}
 
 // Work-item order (p.order): 0 = strip, column group, shot group (fastest);
 // 1 = shot group, strip, column group (fastest): a workgroup's wavefronts
@@ -614,6 +623,36 @@ __device__ __forceinline__ void seg_rows(const StepParams &p, int seg, int &r0, 
 #define SWFD_PREFETCH_ROWS 2
 #endif
 constexpr int kPf = SWFD_PREFETCH_ROWS;  // rows loaded ahead of the march
+
+// Residual injection for the lanes of a wavefront whose cell has scatter
+// entry e >= 0 (others pass e = -1): lane by lane over the flagged lanes, with
+// the entry's (receiver, weight) pairs and the residuals read through scalar
+// loads (wavefront-uniform addresses), so no wait on the vector memory
+// pipeline (which holds the march's prefetched rows) is needed. Per cell the
+// sum is the single-step kernel's: a = 0, a += res[rec_k] * w_k in entry order.
+typedef const int __attribute__((address_space(4))) kci;    // read-only, scalar-loadable
+typedef const float __attribute__((address_space(4))) kcf;
+
+__device__ __forceinline__ float inject_lanes(const StepParams &p, const float *res,
+                                              size_t res_off, int e, int lane, float v)
+{
+    unsigned long long m = __ballot(e >= 0);
+    if (m == 0) return v;
+    const kci *start = (const kci *)p.entry_start;
+    const kci *rec = (const kci *)p.pair_rec;
+    const kcf *w = (const kcf *)p.pair_w;
+    const kcf *r = (const kcf *)res + res_off;
+    do {
+        const int l = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        const int eu = __builtin_amdgcn_readlane(e, l);
+        float a = 0.0f;
+        const int k1 = start[eu + 1];
+        for (int k = start[eu]; k < k1; ++k) a += r[rec[k]] * w[k];
+        v = lane == l ? v + a : v;
+    } while (m != 0);
+    return v;
+}
 
 // Mur update of a top/bottom edge cell (x in [1, W-2]) and of the corners of
 // that row, from the row's current-time values `cur_r`, the interior-side
@@ -1669,9 +1708,11 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     const int row_bytes = W * 4;
     auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * row_bytes; };
     const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
-    const __amdgpu_buffer_rsrc_t re = make_rsrc((const float *)p.cell_map, plane_bytes);
+    // the scatter-table map, Mur side coefficients and imaging fields are read
+    // with global loads off a scalar base (2 SGPRs each instead of a 4-SGPR
+    // buffer descriptor: the adjoint is short of SGPRs)
+    const float *emap = (const float *)p.cell_map;
     const float *mur = p.mur;
-    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
     const int mo = (x == 0) ? 0 : H * 4;
     auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
     const float dt2 = p.dt2;
@@ -1693,24 +1734,19 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
         const size_t soff = (size_t)s * plane;
         const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
         const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
-        __amdgpu_buffer_rsrc_t ro[KL], rf[KL];
-        const float *res[KL];
+        __amdgpu_buffer_rsrc_t ro[KL];
+        const float *rf[KL];
 #pragma unroll
         for (int j = 0; j < KL; ++j) {
             // integer arithmetic: the pointers may be NULL (not stored / not imaged)
             ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
                               plane_bytes);
             // a level without imaging field reads (and ignores) the state instead
-            rf[j] = make_rsrc((p.lv_in[j] != nullptr ? p.lv_in[j] : p.f_cur) + soff, plane_bytes);
-            res[j] = p.lv_resid[j] + (size_t)s * p.n_rec;
+            rf[j] = (p.lv_in[j] != nullptr ? p.lv_in[j] : p.f_cur) + soff;
         }
-        // residual injected at a cell: sum over its scatter entries, in order
-        auto injected = [&](const float *rs, int e) -> float {
-            float a = 0.0f;
-            for (int k = p.entry_start[e]; k < p.entry_start[e + 1]; ++k)
-                a += rs[p.pair_rec[k]] * p.pair_w[k];
-            return a;
-        };
+        // residual rows of the levels: formed only when a lane injects (rare),
+        // to keep scalar registers free
+        const size_t res_off = (size_t)s * p.n_rec;
         // lam of each level's newest three rows (level 0 = the input state,
         // newest row y + 1 at iteration y); P = c^2 lam is formed where it is
         // needed (the same product each time, so the same bits), which keeps
@@ -1740,16 +1776,16 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
             qb[i] = bload(rb, xo, row_off(y + 1));
             qa[i] = bload(ra, xo, row_off(y));
             qk[i] = bload(rk, xo, row_off(y + 1));
-            qe[i] = bload(re, xo, row_off(y));
+            qe[i] = gload_off(emap, xo + row_off(y));
             qm[i] = 0.0f;
-            if constexpr (XE) qm[i] = bload(rm, mo, mrow(y));
+            if constexpr (XE) qm[i] = gload_off(mur, mo + mrow(y));
 #pragma unroll
             for (int j = 0; j < KL; ++j) {
                 // level j + 1: newest field row (y - j), or the Laplacian of the
                 // row it images (y - j - 1)
                 qf[i][j] = 0.0f;
-                if constexpr (IMG == 1) qf[i][j] = bload(rf[j], xo, row_off(y - j));
-                if constexpr (IMG == 2) qf[i][j] = bload(rf[j], xo, row_off(y - j - 1));
+                if constexpr (IMG == 1) qf[i][j] = gload_off(rf[j], xo + row_off(y - j));
+                if constexpr (IMG == 2) qf[i][j] = gload_off(rf[j], xo + row_off(y - j - 1));
             }
         };
 #pragma unroll
@@ -1818,7 +1854,7 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, p.idx2, p.idy2, p.cen),
                                     2.0f * u - old);
                 const int e = eh[j - 1];
-                if (e >= 0) v += injected(res[j - 1], e);
+                v = inject_lanes(p, p.lv_resid[j - 1], res_off, e, lane, v);
                 if constexpr (XE) {
                     if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
                 }
@@ -1930,6 +1966,31 @@ __global__ void __launch_bounds__(256) reduce_groups_kernel(size_t plane, int n_
 __global__ void __launch_bounds__(kGatherThreads) gather_kernel(StepParams p)
 {
     gather_pairs(p, blockIdx.x * (int)blockDim.x + (int)threadIdx.x);
+}
+
+// Bilinear gather of every level of a multi-step launch (p.levels fields
+// p.lv_out[j] into p.lv_samples[j]), same arithmetic as gather_pairs.
+__global__ void __launch_bounds__(kGatherThreads) gather_levels_kernel(StepParams p)
+{
+    const int per = p.n_shots * p.n_rec;
+    int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = t / per;
+    if (j >= p.levels) return;
+    t -= j * per;
+    const float *field = p.lv_out[j];
+    float *dest = p.lv_samples[j];
+    const int R = p.n_rec;
+    const int s = t / R;
+    const int r = t - s * R;
+    const int W = p.nx;
+    const float *f = field + (size_t)s * ((size_t)p.ny * W);
+    const size_t i = (size_t)p.rec_row0[r] * W + p.rec_col0[r];
+    const float *w = p.rec_w;
+    float acc = f[i] * w[r];
+    acc = acc + f[i + 1] * w[R + r];
+    acc = acc + f[i + W] * w[2 * R + r];
+    acc = acc + f[i + W + 1] * w[3 * R + r];
+    dest[t] = acc;
 }
 
 // Stand-alone imaging condition: g_out = g_in + img_scale * sum_s f_cur[s] *
@@ -2817,6 +2878,10 @@ struct ForwardRun {
     int max_levels = 1;  // steps per launch
     bool wide = false;   // full-width launches (else 64-column blocks, two steps)
     bool lap = false;    // history slots receive Laplacians of the fields
+    // samples of every level gathered by a kernel after each launch (full-width
+    // runs with samples and the two extra work fields; else the gather of a
+    // launch's outputs is fused into the next launch)
+    bool sep_gather = false;
     const float *p_old = nullptr, *p_cur = nullptr;
     int k = 0;
     // fields whose receiver samples are still to be taken (the last one or
@@ -2844,8 +2909,12 @@ struct ForwardRun {
         const ForwardPlan plan = forward_plan(g->ny, g->nx, a->work3);
         max_levels = plan.levels;
         wide = plan.wide && !batched;
-        // receiver samples are gathered from stored fields: two levels at most
-        if (batched || !wide || a->samples != nullptr) max_levels = std::min(max_levels, 2);
+        // receiver samples are gathered from stored fields: more than two levels
+        // per launch need two more fields to keep the intermediate levels
+        sep_gather = !batched && wide && a->samples != nullptr && max_levels > 2 &&
+                     a->work4 != nullptr && a->work5 != nullptr;
+        if (batched || !wide || (a->samples != nullptr && !sep_gather))
+            max_levels = std::min(max_levels, 2);
         lap = a->hist_lap != 0;
         if (lap && (!wide || max_levels < 2))
             return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step launches "
@@ -2895,7 +2964,7 @@ struct ForwardRun {
         p.src_values = a->src_values;
         p.src_index = n;
         int gathers = 0;
-        if (a->samples != nullptr && n_pend > 0) {
+        if (a->samples != nullptr && n_pend > 0 && !sep_gather) {
             p.gat_field = pend_field[0];
             p.gat_out = a->samples + (size_t)pend_step[0] * sr;
             if (n_pend == 2) {
@@ -2932,8 +3001,10 @@ struct ForwardRun {
                 out = free_work(work, p_old, p_cur, taken);
                 taken = out;
             }
+            if (out == nullptr && sep_gather) out = j == 0 ? a->work4 : a->work5;
             p.lv_out[j] = out;
             p.lv_out2[j] = ckpt_out(n + j);
+            if (sep_gather) p.lv_samples[j] = a->samples + (size_t)(n + j) * sr;
         }
         p.f_out = p.lv_out[0];
         p.f_out2 = p.lv_out2[0];
@@ -2943,10 +3014,27 @@ struct ForwardRun {
         }
         return gathers;
     }
+    // the gather of a launch's levels when they are gathered after it
+    int launch_gather(const StepParams &p, hipStream_t st) const
+    {
+        if (!sep_gather) return SWFD_OK;
+        const int pairs = p.levels * a->n_shots * rec->n;
+        hipLaunchKernelGGL(gather_levels_kernel, dim3((pairs + kGatherThreads - 1) / kGatherThreads),
+                           dim3(kGatherThreads), 0, st, p);
+        SWFD_HIP_CHECK(hipGetLastError());
+        return SWFD_OK;
+    }
     void advance(const StepParams &p)
     {
         const int n = a->step_begin + k;
         const int L = p.levels;
+        if (sep_gather) {
+            p_old = L >= 2 ? p.lv_out[L - 2] : p_cur;
+            p_cur = p.lv_out[L - 1];
+            n_pend = 0;
+            k += L;
+            return;
+        }
         if (L >= 2) {
             pend_field[0] = p.lv_out[L - 2];
             pend_step[0] = n + L - 2;
@@ -3525,7 +3613,7 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         if (run.next_is_wide()) rc = launch_multi(p, st, gathers);
         else if (run.next_is_pair()) rc = launch_pair(p, st, gathers);
         else rc = launch_step(p, false, st, gathers);
-        if (rc) return rc;
+        if (rc || (rc = run.launch_gather(p, st))) return rc;
         run.advance(p);
     }
     return run.finish(stream);

@@ -134,8 +134,10 @@ class CheckpointedGradient:
     def forward(self, rows_i32, cols_i32, values):
         """All forward steps; samples land in ``self.samples`` (N, S, R)."""
         self.rows, self.cols, self.values = rows_i32, cols_i32, values
+        # the adjoint's fields are idle here: they keep the intermediate steps of
+        # each launch for the gather
         forward_run(self.solver, self.n_shots, rows_i32, cols_i32, values, self.n_steps,
-                    samples=self.samples, sampler=self.sampler, work=self.work,
+                    samples=self.samples, sampler=self.sampler, work=self.work + self.lam[:2],
                     ckpt=self.ckpt, ckpt_every=self.K if self.ckpt is not None else 0,
                     final=False)
         return self.samples

@@ -463,8 +463,9 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
 
     ``init`` is the state before ``step_begin`` (zeros if None); ``samples``
     (N, S, R), ``hist`` (slots, S, ny, nx) and ``ckpt`` (slots, S, ny, nx) are
-    optional outputs (see ``swfd_forward_args``). ``work`` holds three or four
-    scratch fields; with four the run advances several steps per launch.
+    optional outputs (see ``swfd_forward_args``). ``work`` holds three, four or
+    six scratch fields; with four the run advances several steps per launch
+    (with samples: two, four with six fields).
     ``hist_lap``: the history receives the Laplacians of the fields (of
     ``init`` and every output but the last) for ``adjoint_run(hist_lap=True)``.
     """
@@ -490,7 +491,9 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
         hist=_native.ptr(hist), hist_first=hist_first, hist_every=hist_every,
         hist_slots=0 if hist is None else hist.shape[0],
         ckpt=_native.ptr(ckpt), ckpt_every=ckpt_every,
-        ckpt_slots=0 if ckpt is None else ckpt.shape[0], hist_lap=1 if hist_lap else 0)
+        ckpt_slots=0 if ckpt is None else ckpt.shape[0], hist_lap=1 if hist_lap else 0,
+        work4=_native.ptr(work[4]) if len(work) > 5 else None,
+        work5=_native.ptr(work[5]) if len(work) > 5 else None)
     rec = None
     if samples is not None:
         if sampler is None:

@@ -144,6 +144,9 @@ def test_two_step_forward_is_bitwise_single_step(ny, nx, n_steps, form):
     assert bool(torch.isfinite(two["cur"]).all())
     assert float(two["cur"].abs().max()) > 0.0
     _assert_identical(one, two, f"{ny}x{nx}")
+    if sampler is not None:  # samples with all levels per launch (two more fields)
+        six = _run(solver, sampler, rows_t, cols_t, values, n_steps, 6)
+        _assert_identical(one, six, f"{ny}x{nx} with six work fields")
     if sampler is not None:  # without samples: full-width launches take all their levels
         one = _run(solver, None, rows_t, cols_t, values, n_steps, 3)
         two = _run(solver, None, rows_t, cols_t, values, n_steps, 4)
@@ -170,6 +173,8 @@ def test_two_step_history_checkpoints_and_offset_start(form):
     two = _run(solver, sampler, rows_t, cols_t, values, n_steps, 4, **kw)
     assert float(two["hist"].abs().sum()) > 0.0 and float(two["ckpt"].abs().sum()) > 0.0
     _assert_identical(one, two, "history/checkpoints")
+    six = _run(solver, sampler, rows_t, cols_t, values, n_steps, 6, **kw)
+    _assert_identical(one, six, "history/checkpoints, six work fields")
     # without samples (all levels per launch), every step kept in the history
     kw.update(hist_slots=n_steps, hist_first=begin, hist_every=1)
     one = _run(solver, None, rows_t, cols_t, values, n_steps, 3, **kw)
