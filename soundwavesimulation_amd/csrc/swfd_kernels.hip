@@ -1673,10 +1673,11 @@ __global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table
 // its step, lv_in[j - 1] (zero outside the grid), one row late so that row 0
 // is final: g += scale sum_s lam_j Lap(F_j). A workgroup takes one row
 // segment of one shot group and marches the group's shots one after the
-// other, adding each cell's terms into an LDS image of its segment (each lane
-// owns its column, so the order is fixed); at the end it adds the segment
-// into the group's partial image (at most kMaxImageGroups), which are summed
-// in group order after the run: the gradient is deterministic.
+// other; a cell's terms of the KL levels are summed in registers and then
+// added into an LDS image of the segment (each lane owns its column, so the
+// order is fixed); at the end the workgroup adds the segment into the group's
+// partial image (at most kMaxImageGroups), which are summed in group order
+// after the run: the gradient is deterministic.
 
 constexpr int kPfA = 2;  // rows loaded ahead in the adjoint march
 
@@ -1757,6 +1758,11 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
         // rows y - j + 1, y - j, y - j - 1
         float kh[KL + 2], mh[KL], fh[KL + 1][3];
         int eh[KL];
+        // imaging terms of rows y - 1, ..., y - KL summed over the levels so
+        // far (each cell: ((t_1 + t_2) + ...) + t_KL, then into the LDS image)
+        float racc[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) racc[j] = 0.0f;
 #pragma unroll
         for (int j = 0; j <= KL; ++j)
             lv[j][0] = lv[j][1] = lv[j][2] = fh[j][0] = fh[j][1] = fh[j][2] = 0.0f;
@@ -1870,8 +1876,8 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 if ((j >= KL - 1) & (rho >= r0) & (rho < r1) & (rho > 0))
                     bstore(ro[j - 1], v, so, rho * row_bytes);
                 if constexpr (IMG != 0) {
-                    const int R = rho - 1;
-                    if ((p.lv_in[j - 1] != nullptr) & (R >= r0) & (R < r1)) {
+                    // row rho - 1 = y - j: its term goes to racc[j - 1]
+                    if (p.lv_in[j - 1] != nullptr) {
                         float t;
                         if constexpr (IMG == 1) {
                             const float F0 = fh[j][1];
@@ -1880,11 +1886,19 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                             t = lv[j][1] * lap5(fw, fe, fh[j][2], fh[j][0], F0, p.idx2, p.idy2,
                                                 p.cen);
                         } else {
-                            t = lv[j][1] * fh[j][0];  // the Laplacian of row R
+                            t = lv[j][1] * fh[j][0];  // the Laplacian of row rho - 1
                         }
-                        if (owned) acc[(R - r0) * W + x] += t;
+                        racc[j - 1] += t;
                     }
                 }
+            }
+            if constexpr (IMG != 0) {
+                // row y - KL has the terms of every level: add it to the image
+                const int Rf = y - KL;
+                if ((Rf >= r0) & (Rf < r1) & owned) acc[(Rf - r0) * W + x] += racc[KL - 1];
+#pragma unroll
+                for (int k = KL - 1; k >= 1; --k) racc[k] = racc[k - 1];
+                racc[0] = 0.0f;
             }
             if (edge_lane) {
                 AdjEdge e;
@@ -2643,20 +2657,28 @@ int launch_adj_pair(StepParams &p, hipStream_t stream)
 constexpr size_t kMaxAdjwLds = 128 * 1024;
 
 // Shape of a full-width multi-step adjoint launch: row segments of
-// SWFD_ADJW_SEG_ROWS rows (default 32, fewer when the segment's LDS image
-// would not fit) and at most kMaxImageGroups shot groups, one partial image
-// each.
+// SWFD_ADJW_SEG_ROWS rows (default 32, halved while segments x shot groups
+// leave CUs idle, down to 8; fewer when the segment's LDS image would not
+// fit) and at most kMaxImageGroups shot groups, one partial image each. One
+// such workgroup fits a CU, so the launch runs best with a workgroup count
+// that is a whole multiple of the CU count.
 struct AdjwShape {
     int n_seg, seg, n_groups, shots_per_group, acc_rows;
 };
 
 AdjwShape adjw_shape(int ny, int nx, int n_shots)
 {
-    static const int seg_env = std::max(2, env_int("SWFD_ADJW_SEG_ROWS", 32));
+    static const int seg_env = env_int("SWFD_ADJW_SEG_ROWS", 0);
     static const int groups_env = env_int("SWFD_ADJW_GROUPS", kMaxImageGroups);
     const TuneEntry *t = find_tune(1, ny, nx);
-    const int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
     const int groups_param = t != nullptr && t->param > 0 ? t->param : groups_env;
+    const int groups0 = std::max(1, std::min(std::min(groups_param, kMaxImageGroups), n_shots));
+    int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : std::max(0, seg_env);
+    if (seg_rows < 2) {
+        seg_rows = 32;
+        while (seg_rows > 8 && (long)std::max(1, ny / seg_rows) * groups0 < cu_count())
+            seg_rows /= 2;
+    }
     AdjwShape a{};
     a.seg = seg_rows;
     a.n_seg = std::max(1, ny / seg_rows);
