@@ -472,8 +472,9 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
     model = solver.model
     dev = model.device
     shape = (n_shots, *model.shape)
-    if work is None:
-        work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
+    if work is None:  # six fields with samples: four steps per launch (see swfd_forward_args)
+        n_work = 6 if samples is not None else 4
+        work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(n_work)]
     if init is None:
         work[0].zero_()
         work[1].zero_()
