@@ -216,7 +216,8 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
 int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
                  const swfd_adjoint_args *args, void *stream);
 
-/* Workspace floats needed by swfd_adjoint with imaging. */
+/* Workspace floats needed by swfd_adjoint with imaging: one ny x nx partial
+ * image per shot group (up to 32 groups, at most n_shots, at least 8). */
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots);
 
 /* Workspace floats needed by swfd_residual. */

@@ -10,11 +10,10 @@ fastest through ``swfd_tune`` (``include/swfd.h``):
   (rows per march);
 * adjoint: two-step blocks (rows per march <= 32 x shot groups), or
   full-width workgroups with 2-4 steps per launch (rows per march; the
-  segment's image lives in LDS).
+  segment's image lives in LDS; shot groups chosen to fill the CUs).
 
-``batched=True`` keeps to the forms the batched lockstep launches use
-(single steps or two-step blocks). Results are cached per device name, grid,
-shot count and mode in a JSON file (``SWFD_TUNE_CACHE``, default
+The batched lockstep launches use the same forms. Results are cached per
+device name, grid and shot count in a JSON file (``SWFD_TUNE_CACHE``, default
 ``~/.cache/swfd_tune.json``), so a later process applies them without timing
 again.
 """
@@ -38,8 +37,8 @@ FORWARD_WAVES_PER_CU = (128, 256)
 ADJOINT_SEG_ROWS = (8, 16, 32)
 ADJOINT_GROUPS = (4, 8)
 WIDE_LEVELS = (2, 3, 4)
-WIDE_FORWARD_SEG_ROWS = (32, 64, 128)
-WIDE_ADJOINT_SEG_ROWS = (16, 32)
+WIDE_FORWARD_SEG_ROWS = (16, 32, 64, 128)
+WIDE_ADJOINT_SEG_ROWS = (8, 16, 32)
 
 
 def _cache_path():
@@ -47,9 +46,8 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "swfd_tune.json"))
 
 
-def _key(device, ny, nx, n_shots, batched):
-    mode = "batched" if batched else "single"
-    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}:{mode}"
+def _key(device, ny, nx, n_shots):
+    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}:v2"
 
 
 def apply(cfg, ny, nx):
@@ -65,7 +63,7 @@ def apply(cfg, ny, nx):
     _native.check(lib.swfd_tune(3, ny, nx, 0, a["wide"], a["levels"]))
 
 
-def _time(fn, reps=2):
+def _time(fn, reps=2, label=None, log=None):
     fn()
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -74,14 +72,18 @@ def _time(fn, reps=2):
         fn()
     b.record()
     torch.cuda.synchronize()
-    return a.elapsed_time(b) / 1e3 / reps
+    t = a.elapsed_time(b) / 1e3 / reps
+    if log is not None and label is not None:
+        log(f"    {label}: {t * 1e3:.3f} ms")
+    return t
 
 
 def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, log=None,
-         batched=False):
-    """Time candidates for (ny, nx, n_shots); install and return the fastest."""
+         verbose=None):
+    """Time candidates for (ny, nx, n_shots); install and return the fastest
+    (``verbose``: a log function for every candidate's time)."""
     path = _cache_path()
-    key = _key(device, ny, nx, n_shots, batched)
+    key = _key(device, ny, nx, n_shots)
     cache = {}
     if use_cache and os.path.exists(path):
         with open(path) as f:
@@ -117,9 +119,10 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
                     hist=hist, hist_first=-2, hist_every=1, image_scale=1e-15, grad=grad,
                     workspace=ws)
 
-    # entries of this grid only (other grids keep theirs); kind 3 depth 0 =
-    # no adjoint entry
-    lib.swfd_tune(3, ny, nx, 0, 0, 0)
+    # entries of this grid only (other grids keep theirs); depth 0 = no entry
+    for kind in (0, 1, 2, 3):
+        lib.swfd_tune(kind, ny, nx, 0, 0, 0)
+    wide = bool(lib.swfd_lap_history(ny, nx))  # full-width launches fit this grid
     lib.swfd_tune(2, ny, nx, 0, 0, 1)
     t1 = _time(fwd) + _time(adj)
     # forward candidates
@@ -130,10 +133,9 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
             continue
         for wpc in FORWARD_WAVES_PER_CU:
             lib.swfd_tune(0, ny, nx, seg, wpc, 0)
-            t = _time(fwd)
+            t = _time(fwd, label=f"forward blocks seg={seg} wpc={wpc}", log=verbose)
             if fwd_best is None or t < fwd_best[0]:
                 fwd_best = (t, {"wide": 0, "levels": 2, "seg": seg, "wpc": wpc})
-    wide = not batched and bool(lib.swfd_lap_history(ny, nx))  # full-width launches fit
     if wide:
         for levels in WIDE_LEVELS:
             lib.swfd_tune(2, ny, nx, 0, 1, levels)
@@ -141,7 +143,7 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
                 if seg > ny:
                     continue
                 lib.swfd_tune(0, ny, nx, seg, 0, 0)
-                t = _time(fwd)
+                t = _time(fwd, label=f"forward wide L={levels} seg={seg}", log=verbose)
                 if t < fwd_best[0]:
                     fwd_best = (t, {"wide": 1, "levels": levels, "seg": seg, "wpc": 0})
     # adjoint candidates
@@ -153,7 +155,7 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
             continue
         for groups in ADJOINT_GROUPS:
             lib.swfd_tune(1, ny, nx, seg, groups, 0)
-            t = _time(adj)
+            t = _time(adj, label=f"adjoint blocks seg={seg} groups={groups}", log=verbose)
             if adj_best is None or t < adj_best[0]:
                 adj_best = (t, {"wide": 0, "levels": 2, "seg": seg, "groups": groups})
     if wide:
@@ -162,10 +164,13 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
             for seg in WIDE_ADJOINT_SEG_ROWS:
                 if seg > ny:
                     continue
-                lib.swfd_tune(1, ny, nx, seg, 8, 0)
-                t = _time(adj)
+                lib.swfd_tune(1, ny, nx, seg, 0, 0)  # shot groups: automatic
+                t = _time(adj, label=f"adjoint wide L={levels} seg={seg}", log=verbose)
                 if t < adj_best[0]:
-                    adj_best = (t, {"wide": 1, "levels": levels, "seg": seg, "groups": 8})
+                    adj_best = (t, {"wide": 1, "levels": levels, "seg": seg, "groups": 0})
+    if log is not None:
+        log(f"  {ny}x{nx}: single {t1 / n_steps * 1e3:.4f}, forward {fwd_best}, adjoint {adj_best}"
+            " (ms per step for the whole run)")
     t2 = fwd_best[0] + adj_best[0]
     cfg = {"depth1": t1 < t2, "forward": fwd_best[1], "adjoint": adj_best[1],
            "ms_per_step_single": round(t1 / n_steps * 1e3, 4),

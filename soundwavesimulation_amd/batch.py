@@ -2,9 +2,11 @@
 
 The reference batches the shots of one problem (``torch_backend.py:601-770``);
 here several problems -- each with its own grid, medium, ring, shot count and
-step count -- advance together, and at every tick the two-step launches of
-all of them go out as one batched kernel launch (``swfd_forward_batch`` /
-``swfd_adjoint_batch`` in ``include/swfd.h``).
+step count -- advance together, and at every tick the multi-step launches of
+all of them go out as one batched kernel launch per kernel variant and row
+width (``swfd_forward_batch`` / ``swfd_adjoint_batch`` in ``include/swfd.h``):
+full-width launches of up to four steps where a row fits, two-step blocks
+otherwise.
 
 * ``BatchedForward`` / ``BatchedAdjoint``: one prepared lockstep run over a
   list of per-problem run descriptions (the keyword arguments of

@@ -624,6 +624,15 @@ __device__ __forceinline__ void seg_rows(const StepParams &p, int seg, int &r0, 
 #endif
 constexpr int kPf = SWFD_PREFETCH_ROWS;  // rows loaded ahead of the march
 
+// Problem of workgroup b in a batched launch: entry e owns workgroups
+// [first[e], first[e + 1]) (a wavefront-uniform scan of a short table).
+__device__ __forceinline__ int batch_entry(const int *first, int n, int b)
+{
+    int e = 0;
+    while (e + 1 < n && first[e + 1] <= b) ++e;
+    return e;
+}
+
 // Residual injection for the lanes of a wavefront whose cell has scatter
 // entry e >= 0 (others pass e = -1): lane by lane over the flagged lanes, with
 // the entry's (receiver, weight) pairs and the residuals read through scalar
@@ -1167,14 +1176,10 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
     }
 }
 
-// At most 64 VGPRs (eight wavefronts per SIMD): two full-width workgroups
-// per CU, so one marches while the other waits at its barrier.
+// Workgroup b of a full-width multi-step forward launch of one problem.
 template <int KL, int SM, unsigned F>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
-fwdm_kernel(StepParams p)
+__device__ __forceinline__ void fwdm_block(const StepParams &p, int b, EdgeRow *ex)
 {
-    __shared__ EdgeRow ex[2];
-    const int b = (int)blockIdx.x;
     if constexpr ((F & kGather) != 0) {
         if (b >= p.n_stencil_blocks) {
             gather_pairs(p, (b - p.n_stencil_blocks) * (int)blockDim.x + (int)threadIdx.x);
@@ -1206,6 +1211,28 @@ fwdm_kernel(StepParams p)
         if (ye) fwdm_wave<KL, SM, false, true, F>(p, lane, wave, grp, seg, ex);
         else fwdm_wave<KL, SM, false, false, F>(p, lane, wave, grp, seg, ex);
     }
+}
+
+// At most 64 VGPRs (eight wavefronts per SIMD): two full-width workgroups
+// per CU, so one marches while the other waits at its barrier.
+template <int KL, int SM, unsigned F>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+fwdm_kernel(StepParams p)
+{
+    __shared__ EdgeRow ex[2];
+    fwdm_block<KL, SM, F>(p, (int)blockIdx.x, ex);
+}
+
+// Batched full-width launch: entry e of `table` is a problem's launch
+// (all entries have the same row width, so the same workgroup size).
+template <int KL, int SM, unsigned F>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+fwdm_batch_kernel(const StepParams *table, const int *first, int n)
+{
+    __shared__ EdgeRow ex[2];
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    fwdm_block<KL, SM, F>(table[e], b - first[e], ex);
 }
 
 // Workgroup b of a two-step forward launch of one problem.
@@ -1244,15 +1271,6 @@ template <unsigned F>
 __global__ void __launch_bounds__(256) fwd2_kernel(StepParams p)
 {
     fwd2_block<F>(p, (int)blockIdx.x);
-}
-
-// Problem of workgroup b in a batched launch: entry e owns workgroups
-// [first[e], first[e + 1]) (a wavefront-uniform scan of a short table).
-__device__ __forceinline__ int batch_entry(const int *first, int n, int b)
-{
-    int e = 0;
-    while (e + 1 < n && first[e + 1] <= b) ++e;
-    return e;
 }
 
 // One launch advancing several problems by two forward steps each: entry e
@@ -1676,7 +1694,7 @@ __global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table
 // other; a cell's terms of the KL levels are summed in registers and then
 // added into an LDS image of the segment (each lane owns its column, so the
 // order is fixed); at the end the workgroup adds the segment into the group's
-// partial image (at most kMaxImageGroups), which are summed in group order
+// partial image (at most kMaxWideGroups), which are summed in group order
 // after the run: the gradient is deterministic.
 
 constexpr int kPfA = 2;  // rows loaded ahead in the adjoint march
@@ -1917,12 +1935,10 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     }
 }
 
-// KL adjoint steps in one launch; workgroup = (row segment, shot group).
+// Workgroup b (row segment, shot group) of a launch of KL adjoint steps.
 template <int KL, int IMG>
-__global__ void __launch_bounds__(1024) adjm_kernel(StepParams p)
+__device__ __forceinline__ void adjm_block(const StepParams &p, int b, AdjEdgeRow *ex)
 {
-    __shared__ AdjEdgeRow ex[2];
-    const int b = (int)blockIdx.x;
     const int wblk = (b % kXcds) * p.chunk + b / kXcds;
     if (wblk >= p.n_work) return;  // uniform over the workgroup
     const int t = (int)threadIdx.x;
@@ -1964,6 +1980,25 @@ __global__ void __launch_bounds__(1024) adjm_kernel(StepParams p)
             }
         }
     }
+}
+
+template <int KL, int IMG>
+__global__ void __launch_bounds__(1024) adjm_kernel(StepParams p)
+{
+    __shared__ AdjEdgeRow ex[2];
+    adjm_block<KL, IMG>(p, (int)blockIdx.x, ex);
+}
+
+// Batched full-width adjoint launch (entries of one row width; the LDS is
+// sized for the largest segment image of the entries).
+template <int KL, int IMG>
+__global__ void __launch_bounds__(1024) adjm_batch_kernel(const StepParams *table,
+                                                          const int *first, int n)
+{
+    __shared__ AdjEdgeRow ex[2];
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    adjm_block<KL, IMG>(table[e], b - first[e], ex);
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
@@ -2462,14 +2497,22 @@ int wide_geometry(StepParams &p, int gather_pairs_count, int &threads)
     static const int seg_env = std::max(2, env_int("SWFD_WIDE_SEG_ROWS", 64));
     static const int spg_env = std::max(1, env_int("SWFD_WIDE_SHOTS_PER_GROUP", 1));
     const TuneEntry *t = find_tune(0, p.ny, p.nx);
-    const int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    const bool fixed = (t != nullptr && t->seg_rows >= 2) || env_int("SWFD_WIDE_SEG_ROWS", 0) >= 2;
+    int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
     p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
     p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
     threads = p.wide_waves * kLanes;
-    p.seg = seg_rows;
-    p.n_seg = std::max(1, p.ny / seg_rows);
     p.shots_per_group = std::min(spg_env, p.n_shots);
     p.n_groups = (p.n_shots + p.shots_per_group - 1) / p.shots_per_group;
+    // narrow rows: shorter segments until the workgroups fill half the CUs'
+    // capacity (64 VGPRs: 32 wavefronts per CU)
+    if (!fixed) {
+        const long target = (long)cu_count() * std::max(1, 32 / p.wide_waves) / 2;
+        while (seg_rows > 16 && (long)std::max(1, p.ny / seg_rows) * p.n_groups < target)
+            seg_rows /= 2;
+    }
+    p.seg = seg_rows;
+    p.n_seg = std::max(1, p.ny / seg_rows);
     p.n_work = p.n_seg * p.n_groups;
     p.chunk = (p.n_work + kXcds - 1) / kXcds;
     p.n_stencil_blocks = p.chunk * kXcds;
@@ -2480,12 +2523,12 @@ int wide_geometry(StepParams &p, int gather_pairs_count, int &threads)
 
 // Kernel variants: store pattern x (point source, checkpoint copies, fused
 // gather). Copies take the per-level checks; gathers come with two levels
-// only (receiver samples are read from the stored fields).
-template <int KL>
-int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
-                       hipStream_t stream)
+// only (receiver samples are read from the stored fields). `fn` receives the
+// single-problem and the batched kernel of the variant.
+template <int KL, typename Fn>
+int fwdm_variant(int sm, unsigned f, Fn &&fn)
 {
-#define SWFD_FWDM(SMV, FV) hipLaunchKernelGGL((fwdm_kernel<KL, SMV, FV>), grid, block, 0, stream, p)
+#define SWFD_FWDM(SMV, FV) fn(fwdm_kernel<KL, SMV, FV>, fwdm_batch_kernel<KL, SMV, FV>)
     if ((f & kLap) != 0) {
         // Laplacian histories: the fields of the last two levels are stored
         if (f != (kPoint | kLap) || sm != kStoreLastTwo)
@@ -2527,21 +2570,32 @@ int launch_fwdm_levels(StepParams &p, int sm, unsigned f, dim3 grid, dim3 block,
     return SWFD_OK;
 }
 
-// p.levels (2..4) forward steps in one full-width launch; level j's outputs
-// p.lv_out[j] / p.lv_out2[j] (either may be NULL; the last two levels must be
-// stored somewhere by the caller's plan).
-int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
+template <typename Fn>
+int fwdm_levels_variant(int levels, int sm, unsigned f, Fn &&fn)
+{
+    switch (levels) {
+        case 1: return fwdm_variant<1>(sm, f, fn);
+        case 2: return fwdm_variant<2>(sm, f, fn);
+        case 3: return fwdm_variant<3>(sm, f, fn);
+        default: return fwdm_variant<4>(sm, f, fn);
+    }
+}
+
+// Shape of a full-width multi-step forward launch (p.levels levels; level
+// j's outputs p.lv_out[j] / p.lv_out2[j], either may be NULL, the last two
+// levels stored by the caller's plan): fills the geometry into p and returns
+// the workgroup count, the threads per workgroup, the flags and the store
+// pattern.
+int multi_shape(StepParams &p, int gather_pairs_count, int &threads, unsigned &f, int &sm)
 {
     const int min_levels = p.hist_lap ? 1 : 2;
     if (p.src_rows == nullptr || p.levels < min_levels || p.levels > kMaxLevels ||
         !wide_fits(p.nx))
-        return fail(SWFD_EINVAL, "multi-step launch: needs point sources, 2-4 levels, nx <= 1023");
+        return -fail(SWFD_EINVAL, "multi-step launch: needs point sources, 2-4 levels, nx <= 1023");
     if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
-        return fail(SWFD_EINVAL, "multi-step launch: a field plane must be under 2 GiB");
-    int threads = 0;
-    const dim3 grid(wide_geometry(p, gather_pairs_count, threads));
-    const dim3 block(threads);
-    unsigned f = kPoint;
+        return -fail(SWFD_EINVAL, "multi-step launch: a field plane must be under 2 GiB");
+    const int blocks = wide_geometry(p, gather_pairs_count, threads);
+    f = kPoint;
     bool all = true, last_two = true;
     for (int j = 0; j < p.levels; ++j) {
         if (p.lv_out2[j] != nullptr) f |= kOut2;
@@ -2551,15 +2605,22 @@ int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
     }
     if (gather_pairs_count > 0) f |= kGather;
     if (p.hist_lap) f |= kLap;
-    const int sm = (p.hist_lap && last_two) ? kStoreLastTwo
-                   : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny));
-    int rc;
-    switch (p.levels) {
-        case 1: rc = launch_fwdm_levels<1>(p, sm, f, grid, block, stream); break;
-        case 2: rc = launch_fwdm_levels<2>(p, sm, f, grid, block, stream); break;
-        case 3: rc = launch_fwdm_levels<3>(p, sm, f, grid, block, stream); break;
-        default: rc = launch_fwdm_levels<4>(p, sm, f, grid, block, stream); break;
-    }
+    sm = (p.hist_lap && last_two) ? kStoreLastTwo
+         : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny));
+    return blocks;
+}
+
+// p.levels (1..4) forward steps in one full-width launch of one problem.
+int launch_multi(StepParams &p, hipStream_t stream, int gather_pairs_count)
+{
+    int threads = 0, sm = 0;
+    unsigned f = 0;
+    const int blocks = multi_shape(p, gather_pairs_count, threads, f, sm);
+    if (blocks < 0) return -blocks;
+    const dim3 grid(blocks), block(threads);
+    const int rc = fwdm_levels_variant(p.levels, sm, f, [&](auto single, auto) {
+        hipLaunchKernelGGL(single, grid, block, 0, stream, p);
+    });
     if (rc) return rc;
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
@@ -2656,12 +2717,18 @@ int launch_adj_pair(StepParams &p, hipStream_t stream)
 // (one such workgroup per CU).
 constexpr size_t kMaxAdjwLds = 128 * 1024;
 
-// Shape of a full-width multi-step adjoint launch: row segments of
-// SWFD_ADJW_SEG_ROWS rows (default 32, halved while segments x shot groups
-// leave CUs idle, down to 8; fewer when the segment's LDS image would not
-// fit) and at most kMaxImageGroups shot groups, one partial image each. One
-// such workgroup fits a CU, so the launch runs best with a workgroup count
-// that is a whole multiple of the CU count.
+// Shape of a full-width multi-step adjoint launch. The kernel needs about
+// 120 VGPRs (four wavefronts per SIMD), so a CU holds 16 / wavefronts-per-row
+// workgroups, fewer if their LDS images of the segment do not fit. A
+// workgroup marches its segment (plus the 3 rows shared with each
+// neighbour) once per shot of its group, so the launch takes about
+// ceil(workgroups / (CUs x resident)) x (rows + 3) x shots-per-group; the
+// shape minimises that over 8-32 row segments and 8-kMaxWideGroups shot
+// groups (one partial image each), preferring longer segments and fewer
+// groups on ties. SWFD_ADJW_SEG_ROWS / SWFD_ADJW_GROUPS or a kind-1 tuning
+// entry fix either.
+constexpr int kMaxWideGroups = 32;
+
 struct AdjwShape {
     int n_seg, seg, n_groups, shots_per_group, acc_rows;
 };
@@ -2669,47 +2736,82 @@ struct AdjwShape {
 AdjwShape adjw_shape(int ny, int nx, int n_shots)
 {
     static const int seg_env = env_int("SWFD_ADJW_SEG_ROWS", 0);
-    static const int groups_env = env_int("SWFD_ADJW_GROUPS", kMaxImageGroups);
+    static const int groups_env = env_int("SWFD_ADJW_GROUPS", 0);
     const TuneEntry *t = find_tune(1, ny, nx);
-    const int groups_param = t != nullptr && t->param > 0 ? t->param : groups_env;
-    const int groups0 = std::max(1, std::min(std::min(groups_param, kMaxImageGroups), n_shots));
-    int seg_rows = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : std::max(0, seg_env);
-    if (seg_rows < 2) {
-        seg_rows = 32;
-        while (seg_rows > 8 && (long)std::max(1, ny / seg_rows) * groups0 < cu_count())
-            seg_rows /= 2;
+    const int seg_fixed = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
+    const int groups_fixed = t != nullptr && t->param > 0 ? t->param : groups_env;
+    const int max_groups = std::max(1, std::min(kMaxWideGroups, n_shots));
+    const int off = (nx % kLanes == 1) ? 1 : 0;
+    const int waves = (nx + off + kLanes - 1) / kLanes;
+    const int max_rows = std::max(1, (int)(kMaxAdjwLds / (sizeof(float) * (size_t)nx)));
+    auto n_seg_of = [&](int seg) {
+        const int n = std::max(1, ny / seg);
+        return std::max(n, (ny + max_rows - 1) / max_rows);
+    };
+    auto cost = [&](int seg, int groups) {
+        const int n_seg = n_seg_of(seg);
+        const int rows = (ny + n_seg - 1) / n_seg;
+        const size_t lds = (size_t)rows * nx * sizeof(float) + 4096;
+        const long resident = std::max(1, std::min(16 / waves, (int)((160 * 1024) / lds)));
+        const long capacity = (long)cu_count() * resident;
+        const long rounds = ((long)n_seg * groups + capacity - 1) / capacity;
+        const long spg = (n_shots + groups - 1) / groups;
+        return rounds * (rows + 3) * spg;
+    };
+    int best_seg = seg_fixed >= 2 ? seg_fixed : 32;
+    int best_groups = groups_fixed > 0 ? std::min(groups_fixed, max_groups) : std::min(8, max_groups);
+    long best = cost(best_seg, best_groups);
+    static const int kGroupChoices[3] = {8, 16, kMaxWideGroups};
+    for (int seg = 32; seg >= 8; seg /= 2) {
+        if (seg_fixed >= 2 && seg != seg_fixed) continue;
+        for (int gi = 0; gi < 3; ++gi) {
+            const int groups = groups_fixed > 0 ? std::min(groups_fixed, max_groups)
+                                                : std::min(kGroupChoices[gi], max_groups);
+            const long c = cost(seg, groups);
+            if (c < best) {
+                best = c;
+                best_seg = seg;
+                best_groups = groups;
+            }
+        }
     }
     AdjwShape a{};
-    a.seg = seg_rows;
-    a.n_seg = std::max(1, ny / seg_rows);
-    const int max_rows = std::max(1, (int)(kMaxAdjwLds / (sizeof(float) * (size_t)nx)));
-    a.n_seg = std::max(a.n_seg, (ny + max_rows - 1) / max_rows);
+    a.seg = best_seg;
+    a.n_seg = n_seg_of(best_seg);
     a.acc_rows = (ny + a.n_seg - 1) / a.n_seg;
-    const int groups = std::max(1, std::min(std::min(groups_param, kMaxImageGroups), n_shots));
-    a.shots_per_group = (n_shots + groups - 1) / groups;
+    a.shots_per_group = (n_shots + best_groups - 1) / best_groups;
     a.n_groups = (n_shots + a.shots_per_group - 1) / a.shots_per_group;
     return a;
 }
 
-// p.levels (2..4) adjoint steps in one full-width launch: outputs of the
-// last two levels in p.lv_out[levels - 2], p.lv_out[levels - 1]; residuals
-// p.lv_resid[j]; imaging fields p.lv_in[j] (NULL: level not imaged).
-int launch_adj_multi(StepParams &p, hipStream_t stream)
+// Imaging variant of a full-width adjoint launch: 0 none, 1 fields, 2
+// Laplacians (history of Laplacians).
+int adjm_image(const StepParams &p)
+{
+    bool image = false;
+    for (int j = 0; j < p.levels; ++j) image = image || p.lv_in[j] != nullptr;
+    return !image ? 0 : (p.hist_lap ? 2 : 1);
+}
+
+// Shape of a full-width adjoint launch of p.levels (2..4) steps: outputs of
+// the last two levels in p.lv_out[levels - 2], p.lv_out[levels - 1];
+// residuals p.lv_resid[j]; imaging fields p.lv_in[j] (NULL: level not
+// imaged). Fills the geometry into p; returns the workgroup count (negative
+// error code on failure), the threads per workgroup and the LDS bytes.
+int adjm_shape(StepParams &p, int &threads, size_t &lds)
 {
     const int L = p.levels;
     if (L < 2 || L > kMaxLevels || !wide_fits(p.nx) || p.cell_map == nullptr)
-        return fail(SWFD_EINVAL, "multi-step adjoint: 2-4 levels, nx <= 1023, scatter table");
+        return -fail(SWFD_EINVAL, "multi-step adjoint: 2-4 levels, nx <= 1023, scatter table");
     if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
-        return fail(SWFD_EINVAL, "multi-step adjoint: a field plane must be under 2 GiB");
-    bool image = false;
-    for (int j = 0; j < L; ++j) {
-        if (p.lv_resid[j] == nullptr) return fail(SWFD_EINVAL, "multi-step adjoint: residual");
-        image = image || p.lv_in[j] != nullptr;
-    }
+        return -fail(SWFD_EINVAL, "multi-step adjoint: a field plane must be under 2 GiB");
+    for (int j = 0; j < L; ++j)
+        if (p.lv_resid[j] == nullptr) return -fail(SWFD_EINVAL, "multi-step adjoint: residual");
     if (p.lv_out[L - 2] == nullptr || p.lv_out[L - 1] == nullptr)
-        return fail(SWFD_EINVAL, "multi-step adjoint needs the last two outputs");
+        return -fail(SWFD_EINVAL, "multi-step adjoint needs the last two outputs");
+    const bool image = adjm_image(p) != 0;
     if (image && p.g_part == nullptr)
-        return fail(SWFD_EINVAL, "multi-step adjoint imaging needs the partial images");
+        return -fail(SWFD_EINVAL, "multi-step adjoint imaging needs the partial images");
     const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots);
     p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
     p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
@@ -2721,36 +2823,66 @@ int launch_adj_multi(StepParams &p, hipStream_t stream)
     p.n_work = p.n_seg * p.n_groups;
     p.chunk = (p.n_work + kXcds - 1) / kXcds;
     p.n_stencil_blocks = p.chunk * kXcds;
-    const size_t lds = image ? (size_t)a.acc_rows * p.nx * sizeof(float) : 0;
+    lds = image ? (size_t)a.acc_rows * p.nx * sizeof(float) : 0;
     if (lds > kMaxAdjwLds)
-        return fail(SWFD_EINVAL, "multi-step adjoint: row segments too long for the LDS image");
-    const dim3 grid((unsigned)p.n_stencil_blocks);
-    const dim3 block((unsigned)(p.wide_waves * kLanes));
-    // dynamic LDS beyond 64 KiB has to be allowed per kernel (done once)
-#define SWFD_ADJM_IMG(KLV, IV)                                                          \
-    do {                                                                                 \
-        static bool big = false;                                                         \
-        if (!big) {                                                                      \
-            SWFD_HIP_CHECK(hipFuncSetAttribute(                                          \
-                reinterpret_cast<const void *>(&adjm_kernel<KLV, IV>),                    \
-                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxAdjwLds));           \
-            big = true;                                                                  \
-        }                                                                                \
-        hipLaunchKernelGGL((adjm_kernel<KLV, IV>), grid, block, lds, stream, p);         \
-    } while (0)
-#define SWFD_ADJM(KLV)                                                                  \
-    do {                                                                                 \
-        if (!image) hipLaunchKernelGGL((adjm_kernel<KLV, 0>), grid, block, 0, stream, p); \
-        else if (p.hist_lap) SWFD_ADJM_IMG(KLV, 2);                                     \
-        else SWFD_ADJM_IMG(KLV, 1);                                                      \
-    } while (0)
-    switch (L) {
-        case 2: SWFD_ADJM(2); break;
-        case 3: SWFD_ADJM(3); break;
-        default: SWFD_ADJM(4); break;
+        return -fail(SWFD_EINVAL, "multi-step adjoint: row segments too long for the LDS image");
+    threads = p.wide_waves * kLanes;
+    return p.n_stencil_blocks;
+}
+
+// The kernels of a full-width adjoint variant, handed to fn(single,
+// batched), after allowing them dynamic LDS beyond 64 KiB (once each).
+template <int KL, int IMG, typename Fn>
+int adjm_variant_img(Fn &&fn)
+{
+    if constexpr (IMG != 0) {
+        static bool big = false;
+        if (!big) {
+            SWFD_HIP_CHECK(hipFuncSetAttribute(
+                reinterpret_cast<const void *>(&adjm_kernel<KL, IMG>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxAdjwLds));
+            SWFD_HIP_CHECK(hipFuncSetAttribute(
+                reinterpret_cast<const void *>(&adjm_batch_kernel<KL, IMG>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxAdjwLds));
+            big = true;
+        }
     }
-#undef SWFD_ADJM
-#undef SWFD_ADJM_IMG
+    fn(adjm_kernel<KL, IMG>, adjm_batch_kernel<KL, IMG>);
+    return SWFD_OK;
+}
+
+template <int KL, typename Fn>
+int adjm_variant(int img, Fn &&fn)
+{
+    switch (img) {
+        case 0: return adjm_variant_img<KL, 0>(fn);
+        case 1: return adjm_variant_img<KL, 1>(fn);
+        default: return adjm_variant_img<KL, 2>(fn);
+    }
+}
+
+template <typename Fn>
+int adjm_levels_variant(int levels, int img, Fn &&fn)
+{
+    switch (levels) {
+        case 2: return adjm_variant<2>(img, fn);
+        case 3: return adjm_variant<3>(img, fn);
+        default: return adjm_variant<4>(img, fn);
+    }
+}
+
+// p.levels (2..4) adjoint steps in one full-width launch of one problem.
+int launch_adj_multi(StepParams &p, hipStream_t stream)
+{
+    int threads = 0;
+    size_t lds = 0;
+    const int blocks = adjm_shape(p, threads, lds);
+    if (blocks < 0) return -blocks;
+    const dim3 grid((unsigned)blocks), block((unsigned)threads);
+    const int rc = adjm_levels_variant(p.levels, adjm_image(p), [&](auto single, auto) {
+        hipLaunchKernelGGL(single, grid, block, lds, stream, p);
+    });
+    if (rc) return rc;
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
 }
@@ -2912,7 +3044,8 @@ struct ForwardRun {
     int pend_step[2] = {0, 0};
     int n_pend = 0;
 
-    // batched: the lockstep tables hold two-step block launches only
+    // batched: lockstep tables (no Laplacian histories, samples from the
+    // fused gather of two-step launches)
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
              const swfd_forward_args *args, bool batched = false)
     {
@@ -2930,14 +3063,16 @@ struct ForwardRun {
         work[3] = a->work3;
         const ForwardPlan plan = forward_plan(g->ny, g->nx, a->work3);
         max_levels = plan.levels;
-        wide = plan.wide && !batched;
+        wide = plan.wide;
         // receiver samples are gathered from stored fields: more than two levels
-        // per launch need two more fields to keep the intermediate levels
+        // per launch need two more fields to keep the intermediate levels (the
+        // batched launches gather the previous launch's two outputs instead)
         sep_gather = !batched && wide && a->samples != nullptr && max_levels > 2 &&
                      a->work4 != nullptr && a->work5 != nullptr;
-        if (batched || !wide || (a->samples != nullptr && !sep_gather))
-            max_levels = std::min(max_levels, 2);
+        if (!wide || (a->samples != nullptr && !sep_gather)) max_levels = std::min(max_levels, 2);
         lap = a->hist_lap != 0;
+        if (lap && batched)
+            return fail(SWFD_EINVAL, "Laplacian history: not in batched runs");
         if (lap && (!wide || max_levels < 2))
             return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step launches "
                                      "(see swfd_lap_history) and work3");
@@ -3156,7 +3291,7 @@ struct AdjointRun {
     const float *p_old = nullptr, *p_cur = nullptr;
     int n = 0;  // next step to run (descending)
 
-    // batched: the lockstep tables hold two-step block launches only
+    // batched: lockstep tables (no Laplacian histories)
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
              const swfd_adjoint_args *args, bool batched = false)
     {
@@ -3175,7 +3310,9 @@ struct AdjointRun {
         work[3] = a->work3;
         const ForwardPlan plan = adjoint_plan(g->ny, g->nx, a->work3, r);
         max_levels = plan.levels;
-        wide = plan.wide && !batched;
+        wide = plan.wide;
+        if (batched && a->hist_lap != 0)
+            return fail(SWFD_EINVAL, "Laplacian history: not in batched runs");
         if (!wide) max_levels = std::min(max_levels, 2);
         if (max_levels >= 2)
             n_part = wide ? adjw_shape(g->ny, g->nx, a->n_shots).n_groups
@@ -3312,11 +3449,16 @@ constexpr size_t kTableAlign = 256;
 
 size_t align_up(size_t v) { return (v + kTableAlign - 1) / kTableAlign * kTableAlign; }
 
-struct TickTable {
-    std::vector<StepParams> entries;
-    std::vector<int> first;
-    size_t lds = 0;
+// One launch of a tick: entries of one kernel variant (and workgroup size).
+struct LaunchGroup {
+    int kind = 0;  // 0: two-step blocks, 1: full width
     unsigned flags = 0;
+    int levels = 0, sm = 0, threads = 0;
+    size_t lds = 0;
+    bool image = false;  // two-step adjoint blocks: some entry images
+    std::vector<StepParams> entries;
+    std::vector<int> first{0};
+    std::vector<int> runs;
     size_t bytes() const
     {
         return align_up(entries.size() * sizeof(StepParams)) +
@@ -3324,8 +3466,66 @@ struct TickTable {
     }
 };
 
+// The group of a tick for this variant (created in first-seen order, so the
+// same in every mode).
+LaunchGroup &group_for(std::vector<LaunchGroup> &groups, int kind, unsigned flags, int levels,
+                       int sm, int threads)
+{
+    for (LaunchGroup &g : groups)
+        if (g.kind == kind && g.flags == flags && g.levels == levels && g.sm == sm &&
+            g.threads == threads)
+            return g;
+    groups.emplace_back();
+    LaunchGroup &g = groups.back();
+    g.kind = kind;
+    g.flags = flags;
+    g.levels = levels;
+    g.sm = sm;
+    g.threads = threads;
+    return g;
+}
+
+// Mode 1: append the group's table to `host`; mode 2: launch it from the
+// device copy at `table` + off (launch(group, table, first, n)).
+template <typename Launch>
+int emit_group(const LaunchGroup &g, int mode, std::vector<char> &host, size_t &off, void *table,
+               size_t table_bytes, const char *what, Launch &&launch)
+{
+    const size_t nb = g.bytes();
+    const size_t first_off = align_up(g.entries.size() * sizeof(StepParams));
+    if (mode == 1) {
+        host.resize(off + nb);
+        std::memcpy(host.data() + off, g.entries.data(), g.entries.size() * sizeof(StepParams));
+        std::memcpy(host.data() + off + first_off, g.first.data(), g.first.size() * sizeof(int));
+    } else if (mode == 2) {
+        if (off + nb > table_bytes)
+            return fail(SWFD_EINVAL, std::string(what) + ": table buffer too small");
+        const char *base = static_cast<const char *>(table) + off;
+        int rc = launch(g, reinterpret_cast<const StepParams *>(base),
+                        reinterpret_cast<const int *>(base + first_off), (int)g.entries.size());
+        if (rc) return rc;
+        SWFD_HIP_CHECK(hipGetLastError());
+    }
+    off += nb;
+    return SWFD_OK;
+}
+
+int finish_tables(int mode, const std::vector<char> &host, size_t off, void *table,
+                  size_t table_bytes, size_t *bytes_out, const char *what)
+{
+    if (bytes_out != nullptr) *bytes_out = off;
+    if (mode == 1) {
+        if (off > table_bytes) return fail(SWFD_EINVAL, std::string(what) + ": table buffer too small");
+        if (off > 0) SWFD_HIP_CHECK(hipMemcpy(table, host.data(), off, hipMemcpyHostToDevice));
+    }
+    return SWFD_OK;
+}
+
 // mode 0: count table bytes; 1: write the tables to `table` (blocking copy);
-// 2: launch (tables already in `table`). Modes 0 and 1 launch nothing.
+// 2: launch (tables already in `table`). Modes 0 and 1 launch nothing. Each
+// tick advances every unfinished run by one launch: the multi-step launches
+// of the runs go out as one batched launch per kernel variant and row width,
+// single steps one by one.
 int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                   const swfd_receivers *recs, const swfd_forward_args *args, int mode,
                   void *table, size_t table_bytes, size_t *bytes_out, void *stream)
@@ -3343,58 +3543,57 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
     size_t off = 0;
     for (;;) {
         bool any = false;
-        TickTable t;
-        t.first.push_back(0);
-        std::vector<int> pair_runs;
+        std::vector<LaunchGroup> groups;
         for (int i = 0; i < n_runs; ++i) {
             ForwardRun &r = runs[i];
             if (r.done()) continue;
             any = true;
             StepParams p;
             const int gathers = r.build(p);
-            if (r.next_is_pair()) {
+            if (r.next_is_wide()) {
+                int threads = 0, sm = 0;
+                unsigned f = 0;
+                const int blocks = multi_shape(p, gathers, threads, f, sm);
+                if (blocks < 0) return -blocks;
+                LaunchGroup &g = group_for(groups, 1, f, p.levels, sm, threads);
+                g.entries.push_back(p);
+                g.first.push_back(g.first.back() + blocks);
+                g.runs.push_back(i);
+            } else if (r.next_is_pair()) {
                 const int blocks = pair_geometry(p, gathers);
-                t.flags |= pair_flags(p, gathers > 0);
-                t.entries.push_back(p);
-                t.first.push_back(t.first.back() + blocks);
-                pair_runs.push_back(i);
+                LaunchGroup &g = group_for(groups, 0, pair_flags(p, gathers > 0), 2, 0,
+                                           kLanes * kWavesPerBlock);
+                g.entries.push_back(p);
+                g.first.push_back(g.first.back() + blocks);
+                g.runs.push_back(i);
             } else {
                 if (mode == 2 && (rc = launch_step(p, false, st, gathers))) return rc;
                 r.advance(p);
             }
         }
         if (!any) break;
-        if (!t.entries.empty()) {
-            const size_t nb = t.bytes();
-            if (mode == 1) {
-                host.resize(off + nb);
-                std::memcpy(host.data() + off, t.entries.data(), t.entries.size() * sizeof(StepParams));
-                std::memcpy(host.data() + off + align_up(t.entries.size() * sizeof(StepParams)),
-                            t.first.data(), t.first.size() * sizeof(int));
-            } else if (mode == 2) {
-                if (off + nb > table_bytes)
-                    return fail(SWFD_EINVAL, "forward batch: table buffer too small");
-                const char *base = static_cast<const char *>(table) + off;
-                const StepParams *tab = reinterpret_cast<const StepParams *>(base);
-                const int *first = reinterpret_cast<const int *>(
-                    base + align_up(t.entries.size() * sizeof(StepParams)));
-                const int ne = (int)t.entries.size();
-                const dim3 grid((unsigned)t.first.back());
-                const dim3 block(kLanes * kWavesPerBlock);
-                launch_fwd2_variant(t.flags, grid, st, [&](auto, auto batched) {
-                    hipLaunchKernelGGL(batched, grid, block, 0, st, tab, first, ne);
-                });
-                SWFD_HIP_CHECK(hipGetLastError());
-            }
-            off += nb;
-            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j]);
+        for (const LaunchGroup &g : groups) {
+            rc = emit_group(g, mode, host, off, table, table_bytes, "forward batch",
+                            [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
+                                int ne) {
+                                const dim3 grid((unsigned)gg.first.back()), block(gg.threads);
+                                if (gg.kind == 0) {
+                                    launch_fwd2_variant(gg.flags, grid, st, [&](auto, auto batched) {
+                                        hipLaunchKernelGGL(batched, grid, block, 0, st, tab, first, ne);
+                                    });
+                                    return SWFD_OK;
+                                }
+                                return fwdm_levels_variant(gg.levels, gg.sm, gg.flags,
+                                                           [&](auto, auto batched) {
+                                    hipLaunchKernelGGL(batched, grid, block, 0, st, tab, first, ne);
+                                });
+                            });
+            if (rc) return rc;
+            for (size_t j = 0; j < g.runs.size(); ++j) runs[g.runs[j]].advance(g.entries[j]);
         }
     }
-    if (bytes_out != nullptr) *bytes_out = off;
-    if (mode == 1) {
-        if (off > table_bytes) return fail(SWFD_EINVAL, "forward batch: table buffer too small");
-        if (off > 0) SWFD_HIP_CHECK(hipMemcpy(table, host.data(), off, hipMemcpyHostToDevice));
-    }
+    if ((rc = finish_tables(mode, host, off, table, table_bytes, bytes_out, "forward batch")))
+        return rc;
     if (mode == 2)
         for (ForwardRun &r : runs)
             if ((rc = r.finish(stream))) return rc;
@@ -3416,9 +3615,7 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
     size_t off = 0;
     for (;;) {
         bool any = false;
-        TickTable t;
-        t.first.push_back(0);
-        std::vector<int> pair_runs;
+        std::vector<LaunchGroup> groups;
         for (int i = 0; i < n_runs; ++i) {
             AdjointRun &r = runs[i];
             if (r.done()) continue;
@@ -3427,7 +3624,17 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
             bool fill = false;
             if ((rc = r.build(p, fill))) return rc;
             if (mode == 2 && fill && (rc = r.launch_fill(st))) return rc;
-            if (r.next_is_pair()) {
+            if (r.next_is_wide()) {
+                int threads = 0;
+                size_t lds = 0;
+                const int blocks = adjm_shape(p, threads, lds);
+                if (blocks < 0) return -blocks;
+                LaunchGroup &g = group_for(groups, 1, (unsigned)adjm_image(p), p.levels, 0, threads);
+                g.lds = std::max(g.lds, lds);
+                g.entries.push_back(p);
+                g.first.push_back(g.first.back() + blocks);
+                g.runs.push_back(i);
+            } else if (r.next_is_pair()) {
                 size_t lds = 0;
                 const int blocks = adj_pair_geometry(p, lds);
                 // the batched launch runs every entry with the imaging variant
@@ -3435,51 +3642,44 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 const size_t acc = (size_t)kWavesPerBlock * p.acc_rows * kLanes * sizeof(float);
                 if (acc > 64 * 1024)
                     return fail(SWFD_EINVAL, "two-step adjoint: row segments too long");
-                if (lds > 0) t.flags |= kImage;
-                t.lds = std::max(t.lds, acc);
-                t.entries.push_back(p);
-                t.first.push_back(t.first.back() + blocks);
-                pair_runs.push_back(i);
+                LaunchGroup &g = group_for(groups, 0, 0, 2, 0, kLanes * kWavesPerBlock);
+                if (lds > 0) g.image = true;
+                g.lds = std::max(g.lds, acc);
+                g.entries.push_back(p);
+                g.first.push_back(g.first.back() + blocks);
+                g.runs.push_back(i);
             } else {
                 if (mode == 2 && (rc = launch_step(p, true, st, 0))) return rc;
                 r.advance(p);
             }
         }
         if (!any) break;
-        if (!t.entries.empty()) {
-            const size_t nb = t.bytes();
-            if (mode == 1) {
-                host.resize(off + nb);
-                std::memcpy(host.data() + off, t.entries.data(), t.entries.size() * sizeof(StepParams));
-                std::memcpy(host.data() + off + align_up(t.entries.size() * sizeof(StepParams)),
-                            t.first.data(), t.first.size() * sizeof(int));
-            } else if (mode == 2) {
-                if (off + nb > table_bytes)
-                    return fail(SWFD_EINVAL, "adjoint batch: table buffer too small");
-                const char *base = static_cast<const char *>(table) + off;
-                const StepParams *tab = reinterpret_cast<const StepParams *>(base);
-                const int *first = reinterpret_cast<const int *>(
-                    base + align_up(t.entries.size() * sizeof(StepParams)));
-                const int ne = (int)t.entries.size();
-                const dim3 grid((unsigned)t.first.back());
-                const dim3 block(kLanes * kWavesPerBlock);
-                if ((t.flags & kImage) != 0)
-                    hipLaunchKernelGGL((adj2_batch_kernel<kResid | kImage>), grid, block, t.lds, st,
-                                       tab, first, ne);
-                else
-                    hipLaunchKernelGGL((adj2_batch_kernel<kResid>), grid, block, 0, st, tab, first,
-                                       ne);
-                SWFD_HIP_CHECK(hipGetLastError());
-            }
-            off += nb;
-            for (size_t j = 0; j < pair_runs.size(); ++j) runs[pair_runs[j]].advance(t.entries[j]);
+        for (const LaunchGroup &g : groups) {
+            rc = emit_group(g, mode, host, off, table, table_bytes, "adjoint batch",
+                            [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
+                                int ne) {
+                                const dim3 grid((unsigned)gg.first.back()), block(gg.threads);
+                                if (gg.kind == 0) {
+                                    if (gg.image)
+                                        hipLaunchKernelGGL((adj2_batch_kernel<kResid | kImage>), grid,
+                                                           block, gg.lds, st, tab, first, ne);
+                                    else
+                                        hipLaunchKernelGGL((adj2_batch_kernel<kResid>), grid, block,
+                                                           0, st, tab, first, ne);
+                                    return SWFD_OK;
+                                }
+                                return adjm_levels_variant(gg.levels, (int)gg.flags,
+                                                           [&](auto, auto batched) {
+                                    hipLaunchKernelGGL(batched, grid, block, gg.lds, st, tab, first,
+                                                       ne);
+                                });
+                            });
+            if (rc) return rc;
+            for (size_t j = 0; j < g.runs.size(); ++j) runs[g.runs[j]].advance(g.entries[j]);
         }
     }
-    if (bytes_out != nullptr) *bytes_out = off;
-    if (mode == 1) {
-        if (off > table_bytes) return fail(SWFD_EINVAL, "adjoint batch: table buffer too small");
-        if (off > 0) SWFD_HIP_CHECK(hipMemcpy(table, host.data(), off, hipMemcpyHostToDevice));
-    }
+    if ((rc = finish_tables(mode, host, off, table, table_bytes, bytes_out, "adjoint batch")))
+        return rc;
     if (mode == 2)
         for (AdjointRun &r : runs)
             if ((rc = r.finish(st))) return rc;
@@ -3753,7 +3953,8 @@ int32_t swfd_lap_history(int32_t ny, int32_t nx)
 
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots)
 {
-    return (size_t)kMaxImageGroups * (size_t)ny * (size_t)nx;
+    const int groups = std::max(kMaxImageGroups, std::min(kMaxWideGroups, (int)n_shots));
+    return (size_t)groups * (size_t)ny * (size_t)nx;
 }
 
 size_t swfd_residual_workspace_floats(int32_t n_steps, int32_t n_shots, int32_t n_rec)

@@ -110,7 +110,7 @@ def main(argv=None):
     if not a.no_tune:
         for h in sorted({s[0] for s in specs}):
             shots = max(r for hh, r, _ in specs if hh == h)
-            tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log, batched=True)
+            tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log)
     t0 = time.perf_counter()
     problems = [build(s, i, seed, dev) for i, s in enumerate(specs)]
     torch.cuda.synchronize()

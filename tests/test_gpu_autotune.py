@@ -15,10 +15,9 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
 
 
-@pytest.mark.parametrize("batched", [False, True])
-def test_tuned_forward_equals_default(batched):
+def test_tuned_forward_equals_default():
     ny, nx, S, N = 96, 80, 4, 13
-    print(f"autotune {ny}x{nx} x {S} shots, batched={batched}", flush=True)
+    print(f"autotune {ny}x{nx} x {S} shots", flush=True)
     lib = _native.lib()
     lib.swfd_tune_clear()
     sp = 0.5e-3
@@ -37,10 +36,8 @@ def test_tuned_forward_equals_default(batched):
 
     ref = run()
     try:
-        cfg = tune(ny, nx, S, DEV, use_cache=False, batched=batched)
+        cfg = tune(ny, nx, S, DEV, use_cache=False)
         assert set(cfg) >= {"depth1", "forward", "adjoint"}
-        if batched:
-            assert cfg["forward"]["wide"] == 0 and cfg["adjoint"]["wide"] == 0
         got = run()
     finally:
         lib.swfd_tune_clear()

@@ -77,21 +77,53 @@ def test_batched_forward_equals_separate_runs():
         assert torch.equal(w2[0], w[0]) and torch.equal(w2[1], w[1])
 
 
-@pytest.fixture
-def adjoint_blocks():
-    """Separate adjoint runs in the two-step block form, the form the batched
-    launches use (full-width runs sum the gradient in another order)."""
+def test_batched_forward_history_equals_separate_runs():
+    """Without samples the batched runs take four steps per launch (full width,
+    one batched launch per row width) and keep every step in the history."""
+    print("batched forward with history, 3 problems", flush=True)
+    runs, refs = [], []
+    for i, (ny, nx, n_el, N) in enumerate(SPECS):
+        solver, sampler, array, rows, cols, dt = _problem(ny, nx, n_el, 40 + i)
+        values = torch.randn(N + 2, generator=torch.Generator().manual_seed(i)).to(DEV)
+        shape = (n_el, ny, nx)
+        res = {}
+        for mode in ("single", "batch"):
+            work = [torch.zeros(shape, device=DEV) for _ in range(4)]
+            hist = torch.full((N, *shape), float("nan"), device=DEV)
+            res[mode] = (work, hist)
+        w, hist = res["single"]
+        forward_run(solver, n_el, rows, cols, values, N, hist=hist, work=w, final=True)
+        refs.append((w, hist))
+        w2, hist2 = res["batch"]
+        runs.append(((solver, None, dict(n_shots=n_el, rows=rows, cols=cols, values=values,
+                                         n_steps=N, init=(w2[0], w2[1]), work=w2, hist=hist2,
+                                         final=(w2[0], w2[1]))), (w2, hist2)))
+    batched = BatchedForward([r[0] for r in runs]).prepare()
+    batched.launch()
+    torch.cuda.synchronize()
+    for (_, (w2, hist2)), (w, hist) in zip(runs, refs):
+        assert bool(torch.isfinite(hist2).all())
+        assert torch.equal(hist2, hist)
+        assert torch.equal(w2[0], w[0]) and torch.equal(w2[1], w[1])
+
+
+@pytest.fixture(params=["wide", "blocks"])
+def adjoint_form(request):
+    """Adjoint form of both the separate and the batched runs: full width (the
+    default) or two-step blocks (forced through the tuning table)."""
     from soundwavesimulation_amd import _native
 
     lib = _native.lib()
-    for ny, nx, _, _ in SPECS:
-        _native.check(lib.swfd_tune(3, ny, nx, 0, 0, 2))
-    yield
+    lib.swfd_tune_clear()
+    if request.param == "blocks":
+        for ny, nx, _, _ in SPECS:
+            _native.check(lib.swfd_tune(3, ny, nx, 0, 0, 2))
+    yield request.param
     lib.swfd_tune_clear()
 
 
-def test_batched_adjoint_equals_separate_runs(adjoint_blocks):
-    print("batched adjoint, 3 problems", flush=True)
+def test_batched_adjoint_equals_separate_runs(adjoint_form):
+    print(f"batched adjoint ({adjoint_form}), 3 problems", flush=True)
     runs, refs = [], []
     for i, (ny, nx, n_el, N) in enumerate(SPECS):
         solver, sampler, array, rows, cols, dt = _problem(ny, nx, n_el, 10 + i)
