@@ -62,6 +62,44 @@ int swno_bias_act(int64_t rows, int32_t cols, const float *x, const float *bias,
 /* dx[i] = dy[i] * GELU'(pre[i]) (dx may equal dy). */
 int swno_gelu_backward(int64_t n, const float *pre, const float *dy, float *dx, void *stream);
 
+/* Weight and bias gradients of a pointwise linear layer applied to `rows`
+ * rows: dW (M x N, row-major) = A^T B and, when db is not NULL, db[c] =
+ * sum_r B[r, c], for A (rows x M, row stride lda) and B (rows x N, row stride
+ * ldb). Operands are rounded to bf16 for the products (MFMA, fp32
+ * accumulation); the column sums are fp32. Both are summed in a fixed order
+ * through `workspace` (swno_weight_grad_workspace_floats floats), so results
+ * are bitwise reproducible. 1 <= M, N <= 128; an operand wider than 32
+ * columns needs its width and row stride to be multiples of 4 and a 16-byte
+ * aligned base; column sums need N (N / 4 for such an operand) to divide 256.
+ * Replaces, for the operator's backward pass, a split-K GEMM plus a column
+ * sum over the same B (one pass over A and B instead of two over B). */
+size_t swno_weight_grad_workspace_floats(int64_t rows, int32_t M, int32_t N);
+int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float *dW, float *db, float *workspace,
+                     void *stream);
+
+/* Projection head, fused (ch = 32 channels, hidden = 128 units only):
+ * hpre = v Q1 + bq1 (rows x hidden, kept for the backward pass) and
+ * out[r] = GELU(hpre[r]) . Q2 + bq2, for v (rows x ch, 16-byte aligned),
+ * Q1 (ch x hidden), Q2 (hidden). v Q1 is a bf16 MFMA product (fp32
+ * accumulation); bias, GELU and the dot with Q2 are fp32. Replaces a GEMM,
+ * bias + GELU and a second GEMM over the hidden units. */
+int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const float *v, const float *Q1,
+                      const float *bq1, const float *Q2, const float *bq2, float *hpre,
+                      float *out, void *stream);
+
+/* Backward pass of swno_head_forward for d loss / d out = dout (rows):
+ * dv = dh Q1^T with dh = dout Q2^T * GELU'(hpre), dQ1 = v^T dh,
+ * dbq1 = sum_r dh, dQ2 = GELU(hpre)^T dout, dbq2 = sum_r dout. dv and dQ1
+ * are bf16 MFMA products (fp32 accumulation); the parameter gradients are
+ * summed in a fixed order through `workspace`
+ * (swno_head_backward_workspace_floats floats). */
+size_t swno_head_backward_workspace_floats(int64_t rows);
+int swno_head_backward(int64_t rows, int32_t ch, int32_t hidden, const float *dout,
+                       const float *hpre, const float *v, const float *Q1, const float *Q2,
+                       float *dv, float *dQ1, float *dbq1, float *dQ2, float *dbq2,
+                       float *workspace, void *stream);
+
 /* out[c] = beta * out[c] + sum_r x[r, c], summed in a fixed order through
  * `workspace` (swno_col_sum_workspace_floats floats). */
 size_t swno_col_sum_workspace_floats(int64_t rows, int32_t cols);

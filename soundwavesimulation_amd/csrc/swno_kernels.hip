@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "swno.h"
 
@@ -174,6 +175,197 @@ __global__ void __launch_bounds__(256) gemm_reduce_kernel(GemmP p)
     }
 }
 
+// Weight and bias gradients of a pointwise linear layer over many rows:
+// dW (M x N) = A^T B and db (N) = column sums of B, for A (rows x M) and
+// B (rows x N) in row-major layout. Workgroup p takes rows
+// [p rows_per_part, (p + 1) rows_per_part) in slabs of kWgRows rows: the slab
+// is loaded with float4 (or scalar) loads, its A^T and B tiles rounded to
+// bf16 into LDS with K (the row) contiguous, and each wavefront accumulates
+// its 16 x 16 output tiles with MFMA 16x16x32 over the slab. B's column sums
+// are kept in fp32 per thread (a thread's columns do not change from slab to
+// slab) and combined across threads in thread order. Each workgroup writes
+// one partial (M N values, then N) to the workspace; wgrad_reduce_kernel sums
+// the partials in order. The next slab's loads are issued before the current
+// slab's MFMA so that they are in flight meanwhile.
+constexpr int kWgRows = 64;      // rows per slab (two MFMA K steps)
+constexpr int kWgMaxParts = 1024;
+
+struct WgradP {
+    int64_t rows, rows_per_part;
+    int M, N;
+    const float *A;
+    int64_t lda;
+    const float *B;
+    int64_t ldb;
+    int bias;          // write column sums of B
+    float *ws;         // [part][M N + N]
+};
+
+// One operand's share of a slab: item i of thread t is float4 (V) or scalar
+// number e = t + 256 i of the slab in row-major order. Its row and its offset
+// in the operand do not change from slab to slab.
+template <int CP, int V>
+struct WgOperand {
+    static constexpr int kItems = V ? CP / 16 : CP / 4;  // 64 rows x CP columns / 256 threads
+    typedef typename std::conditional<V, f4, float>::type item_t;
+    item_t reg[kItems];
+    int row[kItems];  // slab row, or kWgRows when the item is past the slab
+    int64_t off[kItems];
+    int col[kItems];
+
+    __device__ void init(int cols, int64_t ld, int tid)
+    {
+        const int per_row = V ? cols >> 2 : cols;
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            const int e = tid + 256 * i;
+            const int r = e / per_row, c = (e - r * per_row) * (V ? 4 : 1);
+            row[i] = e < kWgRows * per_row ? r : kWgRows;
+            col[i] = c;
+            off[i] = (int64_t)r * ld + c;
+        }
+    }
+    __device__ void load(const float *X, int64_t ld, int64_t r0, int64_t r_end)
+    {
+        const float *base = X + r0 * ld;
+        // rows of this slab inside the range (items past the slab have row == kWgRows)
+        const int64_t lim = min(r_end - r0, (int64_t)kWgRows);
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            item_t v = item_t{};
+            if (row[i] < lim) v = *(const item_t *)(base + off[i]);
+            reg[i] = v;
+        }
+    }
+    // LDS tile T[col][row] in bf16; rows past the range hold zeros
+    __device__ void store(__bf16 (*T)[kWgRows + kPadK]) const
+    {
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            if (row[i] < kWgRows) {
+                if constexpr (V) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) T[col[i] + j][row[i]] = (__bf16)reg[i][j];
+                } else {
+                    T[col[i]][row[i]] = (__bf16)reg[i];
+                }
+            }
+        }
+    }
+};
+
+template <int MP, int NP, int VA, int VB>
+__global__ void __launch_bounds__(256) wgrad_kernel(WgradP p)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[MP][kWgRows + kPadK];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NP][kWgRows + kPadK];
+    __shared__ float bpart[256][4];
+    constexpr int TM = MP / 16, TN = NP / 16, T = TM * TN;
+    constexpr int TPW = (T + 3) / 4;  // tiles per wavefront
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // padded rows / columns must hold finite values (their outputs are dropped)
+    for (int i = tid; i < MP * (kWgRows + kPadK); i += 256) (&As[0][0])[i] = (__bf16)0.0f;
+    for (int i = tid; i < NP * (kWgRows + kPadK); i += 256) (&Bs[0][0])[i] = (__bf16)0.0f;
+    const int64_t r_begin = (int64_t)blockIdx.x * p.rows_per_part;
+    const int64_t r_end = min(p.rows, r_begin + p.rows_per_part);
+    f4 acc[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    WgOperand<MP, VA> a;
+    WgOperand<NP, VB> b;
+    a.init(p.M, p.lda, tid);
+    b.init(p.N, p.ldb, tid);
+    typename WgOperand<NP, VB>::item_t bsum{};
+    a.load(p.A, p.lda, r_begin, r_end);
+    b.load(p.B, p.ldb, r_begin, r_end);
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += kWgRows) {
+        __syncthreads();
+        a.store(As);
+        b.store(Bs);
+#pragma unroll
+        for (int i = 0; i < WgOperand<NP, VB>::kItems; ++i) bsum += b.reg[i];  // fixed columns
+        __syncthreads();
+        if (r0 + kWgRows < r_end) {
+            a.load(p.A, p.lda, r0 + kWgRows, r_end);
+            b.load(p.B, p.ldb, r0 + kWgRows, r_end);
+        }
+#pragma unroll
+        for (int ks = 0; ks < kWgRows; ks += kTK) {
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) {
+                const int t = wave + 4 * i;
+                if (t < T) {
+                    const int tm = t / TN, tn = t - tm * TN;
+                    const bf16x8 af = *(const bf16x8 *)&As[tm * 16 + r][ks + kq];
+                    const bf16x8 bf = *(const bf16x8 *)&Bs[tn * 16 + r][ks + kq];
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[i], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const int MN = p.M * p.N;
+    const int J = MN + (p.bias ? p.N : 0);
+    float *out = p.ws + (int64_t)blockIdx.x * J;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+        const int t = wave + 4 * i;
+        if (t < T) {
+            const int tm = t / TN, tn = t - tm * TN;
+            const int col = tn * 16 + (lane & 15);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = tm * 16 + (lane >> 4) * 4 + e;
+                if (row < p.M && col < p.N) out[row * p.N + col] = acc[i][e];
+            }
+        }
+    }
+    if (p.bias) {
+        // thread t holds the sums of columns c(t) .. c(t) + 3 (float4) or c(t)
+        if constexpr (VB) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bpart[tid][j] = bsum[j];
+        } else {
+            bpart[tid][0] = bsum;
+        }
+        __syncthreads();
+        const int groups = VB ? p.N >> 2 : p.N;  // distinct column groups
+        for (int c = tid; c < p.N; c += 256) {
+            const int g = VB ? c >> 2 : c, j = VB ? c & 3 : 0;
+            float sum = 0.0f;
+            for (int u = g; u < 256; u += groups) sum += bpart[u][j];
+            out[MN + c] = sum;
+        }
+    }
+}
+
+// out[j] = sum over parts, in part order within 16 contiguous ranges that are
+// then added in range order; j < M N goes to dW, the rest to db.
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(int n_part, int MN, int J,
+                                                           const float *ws, float *dW, float *db)
+{
+    __shared__ float part[16][16];
+    const int tid = threadIdx.x;
+    const int jj = tid & 15, q = tid >> 4;
+    const int j = blockIdx.x * 16 + jj;
+    const int per = (n_part + 15) / 16;
+    const int p0 = q * per, p1 = min(n_part, p0 + per);
+    float s = 0.0f;
+    if (j < J)
+        for (int pp = p0; pp < p1; ++pp) s += ws[(int64_t)pp * J + j];
+    part[q][jj] = s;
+    __syncthreads();
+    if (q == 0 && j < J) {
+        float t = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t += part[u][jj];
+        if (j < MN)
+            dW[j] = t;
+        else
+            db[j - MN] = t;
+    }
+}
+
 __device__ __forceinline__ float gelu(float x)
 {
     return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
@@ -184,6 +376,245 @@ __device__ __forceinline__ float gelu_grad(float x)
     const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
     const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
     return cdf + x * pdf;
+}
+
+// Projection head of the operator, fused: for each pixel row,
+//   hpre = v Q1 + bq1 (H values), out = GELU(hpre) . Q2 + bq2,
+// with v (rows x kHeadC) and the H = kHeadH hidden values never leaving the
+// workgroup except as hpre (kept for the backward pass). Each wavefront takes
+// 16 rows: one MFMA 16x16x32 per 16 hidden columns (K = the 32 channels, Q1
+// held in registers as bf16 B operands), then bias, GELU and the dot with Q2
+// in fp32 in the MFMA's output layout, and a fixed-order reduction over the
+// 16 lanes that hold a row's columns.
+constexpr int kHeadC = 32, kHeadH = 128;
+constexpr int kHeadTiles = kHeadH / 16;
+
+__device__ __forceinline__ float gelu(float x);
+__device__ __forceinline__ float gelu_grad(float x);
+
+__global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const float *v,
+                                                           const float *Q1, const float *bq1,
+                                                           const float *Q2, const float *bq2,
+                                                           float *hpre, float *out)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    bf16x8 qb[kHeadTiles];
+    float bias[kHeadTiles], q2[kHeadTiles];
+#pragma unroll
+    for (int t = 0; t < kHeadTiles; ++t) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) qb[t][j] = (__bf16)Q1[(8 * g + j) * kHeadH + 16 * t + r16];
+        bias[t] = bq1[16 * t + r16];
+        q2[t] = Q2[16 * t + r16];
+    }
+    const float b2 = bq2[0];
+    const int64_t n_blocks = (rows + 63) / 64;
+    for (int64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+        const int64_t row0 = blk * 64 + wave * 16;
+        const int64_t ra = row0 + r16;
+        bf16x8 a;
+        if (ra < rows) {
+            const f4 x0 = *(const f4 *)(v + ra * kHeadC + 8 * g);
+            const f4 x1 = *(const f4 *)(v + ra * kHeadC + 8 * g + 4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[j] = (__bf16)x0[j];
+                a[4 + j] = (__bf16)x1[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.0f;
+        }
+        float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < kHeadTiles; ++t) {
+            const f4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                a, qb[t], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 4 * g + e;
+                const float hp = acc[e] + bias[t];
+                if (row < rows) hpre[row * kHeadH + 16 * t + r16] = hp;
+                part[e] += gelu(hp) * q2[t];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) part[e] += __shfl_xor(part[e], m);
+            const int64_t row = row0 + 4 * g + e;
+            if (r16 == 0 && row < rows) out[row] = part[e] + b2;
+        }
+    }
+}
+
+// Backward pass of the projection head for d loss / d out = dout: with
+// h = GELU(hpre) recomputed and dh = dout Q2^T * GELU'(hpre),
+//   dv = dh Q1^T (rows x kHeadC), and per workgroup the partial sums of
+//   dQ1 = v^T dh, dbq1 = sum dh, dQ2 = h^T dout, dbq2 = sum dout
+// over its rows, written as one partial [dQ1 | dbq1 | dQ2 | dbq2] to the
+// workspace (head_reduce_kernel sums the partials in order). A 64-row block
+// stages dh in LDS as bf16 twice (row-major for dv, transposed for dQ1) and
+// v^T once; dv and dQ1 are MFMA products, the rest fp32.
+constexpr int kHeadJ = kHeadC * kHeadH + 2 * kHeadH + 1;
+
+__global__ void __launch_bounds__(256) head_backward_kernel(int64_t rows, int64_t rows_per_part,
+                                                            const float *dout, const float *hpre,
+                                                            const float *v, const float *Q1,
+                                                            const float *Q2, float *dv,
+                                                            float *ws)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 DH[64][kHeadH + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 DHT[kHeadH][64 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 VT[kHeadC][64 + 8];
+    __shared__ float red[16][2 * kHeadH + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    // B operand of dv = dh Q1^T: B[k = h][n = c] = Q1[c][h]
+    bf16x8 q1t[kHeadC / 16][kHeadH / 32];
+#pragma unroll
+    for (int t = 0; t < kHeadC / 16; ++t)
+#pragma unroll
+        for (int kk = 0; kk < kHeadH / 32; ++kk)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                q1t[t][kk][j] = (__bf16)Q1[(16 * t + r16) * kHeadH + 32 * kk + 8 * g + j];
+    float q2[kHeadTiles];
+#pragma unroll
+    for (int t = 0; t < kHeadTiles; ++t) q2[t] = Q2[16 * t + r16];
+    constexpr int kGradTiles = (kHeadC / 16) * kHeadTiles;  // 16 output tiles of dQ1
+    constexpr int kTilesPerWave = kGradTiles / 4;
+    f4 gq1[kTilesPerWave];
+#pragma unroll
+    for (int i = 0; i < kTilesPerWave; ++i) gq1[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    float gq2[kHeadTiles], gb1[kHeadTiles];
+#pragma unroll
+    for (int t = 0; t < kHeadTiles; ++t) gq2[t] = gb1[t] = 0.0f;
+    float gb2 = 0.0f;
+    const int64_t r_begin = (int64_t)blockIdx.x * rows_per_part;
+    const int64_t r_end = min(rows, r_begin + rows_per_part);
+    for (int64_t blk0 = r_begin; blk0 < r_end; blk0 += 64) {
+        const int64_t row0 = blk0 + 16 * wave;
+        float d[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t row = row0 + 4 * g + e;
+            d[e] = row < r_end ? dout[row] : 0.0f;
+        }
+        if (r16 == 0) gb2 += (d[0] + d[1]) + (d[2] + d[3]);
+#pragma unroll
+        for (int t = 0; t < kHeadTiles; ++t) {
+            const int col = 16 * t + r16;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 4 * g + e;
+                const float hp = row < r_end ? hpre[row * kHeadH + col] : 0.0f;
+                gq2[t] += gelu(hp) * d[e];
+                const float dh = d[e] * q2[t] * gelu_grad(hp);
+                gb1[t] += dh;
+                const int rl = 16 * wave + 4 * g + e;
+                DH[rl][col] = (__bf16)dh;
+                DHT[col][rl] = (__bf16)dh;
+            }
+        }
+        // v^T of the block: 64 rows x 32 channels = 512 float4, two per thread
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int e4 = tid + 256 * i;
+            const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
+            f4 x = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            if (blk0 + rr < r_end) x = *(const f4 *)(v + (blk0 + rr) * kHeadC + c4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) VT[c4 + j][rr] = (__bf16)x[j];
+        }
+        __syncthreads();
+        // dv for this wavefront's 16 rows
+#pragma unroll
+        for (int t = 0; t < kHeadC / 16; ++t) {
+            f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < kHeadH / 32; ++kk) {
+                const bf16x8 a = *(const bf16x8 *)&DH[16 * wave + r16][32 * kk + 8 * g];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, q1t[t][kk], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 4 * g + e;
+                if (row < r_end) dv[row * kHeadC + 16 * t + r16] = acc[e];
+            }
+        }
+        // dQ1 tiles of this wavefront over the block's 64 rows
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; ++i) {
+            const int ti = wave + 4 * i;
+            const int tm = ti / kHeadTiles, tn = ti - tm * kHeadTiles;
+#pragma unroll
+            for (int ks = 0; ks < 64; ks += 32) {
+                const bf16x8 a = *(const bf16x8 *)&VT[16 * tm + r16][ks + 8 * g];
+                const bf16x8 b = *(const bf16x8 *)&DHT[16 * tn + r16][ks + 8 * g];
+                gq1[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, gq1[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float *out = ws + (int64_t)blockIdx.x * kHeadJ;
+#pragma unroll
+    for (int i = 0; i < kTilesPerWave; ++i) {
+        const int ti = wave + 4 * i;
+        const int tm = ti / kHeadTiles, tn = ti - tm * kHeadTiles;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            out[(16 * tm + 4 * g + e) * kHeadH + 16 * tn + r16] = gq1[i][e];
+    }
+    // column sums: 16 contributors per column (4 wavefronts x 4 lane groups),
+    // combined in (wavefront, group) order
+    const int slot = wave * 4 + g;
+#pragma unroll
+    for (int t = 0; t < kHeadTiles; ++t) {
+        red[slot][16 * t + r16] = gb1[t];
+        red[slot][kHeadH + 16 * t + r16] = gq2[t];
+    }
+    if (r16 == 0) red[slot][2 * kHeadH] = gb2;
+    __syncthreads();
+    for (int c = tid; c < 2 * kHeadH + 1; c += 256) {
+        float sum = 0.0f;
+        for (int u = 0; u < 16; ++u) sum += red[u][c];
+        out[kHeadC * kHeadH + c] = sum;
+    }
+}
+
+// Sum of the head's partials in part order (16 contiguous ranges added in
+// range order) into dQ1 | dbq1 | dQ2 | dbq2.
+__global__ void __launch_bounds__(256) head_reduce_kernel(int n_part, const float *ws,
+                                                          float *dQ1, float *dbq1, float *dQ2,
+                                                          float *dbq2)
+{
+    __shared__ float part[16][16];
+    const int tid = threadIdx.x;
+    const int jj = tid & 15, q = tid >> 4;
+    const int j = blockIdx.x * 16 + jj;
+    const int per = (n_part + 15) / 16;
+    const int p0 = q * per, p1 = min(n_part, p0 + per);
+    float s = 0.0f;
+    if (j < kHeadJ)
+        for (int pp = p0; pp < p1; ++pp) s += ws[(int64_t)pp * kHeadJ + j];
+    part[q][jj] = s;
+    __syncthreads();
+    if (q == 0 && j < kHeadJ) {
+        float t = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t += part[u][jj];
+        constexpr int o1 = kHeadC * kHeadH, o2 = o1 + kHeadH, o3 = o2 + kHeadH;
+        if (j < o1)
+            dQ1[j] = t;
+        else if (j < o2)
+            dbq1[j - o1] = t;
+        else if (j < o3)
+            dQ2[j - o2] = t;
+        else
+            dbq2[0] = t;
+    }
 }
 
 __global__ void __launch_bounds__(256) bias_act_kernel(int64_t rows, int cols, const float *x,
@@ -408,9 +839,56 @@ int k_chunk_of(const swno_gemm_args *g)
     return std::max(chunk, kTK);
 }
 
+template <int MP, int NP, int VA, int VB>
+void launch_wgrad(dim3 grid, hipStream_t st, const WgradP &p)
+{
+    hipLaunchKernelGGL((wgrad_kernel<MP, NP, VA, VB>), grid, dim3(256), 0, st, p);
+}
+
+template <int MP, int NP>
+void launch_wgrad_v(bool va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
+{
+    if (va) {
+        if (vb) {
+            launch_wgrad<MP, NP, 1, 1>(grid, st, p);
+        } else if constexpr (NP <= 32) {
+            launch_wgrad<MP, NP, 1, 0>(grid, st, p);
+        }
+    } else if constexpr (MP <= 32) {
+        if (vb) {
+            launch_wgrad<MP, NP, 0, 1>(grid, st, p);
+        } else if constexpr (NP <= 32) {
+            launch_wgrad<MP, NP, 0, 0>(grid, st, p);
+        }
+    }
+}
+
+template <int MP>
+void launch_wgrad_n(int np, bool va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
+{
+    switch (np) {
+    case 16: launch_wgrad_v<MP, 16>(va, vb, grid, st, p); break;
+    case 32: launch_wgrad_v<MP, 32>(va, vb, grid, st, p); break;
+    case 64: launch_wgrad_v<MP, 64>(va, vb, grid, st, p); break;
+    default: launch_wgrad_v<MP, 128>(va, vb, grid, st, p); break;
+    }
+}
+
+void launch_wgrad_m(int mp, int np, bool va, bool vb, dim3 grid, hipStream_t st,
+                    const WgradP &p)
+{
+    switch (mp) {
+    case 16: launch_wgrad_n<16>(np, va, vb, grid, st, p); break;
+    case 32: launch_wgrad_n<32>(np, va, vb, grid, st, p); break;
+    case 64: launch_wgrad_n<64>(np, va, vb, grid, st, p); break;
+    default: launch_wgrad_n<128>(np, va, vb, grid, st, p); break;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
 
 const char *swno_last_error(void) { return g_err.c_str(); }
 
@@ -492,6 +970,121 @@ int swno_gelu_backward(int64_t n, const float *pre, const float *dy, float *dx, 
         return fail(SWNO_EINVAL, "gelu_backward: bad arguments");
     hipLaunchKernelGGL(gelu_backward_kernel, dim3(blocks_for(n)), dim3(256), 0,
                        as_stream(stream), n, pre, dy, dx);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
+size_t swno_weight_grad_workspace_floats(int64_t rows, int32_t M, int32_t N)
+{
+    if (rows < 1 || M < 1 || N < 1) return 0;
+    const int64_t slabs = (rows + kWgRows - 1) / kWgRows;
+    const int64_t parts = std::min<int64_t>(kWgMaxParts, slabs);
+    return (size_t)parts * (size_t)(M * N + N);
+}
+
+int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float *dW, float *db, float *workspace,
+                     void *stream)
+{
+    if (rows < 1 || M < 1 || N < 1 || M > 128 || N > 128)
+        return fail(SWNO_EINVAL, "weight_grad: rows >= 1 and 1 <= M, N <= 128 required");
+    if (A == nullptr || B == nullptr || dW == nullptr || workspace == nullptr)
+        return fail(SWNO_EINVAL, "weight_grad: operand or workspace is NULL");
+    if (lda < M || ldb < N) return fail(SWNO_EINVAL, "weight_grad: lda < M or ldb < N");
+    WgradP p;
+    std::memset(&p, 0, sizeof p);
+    p.rows = rows;
+    p.M = M;
+    p.N = N;
+    p.A = A;
+    p.lda = lda;
+    p.B = B;
+    p.ldb = ldb;
+    const bool va = M % 4 == 0 && lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    const bool vb = N % 4 == 0 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0;
+    // scalar loads keep up to 8 values per thread: 64 rows x 32 columns
+    if ((!va && M > 32) || (!vb && N > 32))
+        return fail(SWNO_EINVAL, "weight_grad: an operand wider than 32 columns must allow "
+                                 "float4 loads (width and row stride multiples of 4, 16-byte "
+                                 "aligned)");
+    p.bias = db != nullptr;
+    if (p.bias && (vb ? 256 % (N / 4) : 256 % N) != 0)
+        return fail(SWNO_EINVAL, "weight_grad: column sums need N (N / 4 with float4 loads) "
+                                 "to divide 256");
+    const int64_t slabs = (rows + kWgRows - 1) / kWgRows;
+    const int64_t parts0 = std::min<int64_t>(kWgMaxParts, slabs);
+    p.rows_per_part = (slabs + parts0 - 1) / parts0 * kWgRows;
+    const int n_part = (int)((rows + p.rows_per_part - 1) / p.rows_per_part);
+    p.ws = workspace;
+    hipStream_t st = as_stream(stream);
+    const int mp = M <= 16 ? 16 : M <= 32 ? 32 : M <= 64 ? 64 : 128;
+    const int np = N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 128;
+    launch_wgrad_m(mp, np, va, vb, dim3((unsigned)n_part), st, p);
+    SWNO_HIP_CHECK(hipGetLastError());
+    const int MN = M * N, J = MN + (p.bias ? N : 0);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((J + 15) / 16)), dim3(256), 0, st,
+                       n_part, MN, J, (const float *)workspace, dW, db);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
+constexpr int kHeadMaxParts = 1024;
+
+int head_parts(int64_t rows, int64_t *rows_per_part)
+{
+    const int64_t blocks = (rows + 63) / 64;
+    const int64_t parts0 = std::min<int64_t>(kHeadMaxParts, blocks);
+    *rows_per_part = (blocks + parts0 - 1) / parts0 * 64;
+    return (int)((rows + *rows_per_part - 1) / *rows_per_part);
+}
+
+int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const float *v, const float *Q1,
+                      const float *bq1, const float *Q2, const float *bq2, float *hpre,
+                      float *out, void *stream)
+{
+    if (ch != kHeadC || hidden != kHeadH)
+        return fail(SWNO_EINVAL, "head_forward: built for 32 channels and 128 hidden units");
+    if (rows < 1 || v == nullptr || Q1 == nullptr || bq1 == nullptr || Q2 == nullptr ||
+        bq2 == nullptr || hpre == nullptr || out == nullptr)
+        return fail(SWNO_EINVAL, "head_forward: bad arguments");
+    if (reinterpret_cast<uintptr_t>(v) % 16 != 0)
+        return fail(SWNO_EINVAL, "head_forward: v must be 16-byte aligned");
+    const int64_t blocks = (rows + 63) / 64;
+    const unsigned grid = (unsigned)std::min<int64_t>(blocks, 4096);
+    hipLaunchKernelGGL(head_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), rows,
+                       v, Q1, bq1, Q2, bq2, hpre, out);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
+size_t swno_head_backward_workspace_floats(int64_t rows)
+{
+    if (rows < 1) return 0;
+    int64_t rpp = 0;
+    return (size_t)head_parts(rows, &rpp) * kHeadJ;
+}
+
+int swno_head_backward(int64_t rows, int32_t ch, int32_t hidden, const float *dout,
+                       const float *hpre, const float *v, const float *Q1, const float *Q2,
+                       float *dv, float *dQ1, float *dbq1, float *dQ2, float *dbq2,
+                       float *workspace, void *stream)
+{
+    if (ch != kHeadC || hidden != kHeadH)
+        return fail(SWNO_EINVAL, "head_backward: built for 32 channels and 128 hidden units");
+    if (rows < 1 || dout == nullptr || hpre == nullptr || v == nullptr || Q1 == nullptr ||
+        Q2 == nullptr || dv == nullptr || dQ1 == nullptr || dbq1 == nullptr ||
+        dQ2 == nullptr || dbq2 == nullptr || workspace == nullptr)
+        return fail(SWNO_EINVAL, "head_backward: bad arguments");
+    if (reinterpret_cast<uintptr_t>(v) % 16 != 0)
+        return fail(SWNO_EINVAL, "head_backward: v must be 16-byte aligned");
+    int64_t rpp = 0;
+    const int n_part = head_parts(rows, &rpp);
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(head_backward_kernel, dim3((unsigned)n_part), dim3(256), 0, st, rows, rpp,
+                       dout, hpre, v, Q1, Q2, dv, workspace);
+    SWNO_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(head_reduce_kernel, dim3((unsigned)((kHeadJ + 15) / 16)), dim3(256), 0,
+                       st, n_part, (const float *)workspace, dQ1, dbq1, dQ2, dbq2);
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }

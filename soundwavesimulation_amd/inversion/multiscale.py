@@ -76,11 +76,18 @@ class StageResult:
     times: list = field(default_factory=list)
 
 
-def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out):
-    """Wrap misfit_and_grad to record interior RMS speed error and wall time."""
+def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out, cached=None):
+    """Wrap misfit_and_grad to record interior RMS speed error and wall time.
+
+    ``cached = (m, value, grad)`` answers the first call at exactly that ``m``."""
+    pending = [cached] if cached is not None else []
 
     def misfit_and_grad(m):
-        value, grad = problem.misfit_and_grad(m)
+        if pending and np.array_equal(m, pending[0][0]):
+            _, value, grad = pending.pop()
+        else:
+            pending.clear()
+            value, grad = problem.misfit_and_grad(m)
         rmse_out.append(rms_c_error(c_from_m(problem.project(m)), c_true_grid, mask))
         times_out.append(time.perf_counter() - t0)
         return value, grad
@@ -90,8 +97,13 @@ def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out):
 
 def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_min=1400.0,
               c_max=1650.0, c_background=1500.0, cfl=0.3, parallel_shots="auto",
-              optimizer="cg", learning_rate=None, margin_cells=2, verbose=False, t0=None):
-    """Run one stage; ``c_true_fn(x_m, y_m)`` gives the true speed on the stage grid."""
+              optimizer="cg", learning_rate=None, margin_cells=2, verbose=False, t0=None,
+              start_hook=None):
+    """Run one stage; ``c_true_fn(x_m, y_m)`` gives the true speed on the stage grid.
+
+    ``start_hook(problem, m_start, c_true, mask)`` may replace the start once the
+    stage's objective exists (its time counts in the stage); it returns ``m`` or
+    ``(m, (value, grad))`` when it has already evaluated the objective at ``m``."""
     t_start = time.perf_counter() if t0 is None else t0
     x = axis_centers(stage.n, stage.spacing_m)
     c_true = c_true_fn(x, x)
@@ -112,8 +124,15 @@ def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_m
     problem = TorchLeastSquaresFWI(solver, sampler, array, pulse, stage.n_steps, n_elements,
                                    observed, mask, c_min=c_min, c_max=c_max,
                                    c_background=c_background, parallel_shots=parallel_shots)
+    cached = None
+    if start_hook is not None:
+        m_start = start_hook(problem, m_start, c_true, mask)
+        if isinstance(m_start, tuple):
+            m_start, (value, grad) = m_start
+            m_start = problem.project(m_start)
+            cached = (m_start, value, grad)
     rmse, times = [], []
-    mg = _objective_with_trace(problem, c_true, mask, t_start, rmse, times)
+    mg = _objective_with_trace(problem, c_true, mask, t_start, rmse, times, cached)
     t_stage = time.perf_counter()
     if optimizer == "cg":
         m, history = polak_ribiere(m_start, problem.misfit, mg, problem.project, mask,
@@ -132,8 +151,11 @@ def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_m
                        times=times), problem, c_true, mask
 
 
-def multiscale_fwi(stages, c_true_fn, m0_fn, device, **kwargs):
-    """Run ``stages`` in order; ``m0_fn(x_m, y_m)`` gives the start on the first grid."""
+def multiscale_fwi(stages, c_true_fn, m0_fn, device, start_hook=None, on_stage=None,
+                   **kwargs):
+    """Run ``stages`` in order; ``m0_fn(x_m, y_m)`` gives the start on the first grid and
+    ``start_hook`` (see :func:`run_stage`) may replace it inside the first stage;
+    ``on_stage(results)`` is called after every stage."""
     t0 = time.perf_counter()
     results = []
     m = None
@@ -147,9 +169,12 @@ def multiscale_fwi(stages, c_true_fn, m0_fn, device, **kwargs):
             m_start = resize_slowness(m, axes, (x, x), background)
         else:
             m_start = m
-        result, problem, c_true, mask = run_stage(stage, c_true_fn, m_start, device, t0=t0,
-                                                  **kwargs)
+        result, problem, c_true, mask = run_stage(
+            stage, c_true_fn, m_start, device, t0=t0,
+            start_hook=start_hook if not results else None, **kwargs)
         results.append(result)
+        if on_stage is not None:
+            on_stage(results)
         m = result.m
         axes = (x, x)
     return results

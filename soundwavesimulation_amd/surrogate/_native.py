@@ -28,6 +28,8 @@ class GemmArgs(ctypes.Structure):
 #: every symbol include/swno.h declares
 EXPORTS = ("swno_last_error", "swno_gemm_workspace_floats", "swno_gemm", "swno_bias_act",
            "swno_gelu_backward", "swno_col_sum_workspace_floats", "swno_col_sum",
+           "swno_weight_grad_workspace_floats", "swno_weight_grad", "swno_head_forward",
+           "swno_head_backward_workspace_floats", "swno_head_backward",
            "swno_spectral_pack", "swno_mix_assemble", "swno_mix_assemble_backward",
            "swno_mse_workspace_floats", "swno_mse", "swno_adamw")
 
@@ -54,6 +56,13 @@ def load_library():
     lib.swno_col_sum_workspace_floats.restype = ctypes.c_size_t
     lib.swno_col_sum_workspace_floats.argtypes = [_i64, _i32]
     lib.swno_col_sum.argtypes = [_i64, _i32, _vp, _f32, _vp, _vp, _vp]
+    lib.swno_weight_grad_workspace_floats.restype = ctypes.c_size_t
+    lib.swno_weight_grad_workspace_floats.argtypes = [_i64, _i32, _i32]
+    lib.swno_weight_grad.argtypes = [_i64, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp]
+    lib.swno_head_forward.argtypes = [_i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.swno_head_backward_workspace_floats.restype = ctypes.c_size_t
+    lib.swno_head_backward_workspace_floats.argtypes = [_i64]
+    lib.swno_head_backward.argtypes = [_i64, _i32, _i32] + [_vp] * 12
     lib.swno_spectral_pack.argtypes = [_i32, _i32, _i32, _i32, _vp, _vp, _vp]
     lib.swno_mix_assemble.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp]
     lib.swno_mix_assemble_backward.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _vp]

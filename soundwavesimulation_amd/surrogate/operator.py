@@ -30,6 +30,7 @@ out like the parameters (so AdamW is one launch).
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import torch
@@ -119,6 +120,9 @@ class SpectralOperator:
         self.rm = [torch.empty((modes * modes, 2 * width, 2 * width), device=dev, dtype=_F32)
                    for _ in range(layers)]
         self.drm = torch.empty_like(self.rm[0])
+        # the projection head as one fused kernel each way (built for width 32, hidden 128)
+        self.fused_head = ((width, hidden) == (32, 128)
+                           and os.environ.get("SWNO_FUSED_HEAD", "1") != "0")
         self._bufs = {}
         self.adam_m = torch.zeros_like(self.flat)
         self.adam_v = torch.zeros_like(self.flat)
@@ -143,8 +147,9 @@ class SpectralOperator:
             "X": [e(B * m * m * 2 * C) for _ in range(L)], "Y": e(B * m * m * 2 * C),
             "U": e(B * 4 * m * m * C), "Z": e(B * n * 2 * m * C),
             "dv": e(rows, C), "dpre": e(rows, C), "dh": e(rows, H), "dout": e(rows, 1),
-            "ws": e(1 << 22),
-            "colws": e(int(self.lib.swno_col_sum_workspace_floats(rows, max(H, C)))),
+            "wgws": e(max(int(self.lib.swno_weight_grad_workspace_floats(rows, mm, nn))
+                          for mm, nn in ((H, 1), (C, H), (C, C), (self.Cin, C)))),
+            "headws": e(int(self.lib.swno_head_backward_workspace_floats(rows))),
             "msews": e(int(self.lib.swno_mse_workspace_floats(rows)) + 2),
             "loss": e(2),
         }
@@ -161,8 +166,10 @@ class SpectralOperator:
                 raise ValueError("GEMM workspace too small")
         _native.check(self.lib.swno_gemm(g, _ptr(ws), self._stream()))
 
-    def _ksplit(self, K):
-        return max(1, min(256, K // 4096))
+    def weight_grad(self, rows, A, M, B, N, dW, db, ws):
+        """dW (M x N) = A^T B and db = column sums of B over ``rows`` rows."""
+        _native.check(self.lib.swno_weight_grad(rows, M, N, _ptr(A), M, _ptr(B), N, _ptr(dW),
+                                                _ptr(db), _ptr(ws), self._stream()))
 
     # ------------------------------------------------------------ spectral
     def _spectral(self, l, v, out, bufs, B):
@@ -241,14 +248,20 @@ class SpectralOperator:
                       beta=1.0)
             _native.check(lib.swno_bias_act(rows, C, _ptr(pre), _ptr(self.p[f"b{l}"]), _ptr(pre),
                                             _ptr(v[l + 1]), int(l < self.L - 1), st))
-        self.gemm(rows, H, C, 1, v[self.L], (C, 1, 0), self.p["Q1"], (H, 1, 0), bufs["hpre"],
-                  (H, 1, 0))
-        _native.check(lib.swno_bias_act(rows, H, _ptr(bufs["hpre"]), _ptr(self.p["bq1"]),
-                                        _ptr(bufs["hpre"]), _ptr(bufs["h"]), 1, st))
-        self.gemm(rows, 1, H, 1, bufs["h"], (H, 1, 0), self.p["Q2"], (1, 1, 0), bufs["outpre"],
-                  (1, 1, 0))
-        _native.check(lib.swno_bias_act(rows, 1, _ptr(bufs["outpre"]), _ptr(self.p["bq2"]),
-                                        _ptr(bufs["out"]), _ptr(bufs["out"]), 0, st))
+        if self.fused_head:
+            _native.check(lib.swno_head_forward(
+                rows, C, H, _ptr(v[self.L]), _ptr(self.p["Q1"]), _ptr(self.p["bq1"]),
+                _ptr(self.p["Q2"]), _ptr(self.p["bq2"]), _ptr(bufs["hpre"]), _ptr(bufs["out"]),
+                st))
+        else:
+            self.gemm(rows, H, C, 1, v[self.L], (C, 1, 0), self.p["Q1"], (H, 1, 0),
+                      bufs["hpre"], (H, 1, 0))
+            _native.check(lib.swno_bias_act(rows, H, _ptr(bufs["hpre"]), _ptr(self.p["bq1"]),
+                                            _ptr(bufs["hpre"]), _ptr(bufs["h"]), 1, st))
+            self.gemm(rows, 1, H, 1, bufs["h"], (H, 1, 0), self.p["Q2"], (1, 1, 0),
+                      bufs["outpre"], (1, 1, 0))
+            _native.check(lib.swno_bias_act(rows, 1, _ptr(bufs["outpre"]), _ptr(self.p["bq2"]),
+                                            _ptr(bufs["out"]), _ptr(bufs["out"]), 0, st))
         self._last_B = B
         return bufs["out"].view(B, n, n)
 
@@ -276,24 +289,25 @@ class SpectralOperator:
         lib = self.lib
         n, C, H = self.n, self.C, self.H
         rows = B * n * n
-        ws, colws = bufs["ws"], bufs["colws"]
+        wgws = bufs["wgws"]
         g = self.g
-        ks = self._ksplit(rows)
-        # projection
-        self.gemm(H, 1, rows, 1, bufs["h"], (1, H, 0), dout, (1, 1, 0), g["Q2"], (1, 1, 0),
-                  k_split=ks, ws=ws)
-        _native.check(lib.swno_col_sum(rows, 1, _ptr(dout), 0.0, _ptr(g["bq2"]), _ptr(colws),
-                                       st))
-        self.gemm(rows, H, 1, 1, dout, (1, 1, 0), self.p["Q2"], (1, 1, 0), bufs["dh"],
-                  (H, 1, 0))
-        _native.check(lib.swno_gelu_backward(rows * H, _ptr(bufs["hpre"]), _ptr(bufs["dh"]),
-                                             _ptr(bufs["dh"]), st))
-        self.gemm(C, H, rows, 1, bufs["v"][self.L], (1, C, 0), bufs["dh"], (H, 1, 0), g["Q1"],
-                  (H, 1, 0), k_split=ks, ws=ws)
-        _native.check(lib.swno_col_sum(rows, H, _ptr(bufs["dh"]), 0.0, _ptr(g["bq1"]),
-                                       _ptr(colws), st))
         dv, dpre = bufs["dv"], bufs["dpre"]
-        self.gemm(rows, C, H, 1, bufs["dh"], (H, 1, 0), self.p["Q1"], (1, H, 0), dv, (C, 1, 0))
+        # projection
+        if self.fused_head:
+            _native.check(lib.swno_head_backward(
+                rows, C, H, _ptr(dout), _ptr(bufs["hpre"]), _ptr(bufs["v"][self.L]),
+                _ptr(self.p["Q1"]), _ptr(self.p["Q2"]), _ptr(dv), _ptr(g["Q1"]), _ptr(g["bq1"]),
+                _ptr(g["Q2"]), _ptr(g["bq2"]), _ptr(bufs["headws"]), st))
+        else:
+            self.weight_grad(rows, bufs["h"], H, dout, 1, g["Q2"], g["bq2"], wgws)
+            self.gemm(rows, H, 1, 1, dout, (1, 1, 0), self.p["Q2"], (1, 1, 0), bufs["dh"],
+                      (H, 1, 0))
+            _native.check(lib.swno_gelu_backward(rows * H, _ptr(bufs["hpre"]),
+                                                 _ptr(bufs["dh"]), _ptr(bufs["dh"]), st))
+            self.weight_grad(rows, bufs["v"][self.L], C, bufs["dh"], H, g["Q1"], g["bq1"],
+                             wgws)
+            self.gemm(rows, C, H, 1, bufs["dh"], (H, 1, 0), self.p["Q1"], (1, H, 0), dv,
+                      (C, 1, 0))
         # spectral layers, last to first
         for l in range(self.L - 1, -1, -1):
             v, pre = bufs["v"][l], bufs["pre"][l]
@@ -302,16 +316,11 @@ class SpectralOperator:
                                                      st))
             else:
                 dpre.copy_(dv)
-            _native.check(lib.swno_col_sum(rows, C, _ptr(dpre), 0.0, _ptr(g[f"b{l}"]),
-                                           _ptr(colws), st))
-            self.gemm(C, C, rows, 1, v, (1, C, 0), dpre, (C, 1, 0), g[f"W{l}"], (C, 1, 0),
-                      k_split=ks, ws=ws)
+            self.weight_grad(rows, v, C, dpre, C, g[f"W{l}"], g[f"b{l}"], wgws)
             self.gemm(rows, C, C, 1, dpre, (C, 1, 0), self.p[f"W{l}"], (1, C, 0), dv, (C, 1, 0))
             self._spectral_backward(l, dpre, dv, bufs, B)
         # lifting
-        self.gemm(self.Cin, C, rows, 1, bufs["inp"], (1, self.Cin, 0), dv, (C, 1, 0), g["P"],
-                  (C, 1, 0), k_split=ks, ws=ws)
-        _native.check(lib.swno_col_sum(rows, C, _ptr(dv), 0.0, _ptr(g["bp"]), _ptr(colws), st))
+        self.weight_grad(rows, bufs["inp"], self.Cin, dv, C, g["P"], g["bp"], wgws)
 
     def adamw_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4):
         self.adam_step += 1

@@ -63,6 +63,57 @@ def test_gemm_matches_bf16_rounded_product(M, N, K, batch, ta, tb, k_split, beta
     assert err <= 1e-5, err
 
 
+@pytest.mark.parametrize("rows,M,N,bias,offset", [
+    (70000, 32, 32, True, 0),      # a layer's W_l / b_l
+    (70000, 32, 128, True, 0),     # Q1 / bq1
+    (70000, 128, 1, True, 0),      # Q2 / bq2 (scalar B)
+    (70000, 3, 32, True, 0),       # lifting P / bp (scalar A)
+    (1000, 20, 24, False, 0),      # padded tiles, one partial per 64-row slab
+    (1000, 20, 16, True, 0),
+    (33, 16, 64, False, 0),        # a single partial slab, no column sums
+    (5000, 8, 8, True, 1),         # misaligned bases: scalar loads on both sides
+    (2_200_000, 32, 32, True, 0),  # more slabs than partials
+])
+def test_weight_grad_matches_bf16_rounded_product(rows, M, N, bias, offset):
+    print(f"weight_grad rows={rows} M={M} N={N} bias={bias} offset={offset}", flush=True)
+    lib = _native.load_library()
+    g = torch.Generator().manual_seed(rows + M + N)
+    A = torch.randn(rows * M + offset, generator=g).to(DEV)[offset:].view(rows, M)
+    B = torch.randn(rows * N + offset, generator=g).to(DEV)[offset:].view(rows, N)
+    dW = torch.full((M, N), float("nan"), device=DEV)
+    db = torch.full((N,), float("nan"), device=DEV)
+    ws = torch.empty(int(lib.swno_weight_grad_workspace_floats(rows, M, N)), device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    args = (rows, M, N, A.data_ptr(), M, B.data_ptr(), N, dW.data_ptr(),
+            db.data_ptr() if bias else None, ws.data_ptr(), st)
+    _native.check(lib.swno_weight_grad(*args))
+    first = (dW.clone(), db.clone())
+    _native.check(lib.swno_weight_grad(*args))  # fixed summation order: bitwise repeatable
+    torch.cuda.synchronize()
+    assert torch.equal(first[0], dW)
+    ref = _bf16(A.cpu()).T @ _bf16(B.cpu())
+    err = float((dW.cpu().double() - ref).norm() / ref.norm())
+    assert err <= 1e-5, err
+    if bias:
+        assert torch.equal(first[1], db)
+        ref_b = B.cpu().double().sum(0)
+        errb = float((db.cpu().double() - ref_b).norm() / B.cpu().double().abs().sum(0).norm())
+        assert errb <= 1e-6, errb
+    else:
+        assert torch.isnan(db).all()  # untouched
+
+
+def test_weight_grad_rejects_bad_shapes():
+    lib = _native.load_library()
+    x = torch.zeros(1024, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    p = x.data_ptr()
+    assert lib.swno_weight_grad(16, 129, 4, p, 129, p, 4, p, None, p, st) != 0  # M > 128
+    assert lib.swno_weight_grad(16, 33, 4, p + 4, 36, p, 4, p, None, p, st) != 0  # wide scalar
+    assert lib.swno_weight_grad(16, 4, 12, p, 4, p, 12, p, p, p, st) != 0  # 12 / 4 ∤ 256
+    assert lib.swno_weight_grad(16, 4, 4, p, 2, p, 4, p, None, p, st) != 0  # lda < M
+
+
 def test_elementwise_kernels():
     lib = _native.load_library()
     st = torch.cuda.current_stream().cuda_stream
@@ -131,9 +182,10 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_operator_forward_and_gradients_match_oracle():
-    n, m, C, H, L, B = 32, 8, 8, 16, 4, 3
-    print(f"operator n={n} modes={m} width={C}", flush=True)
+@pytest.mark.parametrize("C,H", [(8, 16), (32, 128)])  # (32, 128): fused projection head
+def test_operator_forward_and_gradients_match_oracle(C, H):
+    n, m, L, B = 32, 8, 4, 3
+    print(f"operator n={n} modes={m} width={C} hidden={H}", flush=True)
     rng = np.random.default_rng(7)
     params = init_params(seed=11, width=C, modes=m, layers=L, hidden=H)
     for l in range(L):
@@ -160,6 +212,40 @@ def test_operator_forward_and_gradients_match_oracle():
         e = _rel(got[k], ref_g[k])
         print(f"  grad {k}: rel-L2 {e:.2e}")
         assert e <= 3e-2, k
+
+
+def test_fused_head_matches_separate_kernels():
+    n, m, C, H, B = 32, 8, 32, 128, 5
+    print("fused head vs GEMM + bias/GELU + GEMM", flush=True)
+    rng = np.random.default_rng(3)
+    params = init_params(seed=4, width=C, modes=m, hidden=H)
+    params["bq1"] = rng.standard_normal(H) * 0.3
+    params["bq2"] = rng.standard_normal(1)
+    inp = torch.as_tensor(rng.standard_normal((B, n, n, 3)), dtype=torch.float32, device=DEV)
+    target = torch.as_tensor(rng.standard_normal((B, n, n)), dtype=torch.float32, device=DEV)
+    res = []
+    for fused in (True, False):
+        op = SpectralOperator(params, n, DEV, width=C, modes=m, hidden=H)
+        op.fused_head = fused
+        out = op.forward(inp).clone()
+        op.loss_and_backward(target)
+        torch.cuda.synchronize()
+        res.append((out.cpu().numpy(), op.gradients(), op))
+    e = _rel(res[0][0], res[1][0])
+    print(f"  out rel-L2 {e:.2e}")
+    assert e <= 5e-3
+    for k in params:
+        e = _rel(res[0][1][k], res[1][1][k])
+        print(f"  grad {k}: rel-L2 {e:.2e}")
+        assert e <= 1e-2, k
+    # repeatable bit for bit (fixed-order sums)
+    op = res[0][2]
+    op.forward(inp)
+    op.loss_and_backward(target)
+    torch.cuda.synchronize()
+    again = op.gradients()
+    for k in params:
+        assert np.array_equal(again[k], res[0][1][k]), k
 
 
 def test_adamw_training_reduces_loss():

@@ -149,3 +149,25 @@ def test_bench_refuses_several_gpus():
                           capture_output=True, text=True, timeout=120)
     assert proc.returncode == 2
     assert "one GPU" in proc.stderr
+
+
+def test_warm_start_network_input():
+    from soundwavesimulation_amd.surrogate.warm_start import block_mean, network_input
+
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((8, 8))
+    assert np.allclose(block_mean(a, 2)[1, 2], a[2:4, 4:6].mean())
+    assert np.allclose(block_mean(a, 8), a.mean())
+    with pytest.raises(ValueError):
+        block_mean(a, 3)
+    mask = np.zeros((8, 8), dtype=bool)
+    mask[2:6, 2:6] = True
+    g = rng.standard_normal((8, 8))
+    c = 1500.0 + 50.0 * rng.standard_normal((8, 8))
+    x = network_input(g, c, mask, 2)
+    assert x.shape == (4, 4, 4)
+    assert np.isclose(np.max(np.abs(x[..., 0])), 1.0)
+    assert np.all(x[0, :, 0] == 0.0)  # outside the mask: no gradient
+    assert np.allclose(x[..., 1], (block_mean(c, 2) - 1500.0) / 100.0)
+    assert np.allclose(x[0, :, 2], np.linspace(-1, 1, 4)) and np.allclose(x[:, 0, 3],
+                                                                           np.linspace(-1, 1, 4))
