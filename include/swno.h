@@ -78,6 +78,26 @@ int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t
                      const float *B, int64_t ldb, float *dW, float *db, float *workspace,
                      void *stream);
 
+/* Tail of a spectral layer, fused, over `slices` slices (one sample row
+ * each: n pixels x ch channels, channel-last; ch = 32, n a multiple of 16 up
+ * to 128, an even number of modes up to 16):
+ *   pre = Gx Z + v W + bias and vnext = GELU(pre) (gelu_act != 0) or pre,
+ * with Gx (n x 2 modes) the inverse x transform, Z (slices, 2 modes, ch) the
+ * spectra after the y inverse transform, v the layer input, W (ch x ch).
+ * Replaces the inverse-x GEMM, the pointwise GEMM (beta = 1) and bias + GELU.
+ * Gx and v must be 16-byte aligned. */
+int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *Gx,
+                       const float *Z, const float *v, const float *W, const float *bias,
+                       float *pre, float *vnext, int32_t gelu_act, void *stream);
+
+/* Backward counterpart: dv = dpre W^T + Fx^T dT1, with Fx (2 modes x n) the
+ * forward x transform and dT1 (slices, 2 modes, ch); out = dv * GELU'(pre_prev)
+ * when pre_prev is not NULL (the previous layer's dpre), else out = dv.
+ * out must not alias dpre or pre_prev; dpre and W must be 16-byte aligned. */
+int swno_layer_backward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *dpre,
+                        const float *W, const float *Fx, const float *dT1,
+                        const float *pre_prev, float *out, void *stream);
+
 /* Projection head, fused (ch = 32 channels, hidden = 128 units only):
  * hpre = v Q1 + bq1 (rows x hidden, kept for the backward pass) and
  * out[r] = GELU(hpre[r]) . Q2 + bq2, for v (rows x ch, 16-byte aligned),

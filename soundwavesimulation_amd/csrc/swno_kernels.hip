@@ -378,6 +378,156 @@ __device__ __forceinline__ float gelu_grad(float x)
     return cdf + x * pdf;
 }
 
+// Tail of a spectral layer, fused, for kLayerC = 32 channels. A slice is one
+// row y of one sample b (n pixels x 32 channels, channel-last). Forward:
+//   pre = Gx Z + v W + bias, vnext = GELU(pre) (or pre for the last layer),
+// with Gx (n x 2m) the inverse x transform, Z the slice's 2m x 32 spectrum,
+// v the slice's input and W the layer's pointwise weights. Backward (for the
+// gradient dpre of a layer's pre-activation):
+//   dv = dpre W^T + Fx^T dT1, out = dv * GELU'(pre_prev) (or dv),
+// with Fx (2m x n) the forward x transform and dT1 the slice's 2m x 32
+// gradient of the x transform's output; pre_prev is the previous layer's
+// pre-activation, so `out` is that layer's dpre. Both are two MFMA 16x16x32
+// products summed in one accumulator (2m <= 32 and 32 channels: one MFMA per
+// product and 16 x 16 tile), then the elementwise epilogue, so the slice is
+// read once and written once. Each wavefront takes row tiles w, w + 4, ... of
+// the slice (n <= 128: at most two) and both 16-column tiles; the shared
+// matrices are held in registers as bf16 operands for the kernel's lifetime.
+constexpr int kLayerC = 32;
+constexpr int kLayerMaxN = 128;
+
+__device__ __forceinline__ float gelu(float x);
+__device__ __forceinline__ float gelu_grad(float x);
+
+// 8 consecutive floats (two float4) at p, or zeros past `valid` of them
+__device__ __forceinline__ bf16x8 bf16_row8(const float *p, int valid)
+{
+    bf16x8 r;
+    if (valid >= 8) {
+        const f4 x0 = *(const f4 *)p;
+        const f4 x1 = *(const f4 *)(p + 4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            r[j] = (__bf16)x0[j];
+            r[4 + j] = (__bf16)x1[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = (__bf16)(j < valid ? p[j] : 0.0f);
+    }
+    return r;
+}
+
+// 8 floats p[0], p[stride], ..., or zeros from index `valid` on
+__device__ __forceinline__ bf16x8 bf16_col8(const float *p, int64_t stride, int valid)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = (__bf16)(j < valid ? p[j * stride] : 0.0f);
+    return r;
+}
+
+__global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n, int k1,
+                                                            const float *Gx, const float *Z,
+                                                            const float *v, const float *W,
+                                                            const float *bias, float *pre,
+                                                            float *vnext, int act)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int row_tiles = n >> 4;
+    const int kv = max(0, min(8, k1 - 8 * g));  // valid K of this lane group in Gx / Z
+    bf16x8 ga[2], wb[2];
+    float bc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int rt = wave + 4 * i;
+        if (rt < row_tiles) ga[i] = bf16_row8(Gx + (int64_t)(16 * rt + r16) * k1 + 8 * g, kv);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        wb[j] = bf16_col8(W + (8 * g) * kLayerC + 16 * j + r16, kLayerC, 8);
+        bc[j] = bias[16 * j + r16];
+    }
+    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+        const float *zs = Z + (int64_t)sl * k1 * kLayerC;
+        const float *vs = v + (int64_t)sl * n * kLayerC;
+        float *ps = pre + (int64_t)sl * n * kLayerC;
+        float *ns = vnext + (int64_t)sl * n * kLayerC;
+        bf16x8 zb[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) zb[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
+                                                      kLayerC, kv);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = wave + 4 * i;
+            if (rt >= row_tiles) break;
+            const bf16x8 va = bf16_row8(vs + (16 * rt + r16) * kLayerC + 8 * g, 8);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                f4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    ga[i], zb[j], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, wb[j], acc, 0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int idx = (16 * rt + 4 * g + e) * kLayerC + 16 * j + r16;
+                    const float pv = acc[e] + bc[j];
+                    ps[idx] = pv;
+                    ns[idx] = act ? gelu(pv) : pv;
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) layer_backward_kernel(int n_slices, int n, int k1,
+                                                             const float *dpre, const float *W,
+                                                             const float *Fx, const float *dT1,
+                                                             const float *pre_prev, float *out)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int row_tiles = n >> 4;
+    const int kv = max(0, min(8, k1 - 8 * g));
+    bf16x8 fa[2], wtb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int rt = wave + 4 * i;
+        // A[x][k] = Fx[k][x]
+        if (rt < row_tiles) fa[i] = bf16_col8(Fx + (int64_t)(8 * g) * n + 16 * rt + r16, n, kv);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)  // B[k][c] = W[c][k]
+        wtb[j] = bf16_row8(W + (16 * j + r16) * kLayerC + 8 * g, 8);
+    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+        const float *ts = dT1 + (int64_t)sl * k1 * kLayerC;
+        const float *ds = dpre + (int64_t)sl * n * kLayerC;
+        const float *pp = pre_prev != nullptr ? pre_prev + (int64_t)sl * n * kLayerC : nullptr;
+        float *os = out + (int64_t)sl * n * kLayerC;
+        bf16x8 tb[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) tb[j] = bf16_col8(ts + (8 * g) * kLayerC + 16 * j + r16,
+                                                      kLayerC, kv);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = wave + 4 * i;
+            if (rt >= row_tiles) break;
+            const bf16x8 da = bf16_row8(ds + (16 * rt + r16) * kLayerC + 8 * g, 8);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                f4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    da, wtb[j], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], tb[j], acc, 0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int idx = (16 * rt + 4 * g + e) * kLayerC + 16 * j + r16;
+                    os[idx] = pp != nullptr ? acc[e] * gelu_grad(pp[idx]) : acc[e];
+                }
+            }
+        }
+    }
+}
+
 // Projection head of the operator, fused: for each pixel row,
 //   hpre = v Q1 + bq1 (H values), out = GELU(hpre) . Q2 + bq2,
 // with v (rows x kHeadC) and the H = kHeadH hidden values never leaving the
@@ -388,9 +538,6 @@ __device__ __forceinline__ float gelu_grad(float x)
 // 16 lanes that hold a row's columns.
 constexpr int kHeadC = 32, kHeadH = 128;
 constexpr int kHeadTiles = kHeadH / 16;
-
-__device__ __forceinline__ float gelu(float x);
-__device__ __forceinline__ float gelu_grad(float x);
 
 __global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const float *v,
                                                            const float *Q1, const float *bq1,
@@ -1024,6 +1171,51 @@ int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t
     const int MN = M * N, J = MN + (p.bias ? N : 0);
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((J + 15) / 16)), dim3(256), 0, st,
                        n_part, MN, J, (const float *)workspace, dW, db);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
+bool layer_shape_ok(int32_t slices, int32_t n, int32_t modes, int32_t ch)
+{
+    return slices >= 1 && ch == kLayerC && n >= 16 && n <= kLayerMaxN && n % 16 == 0 &&
+           modes >= 2 && modes % 2 == 0 && 2 * modes <= 32;
+}
+
+int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *Gx,
+                       const float *Z, const float *v, const float *W, const float *bias,
+                       float *pre, float *vnext, int32_t gelu_act, void *stream)
+{
+    if (!layer_shape_ok(slices, n, modes, ch))
+        return fail(SWNO_EINVAL, "layer_forward: needs 32 channels, n a multiple of 16 up to "
+                                 "128 and an even number of modes up to 16");
+    if (Gx == nullptr || Z == nullptr || v == nullptr || W == nullptr || bias == nullptr ||
+        pre == nullptr || vnext == nullptr)
+        return fail(SWNO_EINVAL, "layer_forward: operand is NULL");
+    if (reinterpret_cast<uintptr_t>(v) % 16 != 0 || reinterpret_cast<uintptr_t>(Gx) % 16 != 0)
+        return fail(SWNO_EINVAL, "layer_forward: v and Gx must be 16-byte aligned");
+    const unsigned grid = (unsigned)std::min(slices, 8192);
+    hipLaunchKernelGGL(layer_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
+                       n, 2 * modes, Gx, Z, v, W, bias, pre, vnext, gelu_act);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
+int swno_layer_backward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *dpre,
+                        const float *W, const float *Fx, const float *dT1,
+                        const float *pre_prev, float *out, void *stream)
+{
+    if (!layer_shape_ok(slices, n, modes, ch))
+        return fail(SWNO_EINVAL, "layer_backward: needs 32 channels, n a multiple of 16 up to "
+                                 "128 and an even number of modes up to 16");
+    if (dpre == nullptr || W == nullptr || Fx == nullptr || dT1 == nullptr || out == nullptr)
+        return fail(SWNO_EINVAL, "layer_backward: operand is NULL");
+    if (out == dpre || (pre_prev != nullptr && out == pre_prev))
+        return fail(SWNO_EINVAL, "layer_backward: out must not alias dpre or pre_prev");
+    if (reinterpret_cast<uintptr_t>(dpre) % 16 != 0 || reinterpret_cast<uintptr_t>(W) % 16 != 0)
+        return fail(SWNO_EINVAL, "layer_backward: dpre and W must be 16-byte aligned");
+    const unsigned grid = (unsigned)std::min(slices, 8192);
+    hipLaunchKernelGGL(layer_backward_kernel, dim3(grid), dim3(256), 0, as_stream(stream),
+                       slices, n, 2 * modes, dpre, W, Fx, dT1, pre_prev, out);
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }

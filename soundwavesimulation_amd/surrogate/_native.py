@@ -29,7 +29,8 @@ class GemmArgs(ctypes.Structure):
 EXPORTS = ("swno_last_error", "swno_gemm_workspace_floats", "swno_gemm", "swno_bias_act",
            "swno_gelu_backward", "swno_col_sum_workspace_floats", "swno_col_sum",
            "swno_weight_grad_workspace_floats", "swno_weight_grad", "swno_head_forward",
-           "swno_head_backward_workspace_floats", "swno_head_backward",
+           "swno_head_backward_workspace_floats", "swno_head_backward", "swno_layer_forward",
+           "swno_layer_backward",
            "swno_spectral_pack", "swno_mix_assemble", "swno_mix_assemble_backward",
            "swno_mse_workspace_floats", "swno_mse", "swno_adamw")
 
@@ -63,6 +64,8 @@ def load_library():
     lib.swno_head_backward_workspace_floats.restype = ctypes.c_size_t
     lib.swno_head_backward_workspace_floats.argtypes = [_i64]
     lib.swno_head_backward.argtypes = [_i64, _i32, _i32] + [_vp] * 12
+    lib.swno_layer_forward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7 + [_i32, _vp]
+    lib.swno_layer_backward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7
     lib.swno_spectral_pack.argtypes = [_i32, _i32, _i32, _i32, _vp, _vp, _vp]
     lib.swno_mix_assemble.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp]
     lib.swno_mix_assemble_backward.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _vp]

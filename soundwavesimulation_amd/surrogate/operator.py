@@ -121,8 +121,12 @@ class SpectralOperator:
                    for _ in range(layers)]
         self.drm = torch.empty_like(self.rm[0])
         # the projection head as one fused kernel each way (built for width 32, hidden 128)
-        self.fused_head = ((width, hidden) == (32, 128)
-                           and os.environ.get("SWNO_FUSED_HEAD", "1") != "0")
+        fuse = os.environ.get("SWNO_FUSED", "1") != "0"
+        self.fused_head = fuse and (width, hidden) == (32, 128)
+        # spectral layer tails (x inverse transform + pointwise linear + bias/GELU, and the
+        # backward counterpart) as one kernel each way
+        self.fused_layers = (fuse and width == 32 and n % 16 == 0 and n <= 128
+                             and modes % 2 == 0 and modes <= 16)
         self._bufs = {}
         self.adam_m = torch.zeros_like(self.flat)
         self.adam_v = torch.zeros_like(self.flat)
@@ -173,7 +177,8 @@ class SpectralOperator:
 
     # ------------------------------------------------------------ spectral
     def _spectral(self, l, v, out, bufs, B):
-        """out = K_l(v) (overwrites out); keeps X for the backward pass."""
+        """out = K_l(v) (overwrites out); keeps X for the backward pass. With out None
+        the last step (the inverse x transform) is left to the caller, from Z."""
         n, C, m = self.n, self.C, self.m
         st = self._stream()
         T1, P, X, Y, U, Z = bufs["T1"], bufs["P"], bufs["X"][l], bufs["Y"], bufs["U"], bufs["Z"]
@@ -188,11 +193,13 @@ class SpectralOperator:
         _native.check(self.lib.swno_spectral_pack(2, B, m, C, _ptr(Y), _ptr(U), st))
         self.gemm(n, 2 * m * C, 2 * m, B, self.Gy, (2 * m, 1, 0), U,
                   (2 * m * C, 1, 4 * m * m * C), Z, (2 * m * C, 1, n * 2 * m * C))
-        self.gemm(n, C, 2 * m, B * n, self.Gx, (2 * m, 1, 0), Z, (C, 1, 2 * m * C), out,
-                  (C, 1, n * C))
+        if out is not None:
+            self.gemm(n, C, 2 * m, B * n, self.Gx, (2 * m, 1, 0), Z, (C, 1, 2 * m * C), out,
+                      (C, 1, n * C))
 
     def _spectral_backward(self, l, dpre, dv, bufs, B):
-        """dv += K_l^T(dpre); writes the gradients of R_l."""
+        """dv += K_l^T(dpre); writes the gradients of R_l. With dv None the last step
+        (dv += Fx^T dT1) is left to the caller, from T1."""
         n, C, m = self.n, self.C, self.m
         st = self._stream()
         X, Y, U, Z, T1, P = bufs["X"][l], bufs["Y"], bufs["U"], bufs["Z"], bufs["T1"], bufs["P"]
@@ -218,8 +225,9 @@ class SpectralOperator:
         self.gemm(n, 2 * m * C, 2 * m, B, self.Fy, (1, n, 0), U, (2 * m * C, 1, 4 * m * m * C),
                   T1, (2 * m * C, 1, n * 2 * m * C))
         # dv[b,y] += Fx^T dT1[b,y]
-        self.gemm(n, C, 2 * m, B * n, self.Fx, (1, n, 0), T1, (C, 1, 2 * m * C), dv,
-                  (C, 1, n * C), beta=1.0)
+        if dv is not None:
+            self.gemm(n, C, 2 * m, B * n, self.Fx, (1, n, 0), T1, (C, 1, 2 * m * C), dv,
+                      (C, 1, n * C), beta=1.0)
 
     # --------------------------------------------------------------- public
     def forward(self, inp):
@@ -243,6 +251,13 @@ class SpectralOperator:
                                         _ptr(v[0]), 0, st))
         for l in range(self.L):
             pre = bufs["pre"][l]
+            if self.fused_layers:
+                self._spectral(l, v[l], None, bufs, B)
+                _native.check(lib.swno_layer_forward(
+                    B * n, n, self.m, C, _ptr(self.Gx), _ptr(bufs["Z"]), _ptr(v[l]),
+                    _ptr(self.p[f"W{l}"]), _ptr(self.p[f"b{l}"]), _ptr(pre), _ptr(v[l + 1]),
+                    int(l < self.L - 1), st))
+                continue
             self._spectral(l, v[l], pre, bufs, B)
             self.gemm(rows, C, C, 1, v[l], (C, 1, 0), self.p[f"W{l}"], (C, 1, 0), pre, (C, 1, 0),
                       beta=1.0)
@@ -309,6 +324,20 @@ class SpectralOperator:
             self.gemm(rows, C, H, 1, bufs["dh"], (H, 1, 0), self.p["Q1"], (1, H, 0), dv,
                       (C, 1, 0))
         # spectral layers, last to first
+        if self.fused_layers:
+            # dpre of the last layer is dv (no GELU after it); each fused kernel turns a
+            # layer's dpre into the previous layer's dpre (or, for layer 0, into dv)
+            cur, spare = dv, dpre
+            for l in range(self.L - 1, -1, -1):
+                self.weight_grad(rows, bufs["v"][l], C, cur, C, g[f"W{l}"], g[f"b{l}"], wgws)
+                self._spectral_backward(l, cur, None, bufs, B)
+                prev = bufs["pre"][l - 1] if l > 0 else None
+                _native.check(lib.swno_layer_backward(
+                    B * n, n, self.m, C, _ptr(cur), _ptr(self.p[f"W{l}"]), _ptr(self.Fx),
+                    _ptr(bufs["T1"]), _ptr(prev), _ptr(spare), st))
+                cur, spare = spare, cur
+            self.weight_grad(rows, bufs["inp"], self.Cin, cur, C, g["P"], g["bp"], wgws)
+            return
         for l in range(self.L - 1, -1, -1):
             v, pre = bufs["v"][l], bufs["pre"][l]
             if l < self.L - 1:

@@ -214,9 +214,10 @@ def test_operator_forward_and_gradients_match_oracle(C, H):
         assert e <= 3e-2, k
 
 
-def test_fused_head_matches_separate_kernels():
+@pytest.mark.parametrize("switch", ["fused_head", "fused_layers"])
+def test_fused_kernels_match_separate_kernels(switch):
     n, m, C, H, B = 32, 8, 32, 128, 5
-    print("fused head vs GEMM + bias/GELU + GEMM", flush=True)
+    print(f"{switch}: fused vs separate GEMM / elementwise kernels", flush=True)
     rng = np.random.default_rng(3)
     params = init_params(seed=4, width=C, modes=m, hidden=H)
     params["bq1"] = rng.standard_normal(H) * 0.3
@@ -226,7 +227,8 @@ def test_fused_head_matches_separate_kernels():
     res = []
     for fused in (True, False):
         op = SpectralOperator(params, n, DEV, width=C, modes=m, hidden=H)
-        op.fused_head = fused
+        assert op.fused_head and op.fused_layers
+        setattr(op, switch, fused)
         out = op.forward(inp).clone()
         op.loss_and_backward(target)
         torch.cuda.synchronize()
