@@ -78,6 +78,15 @@ int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t
                      const float *B, int64_t ldb, float *dW, float *db, float *workspace,
                      void *stream);
 
+/* Truncated transform along x over `slices` slices (one sample row each:
+ * n pixels x ch channels, channel-last; ch = 32, n <= 128):
+ * out[s] (rows_out x ch) = A X[s], with A (rows_out x n, rows_out <= 32)
+ * given by element strides a_rs, a_cs (the forward transform Fx, or Gx^T in
+ * the backward pass). bf16 MFMA with fp32 accumulation; replaces the batched
+ * GEMM for this shape. */
+int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, const float *A,
+                     int64_t a_rs, int64_t a_cs, const float *X, float *out, void *stream);
+
 /* Tail of a spectral layer, fused, over `slices` slices (one sample row
  * each: n pixels x ch channels, channel-last; ch = 32, n a multiple of 16 up
  * to 128, an even number of modes up to 16):

@@ -427,6 +427,83 @@ __device__ __forceinline__ bf16x8 bf16_col8(const float *p, int64_t stride, int 
     return r;
 }
 
+// Truncated transform along x, per slice: out[s] (M x 32) = A (M x n) X[s]
+// (n x 32), for M = 2 modes <= 32 and n <= 128 (A = Fx forward, or Gx^T in
+// the backward pass, given by its row and column strides). A is held in
+// registers as bf16 MFMA operands for the kernel's lifetime. A slice's X is
+// loaded with coalesced float4 loads (the next slice's while the current one
+// is multiplied), rounded to bf16 into LDS transposed (channel-major, the row
+// index contiguous) in one of two buffers, so one barrier per slice suffices;
+// wavefront w then computes the output tile (w / 2, w % 2).
+constexpr int kXtItems = kLayerMaxN * kLayerC / 4 / 256;  // float4 per thread and slice
+
+__global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, int n,
+                                                          const float *A, int64_t a_rs,
+                                                          int64_t a_cs, const float *X,
+                                                          float *out)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][kLayerC][kLayerMaxN + 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int tm = wave >> 1, tn = wave & 1;
+    const bool active = 16 * tm < M;  // M <= 16: two wavefronts only load and wait
+    constexpr int kSteps = kLayerMaxN / 32;
+    const int steps = (n + 31) / 32;
+    const int am = 16 * tm + r16;
+    bf16x8 a[kSteps];
+#pragma unroll
+    for (int kk = 0; kk < kSteps; ++kk)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 32 * kk + 8 * g + j;
+            a[kk][j] = (__bf16)(am < M && k < n ? A[am * a_rs + k * a_cs] : 0.0f);
+        }
+    // rows k >= n of both buffers stay zero (they pad the last 32-row step)
+    for (int i = tid; i < 2 * kLayerC * (kLayerMaxN + 8); i += 256) (&Bs[0][0][0])[i] = (__bf16)0.0f;
+    const int count = n * (kLayerC / 4);
+    f4 rx[kXtItems];
+    auto load = [&](int sl) {
+        const float *xs = X + (int64_t)sl * n * kLayerC;
+#pragma unroll
+        for (int i = 0; i < kXtItems; ++i) {
+            const int e4 = tid + 256 * i;
+            rx[i] = e4 < count ? *(const f4 *)(xs + 4 * e4) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    };
+    int sl = blockIdx.x;
+    if (sl < n_slices) load(sl);
+    __syncthreads();
+    const int col = 16 * tn + r16;
+    for (int buf = 0; sl < n_slices; sl += gridDim.x, buf ^= 1) {
+#pragma unroll
+        for (int i = 0; i < kXtItems; ++i) {
+            const int e4 = tid + 256 * i;
+            if (e4 < count) {
+                const int k = e4 >> 3, c4 = (e4 & 7) * 4;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Bs[buf][c4 + j][k] = (__bf16)rx[i][j];
+            }
+        }
+        __syncthreads();
+        if (sl + (int)gridDim.x < n_slices) load(sl + gridDim.x);
+        if (active) {
+            f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < kSteps; ++kk)
+                if (kk < steps) {
+                    const bf16x8 b = *(const bf16x8 *)&Bs[buf][col][32 * kk + 8 * g];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk], b, acc, 0, 0, 0);
+                }
+            float *os = out + (int64_t)sl * M * kLayerC;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = 16 * tm + 4 * g + e;
+                if (row < M) os[row * kLayerC + col] = acc[e];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n, int k1,
                                                             const float *Gx, const float *Z,
                                                             const float *v, const float *W,
@@ -449,25 +526,34 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n,
         wb[j] = bf16_col8(W + (8 * g) * kLayerC + 16 * j + r16, kLayerC, 8);
         bc[j] = bias[16 * j + r16];
     }
-    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+    // a slice's operands: the next slice's are loaded while the current one is computed
+    bf16x8 zb[2], va[2], zn[2], vn[2];
+    auto load = [&](int sl, bf16x8 (&z)[2], bf16x8 (&x)[2]) {
         const float *zs = Z + (int64_t)sl * k1 * kLayerC;
         const float *vs = v + (int64_t)sl * n * kLayerC;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) z[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
+                                                     kLayerC, kv);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = wave + 4 * i;
+            if (rt < row_tiles) x[i] = bf16_row8(vs + (16 * rt + r16) * kLayerC + 8 * g, 8);
+        }
+    };
+    if (blockIdx.x < n_slices) load(blockIdx.x, zb, va);
+    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+        if (sl + (int)gridDim.x < n_slices) load(sl + gridDim.x, zn, vn);
         float *ps = pre + (int64_t)sl * n * kLayerC;
         float *ns = vnext + (int64_t)sl * n * kLayerC;
-        bf16x8 zb[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) zb[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
-                                                      kLayerC, kv);
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int rt = wave + 4 * i;
             if (rt >= row_tiles) break;
-            const bf16x8 va = bf16_row8(vs + (16 * rt + r16) * kLayerC + 8 * g, 8);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 f4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                     ga[i], zb[j], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, wb[j], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[i], wb[j], acc, 0, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int idx = (16 * rt + 4 * g + e) * kLayerC + 16 * j + r16;
@@ -476,6 +562,11 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n,
                     ns[idx] = act ? gelu(pv) : pv;
                 }
             }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            zb[j] = zn[j];
+            va[j] = vn[j];
         }
     }
 }
@@ -499,31 +590,62 @@ __global__ void __launch_bounds__(256) layer_backward_kernel(int n_slices, int n
 #pragma unroll
     for (int j = 0; j < 2; ++j)  // B[k][c] = W[c][k]
         wtb[j] = bf16_row8(W + (16 * j + r16) * kLayerC + 8 * g, 8);
-    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+    // a slice's operands, loaded before any store of the slice before it (out may
+    // not alias the inputs, but the compiler cannot know that): the next slice's are
+    // in flight while the current one is computed
+    bf16x8 tb[2], da[2], tn[2], dn[2];
+    float pv[2][2][4], pn[2][2][4];
+    auto load = [&](int sl, bf16x8 (&t)[2], bf16x8 (&d)[2], float (&q)[2][2][4]) {
         const float *ts = dT1 + (int64_t)sl * k1 * kLayerC;
         const float *ds = dpre + (int64_t)sl * n * kLayerC;
         const float *pp = pre_prev != nullptr ? pre_prev + (int64_t)sl * n * kLayerC : nullptr;
-        float *os = out + (int64_t)sl * n * kLayerC;
-        bf16x8 tb[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) tb[j] = bf16_col8(ts + (8 * g) * kLayerC + 16 * j + r16,
-                                                      kLayerC, kv);
+        for (int j = 0; j < 2; ++j) t[j] = bf16_col8(ts + (8 * g) * kLayerC + 16 * j + r16,
+                                                     kLayerC, kv);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int rt = wave + 4 * i;
+            if (rt < row_tiles) {
+                d[i] = bf16_row8(ds + (16 * rt + r16) * kLayerC + 8 * g, 8);
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        q[i][j][e] =
+                            pp != nullptr ? pp[(16 * rt + 4 * g + e) * kLayerC + 16 * j + r16]
+                                          : 0.0f;
+            }
+        }
+    };
+    if (blockIdx.x < n_slices) load(blockIdx.x, tb, da, pv);
+    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
+        if (sl + (int)gridDim.x < n_slices) load(sl + gridDim.x, tn, dn, pn);
+        const bool gp = pre_prev != nullptr;
+        float *os = out + (int64_t)sl * n * kLayerC;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int rt = wave + 4 * i;
             if (rt >= row_tiles) break;
-            const bf16x8 da = bf16_row8(ds + (16 * rt + r16) * kLayerC + 8 * g, 8);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 f4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    da, wtb[j], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                    da[i], wtb[j], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], tb[j], acc, 0, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int idx = (16 * rt + 4 * g + e) * kLayerC + 16 * j + r16;
-                    os[idx] = pp != nullptr ? acc[e] * gelu_grad(pp[idx]) : acc[e];
+                    os[idx] = gp ? acc[e] * gelu_grad(pv[i][j][e]) : acc[e];
                 }
             }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            tb[i] = tn[i];
+            da[i] = dn[i];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) pv[i][j][e] = pn[i][j][e];
         }
     }
 }
@@ -606,27 +728,56 @@ __global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const f
 // v^T once; dv and dQ1 are MFMA products, the rest fp32.
 constexpr int kHeadJ = kHeadC * kHeadH + 2 * kHeadH + 1;
 
-__global__ void __launch_bounds__(256) head_backward_kernel(int64_t rows, int64_t rows_per_part,
-                                                            const float *dout, const float *hpre,
-                                                            const float *v, const float *Q1,
-                                                            const float *Q2, float *dv,
-                                                            float *ws)
+// One 64-row block's inputs of head_backward_kernel, as one wavefront's lane
+// holds them: hpre at rows 16 w + 4 g + e and columns 16 t + r16, dout at
+// rows 16 w + 4 g + e, and two float4 of the block's v (for v^T in LDS).
+struct HeadBlock {
+    float hp[kHeadTiles][4];
+    float d[4];
+    f4 vx[2];
+};
+
+__device__ __forceinline__ void head_load(HeadBlock &b, int64_t blk0, int64_t r_end,
+                                          const float *dout, const float *hpre, const float *v,
+                                          int tid)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 DH[64][kHeadH + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 DHT[kHeadH][64 + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 VT[kHeadC][64 + 8];
-    __shared__ float red[16][2 * kHeadH + 1];
+    const int lane = tid & 63, wave = tid >> 6, r16 = lane & 15, g = lane >> 4;
+    const int64_t row0 = blk0 + 16 * wave + 4 * g;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool in = row0 + e < r_end;
+        b.d[e] = in ? dout[row0 + e] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < kHeadTiles; ++t)
+            b.hp[t][e] = in ? hpre[(row0 + e) * kHeadH + 16 * t + r16] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int e4 = tid + 256 * i;
+        const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
+        b.vx[i] = blk0 + rr < r_end ? *(const f4 *)(v + (blk0 + rr) * kHeadC + c4)
+                                    : f4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+}
+
+// two workgroups per CU: at most 256 VGPRs, LDS below half of the CU's
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+head_backward_kernel(int64_t rows, int64_t rows_per_part, const float *dout, const float *hpre,
+                     const float *v, const float *Q1, const float *Q2, float *dv, float *ws)
+{
+    constexpr int kDH = 64 * (kHeadH + 8) * 2, kDHT = kHeadH * (64 + 8) * 2;
+    constexpr int kVT = kHeadC * (64 + 8) * 2, kQS = kHeadC * (kHeadH + 8) * 2;
+    static_assert(16 * (2 * kHeadH + 1) * 4 <= kDH + kDHT, "red overlays DH and DHT");
+    __shared__ __attribute__((aligned(16))) char smem[kDH + kDHT + kVT + kQS];
+    auto DH = reinterpret_cast<__bf16 (*)[kHeadH + 8]>(smem);
+    auto DHT = reinterpret_cast<__bf16 (*)[64 + 8]>(smem + kDH);
+    auto VT = reinterpret_cast<__bf16 (*)[64 + 8]>(smem + kDH + kDHT);
+    auto QS = reinterpret_cast<__bf16 (*)[kHeadH + 8]>(smem + kDH + kDHT + kVT);  // Q1, bf16
+    auto red = reinterpret_cast<float (*)[2 * kHeadH + 1]>(smem);  // after the last block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, g = lane >> 4;
-    // B operand of dv = dh Q1^T: B[k = h][n = c] = Q1[c][h]
-    bf16x8 q1t[kHeadC / 16][kHeadH / 32];
-#pragma unroll
-    for (int t = 0; t < kHeadC / 16; ++t)
-#pragma unroll
-        for (int kk = 0; kk < kHeadH / 32; ++kk)
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                q1t[t][kk][j] = (__bf16)Q1[(16 * t + r16) * kHeadH + 32 * kk + 8 * g + j];
+    for (int i = tid; i < kHeadC * kHeadH; i += 256)
+        QS[i / kHeadH][i % kHeadH] = (__bf16)Q1[i];
     float q2[kHeadTiles];
 #pragma unroll
     for (int t = 0; t < kHeadTiles; ++t) q2[t] = Q2[16 * t + r16];
@@ -641,49 +792,46 @@ __global__ void __launch_bounds__(256) head_backward_kernel(int64_t rows, int64_
     float gb2 = 0.0f;
     const int64_t r_begin = (int64_t)blockIdx.x * rows_per_part;
     const int64_t r_end = min(rows, r_begin + rows_per_part);
+    HeadBlock cur, nxt;
+    head_load(cur, r_begin, r_end, dout, hpre, v, tid);
     for (int64_t blk0 = r_begin; blk0 < r_end; blk0 += 64) {
-        const int64_t row0 = blk0 + 16 * wave;
-        float d[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t row = row0 + 4 * g + e;
-            d[e] = row < r_end ? dout[row] : 0.0f;
-        }
-        if (r16 == 0) gb2 += (d[0] + d[1]) + (d[2] + d[3]);
+        // the next block's loads are in flight while this block is computed
+        if (blk0 + 64 < r_end) head_load(nxt, blk0 + 64, r_end, dout, hpre, v, tid);
+        if (r16 == 0) gb2 += (cur.d[0] + cur.d[1]) + (cur.d[2] + cur.d[3]);
 #pragma unroll
         for (int t = 0; t < kHeadTiles; ++t) {
             const int col = 16 * t + r16;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t row = row0 + 4 * g + e;
-                const float hp = row < r_end ? hpre[row * kHeadH + col] : 0.0f;
-                gq2[t] += gelu(hp) * d[e];
-                const float dh = d[e] * q2[t] * gelu_grad(hp);
+                const float x = cur.hp[t][e];
+                const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+                const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+                gq2[t] += x * cdf * cur.d[e];
+                const float dh = cur.d[e] * q2[t] * (cdf + x * pdf);
                 gb1[t] += dh;
                 const int rl = 16 * wave + 4 * g + e;
                 DH[rl][col] = (__bf16)dh;
                 DHT[col][rl] = (__bf16)dh;
             }
         }
-        // v^T of the block: 64 rows x 32 channels = 512 float4, two per thread
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int e4 = tid + 256 * i;
             const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
-            f4 x = f4{0.0f, 0.0f, 0.0f, 0.0f};
-            if (blk0 + rr < r_end) x = *(const f4 *)(v + (blk0 + rr) * kHeadC + c4);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) VT[c4 + j][rr] = (__bf16)x[j];
+            for (int j = 0; j < 4; ++j) VT[c4 + j][rr] = (__bf16)cur.vx[i][j];
         }
         __syncthreads();
-        // dv for this wavefront's 16 rows
+        const int64_t row0 = blk0 + 16 * wave;
+        // dv for this wavefront's 16 rows: B[k = h][n = c] = Q1[c][h]
 #pragma unroll
         for (int t = 0; t < kHeadC / 16; ++t) {
             f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kk = 0; kk < kHeadH / 32; ++kk) {
                 const bf16x8 a = *(const bf16x8 *)&DH[16 * wave + r16][32 * kk + 8 * g];
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, q1t[t][kk], acc, 0, 0, 0);
+                const bf16x8 b = *(const bf16x8 *)&QS[16 * t + r16][32 * kk + 8 * g];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -704,6 +852,7 @@ __global__ void __launch_bounds__(256) head_backward_kernel(int64_t rows, int64_
             }
         }
         __syncthreads();
+        cur = nxt;
     }
     float *out = ws + (int64_t)blockIdx.x * kHeadJ;
 #pragma unroll
@@ -1181,6 +1330,25 @@ bool layer_shape_ok(int32_t slices, int32_t n, int32_t modes, int32_t ch)
            modes >= 2 && modes % 2 == 0 && 2 * modes <= 32;
 }
 
+int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, const float *A,
+                     int64_t a_rs, int64_t a_cs, const float *X, float *out, void *stream)
+{
+    if (slices < 1 || ch != kLayerC || rows_out < 1 || rows_out > 32 || n < 1 ||
+        n > kLayerMaxN)
+        return fail(SWNO_EINVAL, "x_transform: needs 32 channels, 1 <= rows_out <= 32 and "
+                                 "1 <= n <= 128");
+    if (A == nullptr || X == nullptr || out == nullptr)
+        return fail(SWNO_EINVAL, "x_transform: operand is NULL");
+    if (reinterpret_cast<uintptr_t>(X) % 16 != 0)
+        return fail(SWNO_EINVAL, "x_transform: X must be 16-byte aligned");
+    // a few slices per workgroup, so that the next slice's loads overlap the current one
+    const unsigned grid = (unsigned)std::min(slices, 2048);
+    hipLaunchKernelGGL(x_transform_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
+                       rows_out, n, A, a_rs, a_cs, X, out);
+    SWNO_HIP_CHECK(hipGetLastError());
+    return SWNO_OK;
+}
+
 int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *Gx,
                        const float *Z, const float *v, const float *W, const float *bias,
                        float *pre, float *vnext, int32_t gelu_act, void *stream)
@@ -1193,6 +1361,8 @@ int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, con
         return fail(SWNO_EINVAL, "layer_forward: operand is NULL");
     if (reinterpret_cast<uintptr_t>(v) % 16 != 0 || reinterpret_cast<uintptr_t>(Gx) % 16 != 0)
         return fail(SWNO_EINVAL, "layer_forward: v and Gx must be 16-byte aligned");
+    // one slice per workgroup measured faster here than a few with prefetch (the stores
+    // of pre and vnext dominate)
     const unsigned grid = (unsigned)std::min(slices, 8192);
     hipLaunchKernelGGL(layer_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
                        n, 2 * modes, Gx, Z, v, W, bias, pre, vnext, gelu_act);
@@ -1213,7 +1383,7 @@ int swno_layer_backward(int32_t slices, int32_t n, int32_t modes, int32_t ch, co
         return fail(SWNO_EINVAL, "layer_backward: out must not alias dpre or pre_prev");
     if (reinterpret_cast<uintptr_t>(dpre) % 16 != 0 || reinterpret_cast<uintptr_t>(W) % 16 != 0)
         return fail(SWNO_EINVAL, "layer_backward: dpre and W must be 16-byte aligned");
-    const unsigned grid = (unsigned)std::min(slices, 8192);
+    const unsigned grid = (unsigned)std::min(slices, 2048);  // a few slices each (prefetch)
     hipLaunchKernelGGL(layer_backward_kernel, dim3(grid), dim3(256), 0, as_stream(stream),
                        slices, n, 2 * modes, dpre, W, Fx, dT1, pre_prev, out);
     SWNO_HIP_CHECK(hipGetLastError());

@@ -30,7 +30,7 @@ EXPORTS = ("swno_last_error", "swno_gemm_workspace_floats", "swno_gemm", "swno_b
            "swno_gelu_backward", "swno_col_sum_workspace_floats", "swno_col_sum",
            "swno_weight_grad_workspace_floats", "swno_weight_grad", "swno_head_forward",
            "swno_head_backward_workspace_floats", "swno_head_backward", "swno_layer_forward",
-           "swno_layer_backward",
+           "swno_layer_backward", "swno_x_transform",
            "swno_spectral_pack", "swno_mix_assemble", "swno_mix_assemble_backward",
            "swno_mse_workspace_floats", "swno_mse", "swno_adamw")
 
@@ -66,6 +66,7 @@ def load_library():
     lib.swno_head_backward.argtypes = [_i64, _i32, _i32] + [_vp] * 12
     lib.swno_layer_forward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7 + [_i32, _vp]
     lib.swno_layer_backward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7
+    lib.swno_x_transform.argtypes = [_i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp, _vp]
     lib.swno_spectral_pack.argtypes = [_i32, _i32, _i32, _i32, _vp, _vp, _vp]
     lib.swno_mix_assemble.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp]
     lib.swno_mix_assemble_backward.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _vp]

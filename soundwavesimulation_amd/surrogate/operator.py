@@ -182,8 +182,12 @@ class SpectralOperator:
         n, C, m = self.n, self.C, self.m
         st = self._stream()
         T1, P, X, Y, U, Z = bufs["T1"], bufs["P"], bufs["X"][l], bufs["Y"], bufs["U"], bufs["Z"]
-        self.gemm(2 * m, C, n, B * n, self.Fx, (n, 1, 0), v, (C, 1, n * C), T1,
-                  (C, 1, 2 * m * C))
+        if self.fused_layers:
+            _native.check(self.lib.swno_x_transform(B * n, 2 * m, n, C, _ptr(self.Fx), n, 1,
+                                                    _ptr(v), _ptr(T1), st))
+        else:
+            self.gemm(2 * m, C, n, B * n, self.Fx, (n, 1, 0), v, (C, 1, n * C), T1,
+                      (C, 1, 2 * m * C))
         self.gemm(2 * m, 2 * m * C, n, B, self.Fy, (n, 1, 0), T1, (2 * m * C, 1, n * 2 * m * C),
                   P, (2 * m * C, 1, 4 * m * m * C))
         _native.check(self.lib.swno_spectral_pack(0, B, m, C, _ptr(P), _ptr(X), st))
@@ -204,8 +208,12 @@ class SpectralOperator:
         st = self._stream()
         X, Y, U, Z, T1, P = bufs["X"][l], bufs["Y"], bufs["U"], bufs["Z"], bufs["T1"], bufs["P"]
         # dZ[b,y] = Gx^T dpre[b,y]
-        self.gemm(2 * m, C, n, B * n, self.Gx, (1, 2 * m, 0), dpre, (C, 1, n * C), Z,
-                  (C, 1, 2 * m * C))
+        if self.fused_layers:
+            _native.check(self.lib.swno_x_transform(B * n, 2 * m, n, C, _ptr(self.Gx), 1, 2 * m,
+                                                    _ptr(dpre), _ptr(Z), st))
+        else:
+            self.gemm(2 * m, C, n, B * n, self.Gx, (1, 2 * m, 0), dpre, (C, 1, n * C), Z,
+                      (C, 1, 2 * m * C))
         # dU[b] = Gy^T dZ[b]
         self.gemm(2 * m, 2 * m * C, n, B, self.Gy, (1, 2 * m, 0), Z,
                   (2 * m * C, 1, n * 2 * m * C), U, (2 * m * C, 1, 4 * m * m * C))

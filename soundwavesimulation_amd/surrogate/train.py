@@ -148,6 +148,32 @@ def fwi_runs(geometry, gen, c_true, c_pred, device, iterations, target_rmse):
     return out
 
 
+def time_steps(n, batch, steps, device, seed=0, warmup=5):
+    """Training steps (forward, masked MSE backward, AdamW) on random data, timed."""
+    op = SpectralOperator(init_params(seed=seed), n, device)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = torch.randn((batch, n, n, 3), generator=g).to(device)
+    y = torch.randn((batch, n, n), generator=g).to(device)
+    for _ in range(warmup):
+        op.forward(x)
+        op.loss_and_backward(y)
+        op.adamw_step(1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.forward(x)
+        op.loss_and_backward(y)
+        op.adamw_step(1e-3)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    flops = step_flops(batch, n)
+    return {"workload": f"operator_training_step_{n}x{n}_batch{batch}", "steps": steps,
+            "ms": round(t * 1e3, 3), "gflop": round(flops / 1e9, 2),
+            "tflops": round(flops / t / 1e12, 2),
+            "mfma_frac_of_bf16_peak": round(flops / t / 1e12 / BF16_PEAK_TFLOPS, 5),
+            "fused": bool(op.fused_head and op.fused_layers)}
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--samples", type=int, default=4096)
@@ -162,9 +188,14 @@ def main(argv=None):
     p.add_argument("--target-rmse", type=float, default=10.0)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--out", default=None)
+    p.add_argument("--step-only", type=int, default=0, metavar="K",
+                   help="only time K training steps on random data (no FDTD pairs)")
     a = p.parse_args(argv)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if a.step_only:
+        print(json.dumps(time_steps(a.grid, a.batch, a.step_only, dev, seed=a.seed)), flush=True)
+        return
     geo = Geometry(n=a.grid, n_steps=a.steps)
     log(f"generating {a.samples} pairs: {a.grid}^2, {geo.n_elements} shots, {a.steps} steps")
     inputs, targets, t_gen, gen = generate(geo, dev, a.samples, seed0=a.seed * 1_000_000, log=log)

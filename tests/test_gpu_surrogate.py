@@ -103,6 +103,25 @@ def test_weight_grad_matches_bf16_rounded_product(rows, M, N, bias, offset):
         assert torch.isnan(db).all()  # untouched
 
 
+@pytest.mark.parametrize("slices,M,n,transposed", [(300, 32, 128, False), (300, 32, 128, True),
+                                                   (77, 16, 32, False), (5, 20, 48, True)])
+def test_x_transform_matches_bf16_rounded_product(slices, M, n, transposed):
+    print(f"x_transform slices={slices} M={M} n={n} transposed={transposed}", flush=True)
+    lib = _native.load_library()
+    g = torch.Generator().manual_seed(slices + M + n)
+    A = torch.randn((n, M) if transposed else (M, n), generator=g).to(DEV)
+    X = torch.randn((slices, n, 32), generator=g).to(DEV)
+    out = torch.full((slices, M, 32), float("nan"), device=DEV)
+    a_rs, a_cs = (1, M) if transposed else (n, 1)
+    _native.check(lib.swno_x_transform(slices, M, n, 32, A.data_ptr(), a_rs, a_cs, X.data_ptr(),
+                                       out.data_ptr(), torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    Am = A.T if transposed else A
+    ref = torch.matmul(_bf16(Am.cpu()), _bf16(X.cpu()))
+    err = float((out.cpu().double() - ref).norm() / ref.norm())
+    assert err <= 1e-5, err
+
+
 def test_weight_grad_rejects_bad_shapes():
     lib = _native.load_library()
     x = torch.zeros(1024, device=DEV)
