@@ -104,7 +104,8 @@ def test_weight_grad_matches_bf16_rounded_product(rows, M, N, bias, offset):
 
 
 @pytest.mark.parametrize("slices,M,n,transposed", [(300, 32, 128, False), (300, 32, 128, True),
-                                                   (77, 16, 32, False), (5, 20, 48, True)])
+                                                   (77, 16, 32, False), (5, 20, 48, True),
+                                                   (5000, 32, 32, False)])  # several per workgroup
 def test_x_transform_matches_bf16_rounded_product(slices, M, n, transposed):
     print(f"x_transform slices={slices} M={M} n={n} transposed={transposed}", flush=True)
     lib = _native.load_library()
@@ -233,10 +234,11 @@ def test_operator_forward_and_gradients_match_oracle(C, H):
         assert e <= 3e-2, k
 
 
-@pytest.mark.parametrize("switch", ["fused_head", "fused_layers"])
-def test_fused_kernels_match_separate_kernels(switch):
-    n, m, C, H, B = 32, 8, 32, 128, 5
-    print(f"{switch}: fused vs separate GEMM / elementwise kernels", flush=True)
+@pytest.mark.parametrize("switch,B", [("fused_head", 5), ("fused_layers", 5),
+                                      ("fused_layers", 300)])  # 9600 slices: several per workgroup
+def test_fused_kernels_match_separate_kernels(switch, B):
+    n, m, C, H = 32, 8, 32, 128
+    print(f"{switch}: fused vs separate GEMM / elementwise kernels, batch {B}", flush=True)
     rng = np.random.default_rng(3)
     params = init_params(seed=4, width=C, modes=m, hidden=H)
     params["bq1"] = rng.standard_normal(H) * 0.3
