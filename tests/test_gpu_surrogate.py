@@ -123,6 +123,50 @@ def test_x_transform_matches_bf16_rounded_product(slices, M, n, transposed):
     assert err <= 1e-5, err
 
 
+@pytest.mark.parametrize("rows", [1000, 70001])  # a partial 64-row block; several per part
+def test_head_kernels_match_float64(rows):
+    print(f"head forward / backward rows={rows}", flush=True)
+    lib = _native.load_library()
+    st = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator().manual_seed(rows)
+    C, H = 32, 128
+    v = torch.randn((rows, C), generator=g).to(DEV)
+    Q1 = (torch.randn((C, H), generator=g) * 0.3).to(DEV)
+    bq1 = (torch.randn(H, generator=g) * 0.3).to(DEV)
+    Q2 = (torch.randn(H, generator=g) * 0.2).to(DEV)
+    bq2 = torch.randn(1, generator=g).to(DEV)
+    dout = torch.randn(rows, generator=g).to(DEV)
+    hpre = torch.empty((rows, H), device=DEV)
+    out = torch.empty(rows, device=DEV)
+    _native.check(lib.swno_head_forward(rows, C, H, v.data_ptr(), Q1.data_ptr(), bq1.data_ptr(),
+                                        Q2.data_ptr(), bq2.data_ptr(), hpre.data_ptr(),
+                                        out.data_ptr(), st))
+    dv = torch.empty((rows, C), device=DEV)
+    dQ1, dbq1 = torch.empty((C, H), device=DEV), torch.empty(H, device=DEV)
+    dQ2, dbq2 = torch.empty(H, device=DEV), torch.empty(1, device=DEV)
+    ws = torch.empty(int(lib.swno_head_backward_workspace_floats(rows)), device=DEV)
+    _native.check(lib.swno_head_backward(rows, C, H, dout.data_ptr(), hpre.data_ptr(),
+                                         v.data_ptr(), Q1.data_ptr(), Q2.data_ptr(),
+                                         dv.data_ptr(), dQ1.data_ptr(), dbq1.data_ptr(),
+                                         dQ2.data_ptr(), dbq2.data_ptr(), ws.data_ptr(), st))
+    torch.cuda.synchronize()
+    Q264 = Q2.cpu().double()
+    hp64 = _bf16(v.cpu()) @ _bf16(Q1.cpu()) + bq1.cpu().double()
+    assert _rel(hpre.cpu().double().numpy(), hp64.numpy()) <= 1e-5
+    hp = hpre.cpu().double().numpy()  # the backward reads the kernel's own hpre
+    out64 = so.gelu(hp) @ Q264.numpy() + float(bq2)
+    assert _rel(out.cpu().double().numpy(), out64) <= 1e-5
+    d = dout.cpu().double().numpy()
+    dh = d[:, None] * Q264.numpy()[None, :] * so.gelu_grad(hp)
+    dh_b = _bf16(torch.as_tensor(dh, dtype=torch.float32)).numpy()
+    assert _rel(dv.cpu().double().numpy(), dh_b @ _bf16(Q1.cpu()).numpy().T) <= 1e-3
+    assert _rel(dQ1.cpu().double().numpy(), _bf16(v.cpu()).numpy().T @ dh_b) <= 1e-3
+    assert _rel(dbq1.cpu().double().numpy(), dh.sum(0)) <= 1e-5
+    assert _rel(dQ2.cpu().double().numpy(), so.gelu(hp).T @ d) <= 1e-5
+    assert abs(float(dbq2) - d.sum()) <= 1e-5 * np.abs(d).sum()
+    assert not torch.isnan(dv).any()
+
+
 def test_weight_grad_rejects_bad_shapes():
     lib = _native.load_library()
     x = torch.zeros(1024, device=DEV)
