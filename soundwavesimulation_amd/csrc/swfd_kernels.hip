@@ -641,6 +641,17 @@ __device__ __forceinline__ int batch_entry(const int *first, int n, int b)
 // sum is the single-step kernel's: a = 0, a += res[rec_k] * w_k in entry order.
 typedef const int __attribute__((address_space(4))) kci;    // read-only, scalar-loadable
 typedef const float __attribute__((address_space(4))) kcf;
+typedef const StepParams __attribute__((address_space(4))) kparams;
+
+// Entry e of a batched launch's parameter table, read through the constant
+// address space: the table does not change while the kernel runs, so its
+// (wavefront-uniform) fields load into scalar registers like kernel arguments
+// instead of occupying vector registers.
+__device__ __forceinline__ const StepParams &batch_params(const StepParams *table, int e)
+{
+    const kparams *t = (const kparams *)table;
+    return *(const StepParams *)(t + e);
+}
 
 __device__ __forceinline__ float inject_lanes(const StepParams &p, const float *res,
                                               size_t res_off, int e, int lane, float v)
@@ -1232,7 +1243,7 @@ fwdm_batch_kernel(const StepParams *table, const int *first, int n)
     __shared__ EdgeRow ex[2];
     const int b = (int)blockIdx.x;
     const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
-    fwdm_block<KL, SM, F>(table[e], b - first[e], ex);
+    fwdm_block<KL, SM, F>(batch_params(table, e), b - first[e], ex);
 }
 
 // Workgroup b of a two-step forward launch of one problem.
@@ -1282,7 +1293,7 @@ __global__ void __launch_bounds__(256) fwd2_batch_kernel(const StepParams *table
 {
     const int b = (int)blockIdx.x;
     const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
-    fwd2_block<F>(table[e], b - first[e]);
+    fwd2_block<F>(batch_params(table, e), b - first[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1672,7 +1683,7 @@ __global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table
 {
     const int b = (int)blockIdx.x;
     const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
-    adj2_block<F>(table[e], b - first[e]);
+    adj2_block<F>(batch_params(table, e), b - first[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1998,7 +2009,7 @@ __global__ void __launch_bounds__(1024) adjm_batch_kernel(const StepParams *tabl
     __shared__ AdjEdgeRow ex[2];
     const int b = (int)blockIdx.x;
     const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
-    adjm_block<KL, IMG>(table[e], b - first[e], ex);
+    adjm_block<KL, IMG>(batch_params(table, e), b - first[e], ex);
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
