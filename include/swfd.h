@@ -237,10 +237,13 @@ int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *
 /* Batched runs (no counterpart in the reference, whose batching stops at the
  * shots of one problem, torch_backend.py:601-770): n independent problems,
  * each with its own grid, medium, receivers and step range, advance in
- * lockstep, and at every tick the two-step launches of all problems that
- * have one go out as ONE batched kernel launch (a per-problem parameter
- * table; odd trailing steps run as single-problem launches). Results equal
- * n separate swfd_forward / swfd_adjoint calls bit for bit.
+ * lockstep, and at every tick the multi-step launches of all problems that
+ * have one go out as one batched kernel launch per kernel variant and row
+ * width (a per-problem parameter table; a trailing single step runs as a
+ * single-problem launch). Forward runs with samples and work4 / work5 take
+ * up to four steps per launch, followed by one batched launch that gathers
+ * the samples of every level. Results equal n separate swfd_forward /
+ * swfd_adjoint calls bit for bit.
  *
  * Three phases, so that the launch phase can be captured in a HIP graph:
  *   *_table_bytes: size of the device table the run needs (host only);

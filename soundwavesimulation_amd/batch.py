@@ -47,6 +47,8 @@ def _forward_args(solver, n_shots, rows, cols, values, n_steps, *, init, work, s
         init_old=_ptr(init[0]), init_u=_ptr(init[1]),
         work0=_ptr(work[0]), work1=_ptr(work[1]), work2=_ptr(work[2]),
         work3=_ptr(work[3]) if len(work) > 3 else None,
+        work4=_ptr(work[4]) if len(work) > 5 else None,
+        work5=_ptr(work[5]) if len(work) > 5 else None,
         final_old=_ptr(final[0]) if final is not None else None,
         final_u=_ptr(final[1]) if final is not None else None,
         src_rows=_ptr(rows), src_cols=_ptr(cols), src_values=_ptr(values),
@@ -203,9 +205,13 @@ class UnevenGradients:
     def __init__(self, problems):
         self.problems = list(problems)
         self.phases = max(p.n_seg for p in self.problems)
+        # the history is idle during the forward pass: two of its fields keep the
+        # intermediate steps of four-step launches until their samples are gathered
         self.forward = BatchedForward([
             (p.solver, p.sampler, dict(n_shots=p.S, rows=p.rows, cols=p.cols, values=p.values,
-                                       n_steps=p.N, init=(p.work[0], p.work[1]), work=p.work,
+                                       n_steps=p.N, init=(p.work[0], p.work[1]),
+                                       work=p.work + ([p.hist[0], p.hist[1]]
+                                                      if p.hist.shape[0] >= 2 else []),
                                        samples=p.samples, ckpt=p.ckpt,
                                        ckpt_every=p.K if p.ckpt is not None else 0))
             for p in self.problems]).prepare()

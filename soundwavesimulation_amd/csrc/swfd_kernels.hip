@@ -2053,6 +2053,36 @@ __global__ void __launch_bounds__(kGatherThreads) gather_levels_kernel(StepParam
     dest[t] = acc;
 }
 
+// Batched form of gather_levels_kernel: entry e of `table` is one run's
+// launch (its levels, fields and receivers); its workgroups are numbered from
+// first[e].
+__global__ void __launch_bounds__(kGatherThreads) gather_levels_batch_kernel(
+    const StepParams *table, const int *first, int n)
+{
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    const StepParams &p = batch_params(table, e);
+    const int per = p.n_shots * p.n_rec;
+    int t = (b - first[e]) * (int)blockDim.x + (int)threadIdx.x;
+    const int j = t / per;
+    if (j >= p.levels) return;
+    t -= j * per;
+    const float *field = p.lv_out[j];
+    float *dest = p.lv_samples[j];
+    const int R = p.n_rec;
+    const int s = t / R;
+    const int r = t - s * R;
+    const int W = p.nx;
+    const float *f = field + (size_t)s * ((size_t)p.ny * W);
+    const size_t i = (size_t)p.rec_row0[r] * W + p.rec_col0[r];
+    const float *w = p.rec_w;
+    float acc = f[i] * w[r];
+    acc = acc + f[i + 1] * w[R + r];
+    acc = acc + f[i + W] * w[2 * R + r];
+    acc = acc + f[i + W + 1] * w[3 * R + r];
+    dest[t] = acc;
+}
+
 // Stand-alone imaging condition: g_out = g_in + img_scale * sum_s f_cur[s] *
 // Lap(img[s]), with the same per-cell accumulation order as the fused path.
 __global__ void __launch_bounds__(256) image_kernel(StepParams p)
@@ -3055,8 +3085,7 @@ struct ForwardRun {
     int pend_step[2] = {0, 0};
     int n_pend = 0;
 
-    // batched: lockstep tables (no Laplacian histories, samples from the
-    // fused gather of two-step launches)
+    // batched: lockstep tables (no Laplacian histories)
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
              const swfd_forward_args *args, bool batched = false)
     {
@@ -3076,9 +3105,9 @@ struct ForwardRun {
         max_levels = plan.levels;
         wide = plan.wide;
         // receiver samples are gathered from stored fields: more than two levels
-        // per launch need two more fields to keep the intermediate levels (the
-        // batched launches gather the previous launch's two outputs instead)
-        sep_gather = !batched && wide && a->samples != nullptr && max_levels > 2 &&
+        // per launch need two more fields to keep the intermediate levels (without
+        // them, launches take two steps and the next launch gathers their outputs)
+        sep_gather = wide && a->samples != nullptr && max_levels > 2 &&
                      a->work4 != nullptr && a->work5 != nullptr;
         if (!wide || (a->samples != nullptr && !sep_gather)) max_levels = std::min(max_levels, 2);
         lap = a->hist_lap != 0;
@@ -3578,11 +3607,28 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 g.first.push_back(g.first.back() + blocks);
                 g.runs.push_back(i);
             } else {
-                if (mode == 2 && (rc = launch_step(p, false, st, gathers))) return rc;
+                if (mode == 2 && ((rc = launch_step(p, false, st, gathers)) ||
+                                  (rc = r.launch_gather(p, st))))
+                    return rc;
                 r.advance(p);
             }
         }
         if (!any) break;
+        // receiver samples of every level of this tick's full-width launches, for
+        // the runs that gather after their launches: one batched launch after them
+        LaunchGroup gather;
+        gather.kind = 2;
+        gather.threads = kGatherThreads;
+        for (const LaunchGroup &g : groups)
+            for (size_t j = 0; j < g.runs.size(); ++j) {
+                const ForwardRun &r = runs[g.runs[j]];
+                if (!r.sep_gather || g.kind != 1) continue;
+                const StepParams &p = g.entries[j];
+                const int pairs = p.levels * p.n_shots * p.n_rec;
+                gather.entries.push_back(p);
+                gather.first.push_back(gather.first.back() +
+                                       (pairs + kGatherThreads - 1) / kGatherThreads);
+            }
         for (const LaunchGroup &g : groups) {
             rc = emit_group(g, mode, host, off, table, table_bytes, "forward batch",
                             [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
@@ -3601,6 +3647,17 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                             });
             if (rc) return rc;
             for (size_t j = 0; j < g.runs.size(); ++j) runs[g.runs[j]].advance(g.entries[j]);
+        }
+        if (!gather.entries.empty()) {
+            rc = emit_group(gather, mode, host, off, table, table_bytes, "forward batch",
+                            [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
+                                int ne) {
+                                hipLaunchKernelGGL(gather_levels_batch_kernel,
+                                                   dim3((unsigned)gg.first.back()),
+                                                   dim3(kGatherThreads), 0, st, tab, first, ne);
+                                return SWFD_OK;
+                            });
+            if (rc) return rc;
         }
     }
     if ((rc = finish_tables(mode, host, off, table, table_bytes, bytes_out, "forward batch")))

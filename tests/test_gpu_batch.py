@@ -45,8 +45,9 @@ def _problem(ny, nx, n_el, seed):
     return solver, sampler, array, rows, cols, dt
 
 
-def test_batched_forward_equals_separate_runs():
-    print("batched forward, 3 problems", flush=True)
+@pytest.mark.parametrize("extra", [0, 2])  # 2: four-step launches + batched level gathers
+def test_batched_forward_equals_separate_runs(extra):
+    print(f"batched forward, 3 problems, {4 + extra} work fields", flush=True)
     runs, refs = [], []
     for i, (ny, nx, n_el, N) in enumerate(SPECS):
         solver, sampler, array, rows, cols, dt = _problem(ny, nx, n_el, i)
@@ -54,7 +55,8 @@ def test_batched_forward_equals_separate_runs():
         shape = (n_el, ny, nx)
         out = {}
         for mode in ("single", "batch"):
-            work = [torch.zeros(shape, device=DEV) for _ in range(4)]
+            n_work = 4 + (extra if mode == "batch" else 0)
+            work = [torch.zeros(shape, device=DEV) for _ in range(n_work)]
             samples = torch.full((N, n_el, n_el), float("nan"), device=DEV)
             ckpt = torch.zeros((6, *shape), device=DEV)
             kw = dict(n_shots=n_el, rows=rows, cols=cols, values=values, n_steps=N,
