@@ -4,10 +4,12 @@
  * design document (TFNO_Full_Ring_Plan.tex:200-209, 252-254) describes a
  * Fourier neural operator that maps measurements to slowness squared. This
  * library holds the device kernels the host-side operator
- * (soundwavesimulation_amd/surrogate/operator.py) is built from: one batched
- * strided GEMM on bf16 MFMA with fp32 accumulation, which carries every
+ * (soundwavesimulation_amd/surrogate/operator.py) is built from: a batched
+ * strided GEMM on bf16 MFMA with fp32 accumulation, which can carry every
  * contraction (truncated DFTs, per-mode complex mixing, pointwise linears and
- * all their gradients), and the elementwise pieces around it.
+ * all their gradients), the elementwise pieces around it, and fused kernels
+ * for width 32 / hidden 128 (layer tails, projection head, the x transform,
+ * weight and bias gradients in one pass) that move each activation once.
  *
  * All pointers are device pointers to float32 unless stated; every entry
  * point enqueues on `stream` (a hipStream_t) and returns SWNO_OK or an error

@@ -1,7 +1,9 @@
 // Spectral neural operator kernels for MI355X (gfx950): a batched strided
-// GEMM on bf16 MFMA with fp32 accumulation, and the elementwise kernels of
-// the operator (bias + GELU, spectral repacks, mode-mixing weights, masked
-// MSE, AdamW). The C ABI is declared in include/swno.h.
+// GEMM on bf16 MFMA with fp32 accumulation, fused MFMA kernels for the
+// operator's memory-bound stages (truncated x transform, spectral-layer
+// tails, projection head, weight and bias gradients), and the elementwise
+// kernels (bias + GELU, spectral repacks, mode-mixing weights, masked MSE,
+// AdamW). The C ABI is declared in include/swno.h.
 //
 // The operator itself has no counterpart in the reference's code (its design
 // document, TFNO_Full_Ring_Plan.tex:200-209, describes a Fourier neural
