@@ -506,16 +506,20 @@ __global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, i
     }
 }
 
-__global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n, int k1,
-                                                            const float *Gx, const float *Z,
-                                                            const float *v, const float *W,
-                                                            const float *bias, float *pre,
-                                                            float *vnext, int act)
+// Forward tail of a layer: each wavefront's outputs (its 32 rows x 32
+// channels of pre and vnext) are staged in LDS and written with float4
+// stores, consecutive lanes at consecutive 16 bytes (measured faster than
+// stores from the MFMA output layout, which leaves 64-byte pieces).
+__global__ void __launch_bounds__(256) layer_forward_kernel(
+    int n_slices, int n, int k1, const float *Gx, const float *Z, const float *v,
+    const float *W, const float *bias, float *pre, float *vnext, int act)
 {
+    constexpr int kStride = kLayerC + 4;  // floats per staged row (16-byte aligned rows)
+    __shared__ __attribute__((aligned(16))) float stage[4][2][32][kStride];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, g = lane >> 4;
     const int row_tiles = n >> 4;
-    const int kv = max(0, min(8, k1 - 8 * g));  // valid K of this lane group in Gx / Z
+    const int kv = max(0, min(8, k1 - 8 * g));
     bf16x8 ga[2], wb[2];
     float bc[2];
 #pragma unroll
@@ -528,25 +532,20 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n,
         wb[j] = bf16_col8(W + (8 * g) * kLayerC + 16 * j + r16, kLayerC, 8);
         bc[j] = bias[16 * j + r16];
     }
-    // a slice's operands: the next slice's are loaded while the current one is computed
-    bf16x8 zb[2], va[2], zn[2], vn[2];
-    auto load = [&](int sl, bf16x8 (&z)[2], bf16x8 (&x)[2]) {
+    float (*sp)[kStride] = stage[wave][0];
+    float (*sv)[kStride] = stage[wave][1];
+    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
         const float *zs = Z + (int64_t)sl * k1 * kLayerC;
         const float *vs = v + (int64_t)sl * n * kLayerC;
+        bf16x8 zb[2], va[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) z[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
-                                                     kLayerC, kv);
+        for (int j = 0; j < 2; ++j) zb[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
+                                                      kLayerC, kv);
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int rt = wave + 4 * i;
-            if (rt < row_tiles) x[i] = bf16_row8(vs + (16 * rt + r16) * kLayerC + 8 * g, 8);
+            if (rt < row_tiles) va[i] = bf16_row8(vs + (16 * rt + r16) * kLayerC + 8 * g, 8);
         }
-    };
-    if (blockIdx.x < n_slices) load(blockIdx.x, zb, va);
-    for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
-        if (sl + (int)gridDim.x < n_slices) load(sl + gridDim.x, zn, vn);
-        float *ps = pre + (int64_t)sl * n * kLayerC;
-        float *ns = vnext + (int64_t)sl * n * kLayerC;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int rt = wave + 4 * i;
@@ -558,18 +557,37 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(int n_slices, int n,
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[i], wb[j], acc, 0, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int idx = (16 * rt + 4 * g + e) * kLayerC + 16 * j + r16;
+                    const int rl = 16 * i + 4 * g + e;  // row within the wavefront's stage
                     const float pv = acc[e] + bc[j];
-                    ps[idx] = pv;
-                    ns[idx] = act ? gelu(pv) : pv;
+                    sp[rl][16 * j + r16] = pv;
+                    sv[rl][16 * j + r16] = act ? gelu(pv) : pv;
                 }
             }
         }
+        // the wavefront reads back only what it wrote (LDS is in order per wavefront)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float *ps = pre + (int64_t)sl * n * kLayerC;
+        float *ns = vnext + (int64_t)sl * n * kLayerC;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            zb[j] = zn[j];
-            va[j] = vn[j];
+        for (int i = 0; i < 2; ++i) {
+            const int rt = wave + 4 * i;
+            if (rt >= row_tiles) break;
+            // 16 rows x 32 floats = 128 float4, two per lane
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = lane + 64 * h;
+                const int rr = q >> 3, c4 = (q & 7) * 4;
+                const int rl = 16 * i + rr;
+                const int64_t o = (int64_t)(16 * rt + rr) * kLayerC + c4;
+                *(f4 *)(ps + o) = *(const f4 *)&sp[rl][c4];
+                *(f4 *)(ns + o) = *(const f4 *)&sv[rl][c4];
+            }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -659,7 +677,7 @@ __global__ void __launch_bounds__(256) layer_backward_kernel(int n_slices, int n
 // 16 rows: one MFMA 16x16x32 per 16 hidden columns (K = the 32 channels, Q1
 // held in registers as bf16 B operands), then bias, GELU and the dot with Q2
 // in fp32 in the MFMA's output layout, and a fixed-order reduction over the
-// 16 lanes that hold a row's columns.
+// 16 lanes that hold a row's columns; hpre goes out through LDS as float4.
 constexpr int kHeadC = 32, kHeadH = 128;
 constexpr int kHeadTiles = kHeadH / 16;
 
@@ -668,8 +686,11 @@ __global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const f
                                                            const float *Q2, const float *bq2,
                                                            float *hpre, float *out)
 {
+    // per wavefront: its 16 rows of hpre, staged for float4 stores
+    __shared__ __attribute__((aligned(16))) float stage[4][16][kHeadH + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, g = lane >> 4;
+    float (*st)[kHeadH + 4] = stage[wave];
     bf16x8 qb[kHeadTiles];
     float bias[kHeadTiles], q2[kHeadTiles];
 #pragma unroll
@@ -704,12 +725,26 @@ __global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const f
                 a, qb[t], f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t row = row0 + 4 * g + e;
                 const float hp = acc[e] + bias[t];
-                if (row < rows) hpre[row * kHeadH + 16 * t + r16] = hp;
+                st[4 * g + e][16 * t + r16] = hp;
                 part[e] += gelu(hp) * q2[t];
             }
         }
+        // hpre of the wavefront's 16 rows from LDS with float4 stores (the wavefront
+        // reads back only what it wrote; LDS is in order per wavefront)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int h = 0; h < kHeadH * 16 / 4 / 64; ++h) {
+            const int q = lane + 64 * h;
+            const int rr = q / (kHeadH / 4), c4 = (q % (kHeadH / 4)) * 4;
+            const int64_t row = row0 + rr;
+            if (row < rows) *(f4 *)(hpre + row * kHeadH + c4) = *(const f4 *)&st[rr][c4];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
 #pragma unroll
@@ -1363,8 +1398,6 @@ int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, con
         return fail(SWNO_EINVAL, "layer_forward: operand is NULL");
     if (reinterpret_cast<uintptr_t>(v) % 16 != 0 || reinterpret_cast<uintptr_t>(Gx) % 16 != 0)
         return fail(SWNO_EINVAL, "layer_forward: v and Gx must be 16-byte aligned");
-    // one slice per workgroup measured faster here than a few with prefetch (the stores
-    // of pre and vnext dominate)
     const unsigned grid = (unsigned)std::min(slices, 8192);
     hipLaunchKernelGGL(layer_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
                        n, 2 * modes, Gx, Z, v, W, bias, pre, vnext, gelu_act);
