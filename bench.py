@@ -10,10 +10,11 @@ resident in HBM before the timed region.
 
 Printed JSON (one line): Gcell/s = 2 * ny * nx * steps * shots / seconds per
 gradient, plus
-  roofline      -- the adjoint kernel (two steps per launch): 20 B per
-                   cell-update (read lam, lam_old, c^2, forward field; write
-                   lam_next) times the cell-updates of one launch, over its
-                   average launch time, timed with events on torch's stream;
+  roofline      -- the adjoint kernel (the steps per launch the library
+                   reports, four at this size): 20 B per cell-update (read
+                   lam, lam_old, c^2, forward field; write lam_next) times
+                   the cell-updates of one launch, over its average launch
+                   time, timed with events on torch's stream;
   cpu_baseline  -- the float64 C oracle (OpenMP, all host cores) on a
                    bounded sample of the same workload;
   torch_reference -- the reference's op sequence (conv2d Laplacian,
