@@ -63,6 +63,9 @@ typedef struct swfd_receivers {
     const int32_t *entry_cell;  /* (E) */
     const int32_t *pair_rec;    /* (P) */
     const float *pair_w;        /* (P) */
+    /* optional (ny), from swfd_footprint_rows; NULL: sampled forward runs
+     * store every step's field in full */
+    const uint32_t *footprint_rows;
 } swfd_receivers;
 
 /* One time step for n_shots shots (torch_backend.py:177-233 single shot,
@@ -167,6 +170,18 @@ int32_t swfd_lap_history(int32_t ny, int32_t nx);
  * under the current tuning table and environment; 0 for bad arguments (no
  * counterpart in the reference; for profiling). */
 int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samples);
+
+/* Rows and full-width wavefronts that hold receiver footprint cells
+ * (host-only, no launch; host pointers; no counterpart in the reference):
+ * out[y] (ny entries, overwritten) gets bit w when row y has a footprint cell
+ * (row0[i] + {0,1}, col0[i] + {0,1}) among the columns wavefront w of a
+ * full-width launch covers; all zero when nx > 1023 (no full-width launches).
+ * A device copy in swfd_receivers.footprint_rows lets sampled forward runs
+ * that keep intermediate steps only for the receiver gather (work4/work5,
+ * no history slot) store those steps' fields only there; samples, states,
+ * checkpoints and histories are unchanged. */
+int swfd_footprint_rows(int32_t ny, int32_t nx, int32_t n, const int32_t *row0,
+                        const int32_t *col0, uint32_t *out);
 
 /* Thread-local message of the last failure. */
 const char *swfd_last_error(void);

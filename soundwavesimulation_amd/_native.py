@@ -37,7 +37,8 @@ class Medium(ctypes.Structure):
 class Receivers(ctypes.Structure):
     _fields_ = [("n", _c_int), ("row0", _fp), ("col0", _fp), ("weights", _fp),
                 ("n_entries", _c_int), ("cell_map", _fp), ("entry_start", _fp),
-                ("entry_cell", _fp), ("pair_rec", _fp), ("pair_w", _fp)]
+                ("entry_cell", _fp), ("pair_rec", _fp), ("pair_w", _fp),
+                ("footprint_rows", _fp)]
 
 
 class StepArgs(ctypes.Structure):
@@ -75,7 +76,7 @@ EXPORTS = (
     "swfd_residual_workspace_floats", "swfd_residual", "swfd_forward_batch_table_bytes",
     "swfd_forward_batch_prepare", "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
     "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
-    "swfd_lap_history", "swfd_steps_per_launch",
+    "swfd_lap_history", "swfd_steps_per_launch", "swfd_footprint_rows",
 )
 
 _lib = None
@@ -101,6 +102,7 @@ def load_library(path=LIBRARY):
     lib.swfd_prepare_medium.argtypes = [P(Grid), vp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, vp, vp]
     lib.swfd_medium_view.argtypes = [P(Grid), vp, P(Medium)]
+    lib.swfd_footprint_rows.argtypes = [_c_int, _c_int, _c_int, vp, vp, vp]
     lib.swfd_medium_view.restype = None
     lib.swfd_step.argtypes = [P(Grid), P(Medium), P(Receivers), P(StepArgs), vp]
     lib.swfd_gather.argtypes = [P(Grid), P(Receivers), _c_int, vp, vp, vp]
@@ -133,7 +135,8 @@ def load_library(path=LIBRARY):
                  "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
                  "swfd_forward_batch_table_bytes", "swfd_forward_batch_prepare",
                  "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
-                 "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune"):
+                 "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune",
+                 "swfd_footprint_rows"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

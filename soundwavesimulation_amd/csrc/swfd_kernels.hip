@@ -116,6 +116,9 @@ struct StepParams {
     const float *lv_in[4], *lv_resid[4];
     // receiver samples of each level of a multi-step forward launch (S, R)
     float *lv_samples[4];
+    // per row: bit w set when full-width wavefront w holds a receiver
+    // footprint cell there (NULL: intermediate levels stored everywhere)
+    const unsigned *fp_rows;
 };
 
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
@@ -990,8 +993,11 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, int vo
 constexpr int kPfW = 3;
 
 // Which levels a launch stores: kStoreAny = those with a non-NULL output
-// (checked per level), kStoreLastTwo = the last two, kStoreAll = every level.
-enum : int { kStoreAny = 0, kStoreLastTwo = 1, kStoreAll = 2 };
+// (checked per level), kStoreLastTwo = the last two, kStoreAll = every level,
+// kStoreFootprint = the last two everywhere and the others (kept only for the
+// receiver gather after the launch) in the rows and wavefronts p.fp_rows
+// marks.
+enum : int { kStoreAny = 0, kStoreLastTwo = 1, kStoreAll = 2, kStoreFootprint = 3 };
 
 // XE: the wavefront holds column 0 or nx - 1 (Mur side updates); YE: the
 // segment's levels reach rows 0 / 1 or ny - 1 (Mur row updates).
@@ -1096,8 +1102,12 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
         }
         __syncthreads();
         auto store_row = [&](int j, int rho, float v) {
-            const bool stored = SM == kStoreAll || (SM == kStoreLastTwo && j >= KL - 2) ||
-                                (SM == kStoreAny && p.lv_out[j] != nullptr);
+            bool stored = SM == kStoreAll || (SM == kStoreLastTwo && j >= KL - 2) ||
+                          (SM == kStoreAny && p.lv_out[j] != nullptr);
+            if constexpr (SM == kStoreFootprint) {
+                // row and wavefront are uniform: a scalar load of the row's mask
+                stored = j >= KL - 2 || (((unsigned)((const kci *)p.fp_rows)[rho] >> wave) & 1u) != 0;
+            }
             if (stored) bstore(ro[j], v, so, rho * row_bytes);
             if constexpr ((F & kOut2) != 0)
                 if (p.lv_out2[j] != nullptr) bstore(ro2[j], v, so, rho * row_bytes);
@@ -2604,6 +2614,11 @@ int fwdm_variant(int sm, unsigned f, Fn &&fn)
         switch (sm) {
             case kStoreAll: SWFD_FWDM(kStoreAll, kPoint); break;
             case kStoreLastTwo: SWFD_FWDM(kStoreLastTwo, kPoint); break;
+            case kStoreFootprint:
+                // two levels are both stored everywhere
+                if constexpr (KL > 2) SWFD_FWDM(kStoreFootprint, kPoint);
+                else SWFD_FWDM(kStoreAll, kPoint);
+                break;
             default: SWFD_FWDM(kStoreAny, kPoint); break;
         }
     }
@@ -2646,8 +2661,10 @@ int multi_shape(StepParams &p, int gather_pairs_count, int &threads, unsigned &f
     }
     if (gather_pairs_count > 0) f |= kGather;
     if (p.hist_lap) f |= kLap;
+    const bool footprint = all && p.fp_rows != nullptr && p.levels > 2;
     sm = (p.hist_lap && last_two) ? kStoreLastTwo
-         : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny));
+         : (footprint ? kStoreFootprint
+                      : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny)));
     return blocks;
 }
 
@@ -3203,6 +3220,11 @@ struct ForwardRun {
             p.lv_out2[j] = ckpt_out(n + j);
             if (sep_gather) p.lv_samples[j] = a->samples + (size_t)(n + j) * sr;
         }
+        // levels before the last two that only feed the gather (no history
+        // slot) need their fields only where the receivers sit
+        bool gather_only = sep_gather && rec != nullptr && rec->footprint_rows != nullptr;
+        for (int j = 0; j + 2 < L; ++j) gather_only = gather_only && hist_out(n + j) == nullptr;
+        if (gather_only) p.fp_rows = rec->footprint_rows;
         p.f_out = p.lv_out[0];
         p.f_out2 = p.lv_out2[0];
         if (L >= 2) {
@@ -3765,6 +3787,26 @@ const char *swfd_version(void) { return "swfd 0.1 gfx950"; }
 size_t swfd_medium_floats(int32_t ny, int32_t nx)
 {
     return 2 * (size_t)ny * nx + 2 * (size_t)ny + 2 * (size_t)nx + 8;
+}
+
+int swfd_footprint_rows(int32_t ny, int32_t nx, int32_t n, const int32_t *row0,
+                        const int32_t *col0, uint32_t *out)
+{
+    if (ny < 3 || nx < 3 || n < 0 || (n > 0 && (row0 == nullptr || col0 == nullptr)) ||
+        out == nullptr)
+        return fail(SWFD_EINVAL, "footprint rows: bad arguments");
+    for (int y = 0; y < ny; ++y) out[y] = 0u;
+    if (!wide_fits(nx)) return SWFD_OK;
+    // the column layout of full-width launches (wide_geometry / adjoint shape)
+    const int off = (nx % kLanes == 1) ? 1 : 0;
+    for (int i = 0; i < n; ++i) {
+        if (row0[i] < 0 || row0[i] + 1 >= ny || col0[i] < 0 || col0[i] + 1 >= nx)
+            return fail(SWFD_EINVAL, "footprint rows: receiver footprint outside the grid");
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                out[row0[i] + dy] |= 1u << ((col0[i] + dx + off) / kLanes);
+    }
+    return SWFD_OK;
 }
 
 void swfd_medium_view(const swfd_grid *grid, const float *coef, swfd_medium *out)

@@ -20,6 +20,7 @@ rotate three internal buffers inside one native call.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import torch
@@ -377,12 +378,23 @@ class TorchRingSampler:
         table = scatter_table(ring_array._row0, ring_array._col0, weights, self.grid_shape)
         (self._cell_map, self._entry_start, self._entry_cell, self._pair_rec,
          self._pair_w) = (torch.as_tensor(a).to(device) for a in table)
+        # rows and full-width wavefronts holding footprint cells: sampled forward
+        # runs keep intermediate steps' fields only there (same samples)
+        self._footprint_rows = None
+        if os.environ.get("SWFD_FOOTPRINT_ROWS", "1") != "0":
+            rows = np.zeros(ny, dtype=np.uint32)
+            r0_h = np.ascontiguousarray(r0, dtype=np.int32)
+            c0_h = np.ascontiguousarray(c0, dtype=np.int32)
+            _native.check(_native.lib().swfd_footprint_rows(
+                ny, nx, int(r0_h.size), r0_h.ctypes.data, c0_h.ctypes.data, rows.ctypes.data))
+            self._footprint_rows = torch.as_tensor(rows.view(np.int32)).to(device)
         self._rec = _native.Receivers(
             n=self.n_elements, row0=_native.ptr(self._r0), col0=_native.ptr(self._c0),
             weights=_native.ptr(self._w), n_entries=int(self._entry_cell.numel()),
             cell_map=_native.ptr(self._cell_map), entry_start=_native.ptr(self._entry_start),
             entry_cell=_native.ptr(self._entry_cell), pair_rec=_native.ptr(self._pair_rec),
-            pair_w=_native.ptr(self._pair_w))
+            pair_w=_native.ptr(self._pair_w),
+            footprint_rows=_native.ptr(self._footprint_rows))
         ny, nx = self.grid_shape
         self._grid = _native.Grid(ny=ny, nx=nx, inv_dx2=0.0, inv_dy2=0.0, center=0.0, dt2=0.0)
 

@@ -68,3 +68,57 @@ def test_batch_table_sizes_are_host_only():
     # 7 steps = 3 two-step ticks (the 7th step runs alone) with 2 entries each
     assert nbytes.value > 0 and nbytes.value % 256 == 0
     lib.swfd_tune_clear()
+
+
+def footprint_rows(ny, nx, row0, col0):
+    import numpy as np
+
+    lib = _native.load_library()
+    out = np.full(ny, 7, dtype=np.uint32)
+    r = np.ascontiguousarray(row0, dtype=np.int32)
+    c = np.ascontiguousarray(col0, dtype=np.int32)
+    rc = lib.swfd_footprint_rows(ny, nx, int(r.size), r.ctypes.data, c.ctypes.data,
+                                 out.ctypes.data)
+    return rc, out
+
+
+def test_footprint_rows_mark_the_wavefronts_of_every_footprint_cell():
+    """Host-only: bit w of row y is set exactly when a receiver's 2x2 footprint
+    has a cell in row y among columns 64w - off .. 64w - off + 63 (off = 1 when
+    nx = 64k + 1, else 0), checked cell by cell against a brute-force map."""
+    import numpy as np
+
+    from soundwavesimulation_amd import RingArray, axis_centers
+
+    for n, h, radius, n_el in ((1024, 0.25e-3, 0.110, 256), (513, 0.5e-3, 0.120, 128),
+                               (200, 0.5e-3, 0.045, 37)):
+        x = axis_centers(n, h)
+        array = RingArray(n_el, radius, x, x)
+        r0, c0, _ = array.footprint()
+        rc, rows = footprint_rows(n, n, r0, c0)
+        assert rc == 0
+        off = 1 if n % 64 == 1 else 0
+        cells = np.zeros((n, n), dtype=bool)
+        for dy in (0, 1):
+            for dx in (0, 1):
+                cells[np.asarray(r0) + dy, np.asarray(c0) + dx] = True
+        waves = (np.arange(n) + off) // 64
+        expect = np.zeros(n, dtype=np.uint32)
+        for y, xcol in zip(*np.nonzero(cells)):
+            expect[y] |= np.uint32(1) << np.uint32(waves[xcol])
+        np.testing.assert_array_equal(rows, expect)
+        # a ring crosses most rows, but in only a few wavefronts per row
+        marked = np.array([bin(int(v)).count("1") for v in rows])
+        assert marked.max() <= 4
+
+
+def test_footprint_rows_validation_and_wide_grids():
+    import numpy as np
+
+    rc, rows = footprint_rows(64, 2048, [10], [2000])
+    assert rc == 0 and not rows.any()  # no full-width launches that wide
+    rc, _ = footprint_rows(64, 64, [63], [5])
+    assert rc == _native.SWFD_EINVAL
+    assert b"outside the grid" in _native.load_library().swfd_last_error()
+    rc, _ = footprint_rows(64, 64, [5], [-1])
+    assert rc == _native.SWFD_EINVAL
