@@ -385,7 +385,8 @@ def main():
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a, c_true, dt, h)
     line = {
-        "metric": "FWI gradient Gcell/s @1024^2, 256 shots (forward+adjoint cell-updates/s)",
+        "metric": f"FWI gradient Gcell/s @{a.grid}^2, {a.shots} shots "
+                  "(forward+adjoint cell-updates/s)",
         "value": round(gcells, 3), "unit": "Gcell/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
