@@ -196,20 +196,25 @@ def cpu_baseline(a, c_true, dt, h):
                       f"(forward+adjoint), {sec:.1f} s"}
 
 
-def torch_reference(a, problem, n_fwd=10):
-    """Reference op sequence in eager PyTorch on the GPU, per gradient (seconds)."""
+def torch_reference(a, problem, n_fwd=10, device=None, S=8):
+    """Reference op sequence in eager PyTorch, per gradient (seconds), on the
+    GPU (default; S = 8 shots per batch, what the reference's memory model
+    admits at 1024^2) or on the host cores (device="cpu")."""
     import torch.nn.functional as F
 
-    dev = problem.model.device
+    dev = problem.model.device if device is None else torch.device(device)
+    sync = torch.cuda.synchronize if dev.type == "cuda" else (lambda: None)
     solver = problem.solver
     sampler = problem.sampler
     ny, nx = problem.model.shape
-    S = 8  # shots per batch the reference's memory model admits at this size
-    c2 = problem.model.c2
-    kernel = problem.model.laplacian_kernel
+    c2 = problem.model.c2.to(dev)
+    kernel = problem.model.laplacian_kernel.to(dev)
     dt2 = solver.dt2
-    kl, kr, kb, kt = solver.k_left, solver.k_right, solver.k_bottom, solver.k_top
+    kl, kr, kb, kt = (t.to(dev) for t in (solver.k_left, solver.k_right, solver.k_bottom,
+                                          solver.k_top))
     ckx, cky = solver.corner_kx, solver.corner_ky
+    if torch.is_tensor(ckx):
+        ckx, cky = ckx.to(dev), cky.to(dev)
     cr = torch.tensor([0, 0, -1, -1], device=dev)
     cc = torch.tensor([0, -1, 0, -1], device=dev)
     cri = torch.tensor([1, 1, -2, -2], device=dev)
@@ -217,7 +222,7 @@ def torch_reference(a, problem, n_fwd=10):
     rows = torch.as_tensor(problem.array.rows[:S], device=dev)
     cols = torch.as_tensor(problem.array.cols[:S], device=dev)
     bi = torch.arange(S, device=dev)
-    row0, row1, col0, col1, w00, w10, w01, w11 = sampler.receivers
+    row0, row1, col0, col1, w00, w10, w01, w11 = (t.to(dev) for t in sampler.receivers)
 
     def mur(u, un):
         un[:, 1:-1, 0] = u[:, 1:-1, 1] + kl * (un[:, 1:-1, 1] - u[:, 1:-1, 0])
@@ -237,7 +242,7 @@ def torch_reference(a, problem, n_fwd=10):
     u_old = torch.zeros((S, ny, nx), device=dev)
     u = torch.zeros_like(u_old)
     hist = torch.empty((n_fwd, S, ny, nx), device=dev)
-    val = problem.source_values
+    val = problem.source_values.to(dev)
     g = torch.zeros((ny, nx), device=dev)
     res = torch.randn((S, sampler.n_elements), device=dev)
 
@@ -267,23 +272,24 @@ def torch_reference(a, problem, n_fwd=10):
         u_old, u = u, un
 
     with torch.inference_mode():
-        for k in range(3):
+        for k in range(min(3, n_fwd)):
             fwd(k)
             adj(k)
-        torch.cuda.synchronize()
+        sync()
         t0 = time.perf_counter()
         for k in range(n_fwd):
             fwd(k)
-        torch.cuda.synchronize()
+        sync()
         t_f = (time.perf_counter() - t0) / n_fwd
         t0 = time.perf_counter()
         for k in range(n_fwd):
             adj(k)
-        torch.cuda.synchronize()
+        sync()
         t_a = (time.perf_counter() - t0) / n_fwd
     per_grad = (t_f + t_a) * a.nsteps * (a.shots / S)
     del hist, u, u_old
-    torch.cuda.empty_cache()
+    if dev.type == "cuda":
+        torch.cuda.empty_cache()
     return per_grad, t_f, t_a, S
 
 
@@ -384,6 +390,19 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a, c_true, dt, h)
+        try:
+            # the reference's own path on the host cores: the same op sequence
+            # on device="cpu", 2 shots x 3 steps each way, scaled to a gradient
+            per_grad, t_f, t_a, S_ref = torch_reference(a, problem, n_fwd=3, device="cpu", S=2)
+            extra["torch_reference_cpu"] = {
+                "seconds_per_gradient": round(per_grad, 1),
+                "gcell_s": round(cells / per_grad / 1e9, 4),
+                "threads": torch.get_num_threads(),
+                "sample": f"eager PyTorch reference op sequence on the host, {S_ref} shots, "
+                          f"fwd {t_f*1e3:.1f} ms + adj {t_a*1e3:.1f} ms per step, scaled to "
+                          f"{a.shots} shots x {a.nsteps} steps"}
+        except Exception as exc:  # report, never hide the main number
+            extra["torch_reference_cpu"] = {"error": repr(exc)[:200]}
     line = {
         "metric": f"FWI gradient Gcell/s @{a.grid}^2, {a.shots} shots "
                   "(forward+adjoint cell-updates/s)",
