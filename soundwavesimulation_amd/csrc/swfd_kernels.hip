@@ -1728,11 +1728,14 @@ typedef AdjEdge AdjEdgeRow[kMaxWideWaves + 2][2];
 
 extern __shared__ float adjm_acc[];
 
-// IMG: 0 no imaging, 1 imaging against fields, 2 against their Laplacians.
+// IMG: 0 no imaging, 1 imaging against fields (levels without one skip it),
+// 2 against their Laplacians, 3 against fields with every level imaged.
 template <int KL, bool XE, bool YE, int IMG>
 __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wave, int grp,
                                           int seg, AdjEdgeRow *ex)
 {
+    // imaging against the fields themselves (3: every level has one)
+    constexpr bool FIELD = IMG == 1 || IMG == 3;
     const int W = p.nx, H = p.ny;
     const int x = wave * kLanes - p.wide_off + lane;
     const bool owned = (x >= 0) & (x < W);
@@ -1829,7 +1832,7 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 // level j + 1: newest field row (y - j), or the Laplacian of the
                 // row it images (y - j - 1)
                 qf[i][j] = 0.0f;
-                if constexpr (IMG == 1) qf[i][j] = gload_off(rf[j], xo + row_off(y - j));
+                if constexpr (FIELD) qf[i][j] = gload_off(rf[j], xo + row_off(y - j));
                 if constexpr (IMG == 2) qf[i][j] = gload_off(rf[j], xo + row_off(y - j - 1));
             }
         };
@@ -1877,7 +1880,7 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
             mh[0] = m0;
 #pragma unroll
             for (int j = 1; j <= KL; ++j) {
-                if constexpr (IMG == 1) {
+                if constexpr (FIELD) {
                     const int fr = y - j + 1;  // row of level j's newest imaging field
                     fh[j][0] = (owned & (fr >= 0) & (fr < H)) ? f[j - 1] : 0.0f;
                 } else {
@@ -1916,9 +1919,9 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                     bstore(ro[j - 1], v, so, rho * row_bytes);
                 if constexpr (IMG != 0) {
                     // row rho - 1 = y - j: its term goes to racc[j - 1]
-                    if (p.lv_in[j - 1] != nullptr) {
+                    if (IMG == 3 || p.lv_in[j - 1] != nullptr) {
                         float t;
-                        if constexpr (IMG == 1) {
+                        if constexpr (FIELD) {
                             const float F0 = fh[j][1];
                             const float fw = from_west(F0, hw->v[j - 1].y),
                                         fe = from_east(F0, he->v[j - 1].y);
@@ -1943,7 +1946,7 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 AdjEdge e;
 #pragma unroll
                 for (int j = 0; j < kMaxLevels; ++j)
-                    e.v[j] = j < KL ? make_float2(kh[j] * lv[j][0], IMG == 1 ? fh[j + 1][0] : 0.0f)
+                    e.v[j] = j < KL ? make_float2(kh[j] * lv[j][0], FIELD ? fh[j + 1][0] : 0.0f)
                                     : make_float2(0.0f, 0.0f);
                 ex[(y + 1) & 1][wave + 1][my_side] = e;
             }
@@ -2846,9 +2849,12 @@ AdjwShape adjw_shape(int ny, int nx, int n_shots)
 // Laplacians (history of Laplacians).
 int adjm_image(const StepParams &p)
 {
-    bool image = false;
-    for (int j = 0; j < p.levels; ++j) image = image || p.lv_in[j] != nullptr;
-    return !image ? 0 : (p.hist_lap ? 2 : 1);
+    bool image = false, every = true;
+    for (int j = 0; j < p.levels; ++j) {
+        image = image || p.lv_in[j] != nullptr;
+        every = every && p.lv_in[j] != nullptr;
+    }
+    return !image ? 0 : (p.hist_lap ? 2 : (every ? 3 : 1));
 }
 
 // Shape of a full-width adjoint launch of p.levels (2..4) steps: outputs of
@@ -2915,6 +2921,7 @@ int adjm_variant(int img, Fn &&fn)
     switch (img) {
         case 0: return adjm_variant_img<KL, 0>(fn);
         case 1: return adjm_variant_img<KL, 1>(fn);
+        case 3: return adjm_variant_img<KL, 3>(fn);
         default: return adjm_variant_img<KL, 2>(fn);
     }
 }
