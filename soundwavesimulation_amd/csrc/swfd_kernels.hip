@@ -1749,7 +1749,11 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     const size_t plane = (size_t)H * W;
     const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
     const int row_bytes = W * 4;
-    auto row_off = [&](int yy) -> int { return min(max(yy, 0), H - 1) * row_bytes; };
+    // segments away from the top and bottom (!YE) load only rows inside the grid
+    auto row_off = [&](int yy) -> int {
+        if constexpr (YE) return min(max(yy, 0), H - 1) * row_bytes;
+        else return yy * row_bytes;
+    };
     const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
     // the scatter-table map, Mur side coefficients and imaging fields are read
     // with global loads off a scalar base (2 SGPRs each instead of a 4-SGPR
@@ -1757,7 +1761,10 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     const float *emap = (const float *)p.cell_map;
     const float *mur = p.mur;
     const int mo = (x == 0) ? 0 : H * 4;
-    auto mrow = [&](int yy) -> int { return min(max(yy, 0), H - 1) * 4; };
+    auto mrow = [&](int yy) -> int {
+        if constexpr (YE) return min(max(yy, 0), H - 1) * 4;
+        else return yy * 4;
+    };
     const float dt2 = p.dt2;
     float kb_x = 0.0f, kt_x = 0.0f;
     if constexpr (YE) {
@@ -1984,7 +1991,10 @@ __device__ __forceinline__ void adjm_block(const StepParams &p, int b, AdjEdgeRo
     }
     // (the first barrier of the march orders these writes before any use)
     const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
-    const bool ye = (r0 <= KL) || (r1 + KL - 2 >= p.ny - 1);
+    // rows the march loads: y_begin - KL .. r1 + KL (imaging fields KL - 1 or KL
+    // rows behind, inputs one ahead); segments whose loads stay inside the grid
+    // take the variant without clamps or Mur row updates
+    const bool ye = (r0 < 2 * KL) || (r1 + KL >= p.ny);
     if (xe) {
         if (ye) adjm_wave<KL, true, true, IMG>(p, lane, wave, grp, seg, ex);
         else adjm_wave<KL, true, false, IMG>(p, lane, wave, grp, seg, ex);
