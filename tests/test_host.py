@@ -171,3 +171,18 @@ def test_warm_start_network_input():
     assert np.allclose(x[..., 1], (block_mean(c, 2) - 1500.0) / 100.0)
     assert np.allclose(x[0, :, 2], np.linspace(-1, 1, 4)) and np.allclose(x[:, 0, 3],
                                                                            np.linspace(-1, 1, 4))
+
+
+def test_default_checkpoint_interval_is_near_sqrt_2n_and_a_multiple_of_four():
+    import math
+
+    from soundwavesimulation_amd.inversion.fwi import default_checkpoint_every, segment_plan
+
+    for n in (20, 300, 1200, 1500, 4000, 5000, 6000, 20000):
+        k = default_checkpoint_every(n)
+        assert k % 4 == 0
+        assert math.sqrt(2 * n) <= k < math.sqrt(2 * n) + 4
+        n_seg, n_ckpt, n_hist = segment_plan(n, k)
+        assert n_seg == -(-n // k) and n_ckpt == 2 * (n_seg - 1) and n_hist == k + 1
+    assert default_checkpoint_every(6000) == 112
+    assert default_checkpoint_every(5) == 3 and default_checkpoint_every(1) == 2
