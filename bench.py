@@ -370,6 +370,12 @@ def main():
                 "steps_per_launch": steps_per_launch, "shots_per_launch": S,
                 "forward_step_ms": round(t_fwd * 1e3, 4),
                 "forward_gbs_16B": round(16.0 * units / t_fwd / 1e9, 1)}
+    if traffic is not None:
+        # measured HBM bytes over the same launch time: the rate the memory
+        # system actually carries (multi-step launches move fewer bytes than
+        # the 20 B per cell-update model, so `achieved` can exceed the peak)
+        roofline["traffic_gbs"] = round(traffic / t_adj / 1e9, 1)
+        roofline["traffic_frac"] = round(traffic / t_adj / 1e9 / HBM_PEAK_GBS, 4)
     print(f"# adjoint kernel {t_adj*1e3:.3f} ms/launch of {steps_per_launch} steps "
           f"({achieved:.0f} GB/s at 20 B/cell-update); forward {t_fwd*1e3:.3f} ms/step",
           file=sys.stderr, flush=True)
