@@ -2606,6 +2606,10 @@ int fwdm_variant(int sm, unsigned f, Fn &&fn)
         if ((f & kGather) != 0) {
             if constexpr (KL == 2) SWFD_FWDM(kStoreAny, kPoint | kOut2 | kGather);
             else return fail(SWFD_EINVAL, "multi-step launch: gathers need two levels");
+        } else if (sm == kStoreFootprint) {
+            // checkpoint copies go to their own outputs in full either way
+            if constexpr (KL > 2) SWFD_FWDM(kStoreFootprint, kPoint | kOut2);
+            else SWFD_FWDM(kStoreAny, kPoint | kOut2);
         } else {
             SWFD_FWDM(kStoreAny, kPoint | kOut2);
         }
