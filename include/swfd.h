@@ -249,6 +249,24 @@ int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *
                   const int32_t *obs_index, const int32_t *mute, float *residual,
                   float *workspace, float *misfit, void *stream);
 
+/* Partial sums the built-in scratch of swfd_residual_noalloc holds (4 MiB). */
+#define SWFD_RESIDUAL_SCRATCH_FLOATS (1u << 20)
+
+/* swfd_residual without a workspace argument: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The per-tile
+ * partial sums live in a device array built into the library (one per GPU,
+ * held until the process ends); the call allocates nothing. Same kernels,
+ * arguments and results as swfd_residual, bit for bit. Limit:
+ * swfd_residual_workspace_floats(n_steps, n_shots, n_rec) must not exceed
+ * SWFD_RESIDUAL_SCRATCH_FLOATS (else SWFD_EINVAL), e.g. 256 shots x 256
+ * receivers x 64 000 steps. Calls that overlap on one GPU (two host threads,
+ * or two streams not ordered by the caller) share that array and race. */
+int swfd_residual_noalloc(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                          const float *observed, int64_t obs_stride_e, int64_t obs_stride_r,
+                          const int32_t *obs_index, const int32_t *mute, float *residual,
+                          float *misfit, void *stream);
+
 /* Batched runs (no counterpart in the reference, whose batching stops at the
  * shots of one problem, torch_backend.py:601-770): n independent problems,
  * each with its own grid, medium, receivers and step range, advance in

@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -3802,6 +3803,67 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
 
 }  // namespace
 
+// Partial misfit sums of swfd_residual_noalloc: one float per residual tile,
+// SWFD_RESIDUAL_SCRATCH_FLOATS of them (4 MiB). Every GPU the library runs on
+// holds its own copy until the process ends. Host code never uses this name
+// as an address: it takes each device's address from hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[SWFD_RESIDUAL_SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(float **out)
+{
+    int dev = -1;
+    SWFD_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(SWFD_EINVAL, "residual (no allocation): device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        SWFD_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) return fail(SWFD_EHIP, "residual (no allocation): no scratch address");
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SWFD_OK;
+}
+
+int check_residual_args(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                        const float *observed, const int32_t *obs_index, const int32_t *mute,
+                        const float *residual, const float *misfit)
+{
+    if (n_steps < 1 || n_shots < 1 || n_rec < 1)
+        return fail(SWFD_EINVAL, "residual: sizes must be positive");
+    if (samples == nullptr || observed == nullptr || obs_index == nullptr ||
+        mute == nullptr || residual == nullptr || misfit == nullptr)
+        return fail(SWFD_EINVAL, "residual: NULL pointer");
+    return SWFD_OK;
+}
+
+// residual_kernel over every tile (one partial per tile into `partials`), then
+// misfit_kernel over those partials; both write before they read, so the
+// partials need no clearing.
+int launch_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                    const float *observed, int64_t obs_stride_e, int64_t obs_stride_r,
+                    const int32_t *obs_index, const int32_t *mute, float *residual,
+                    float *partials, size_t tiles, float *misfit, hipStream_t st)
+{
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n_steps,
+                       n_shots, n_rec, samples, observed, (long long)obs_stride_e,
+                       (long long)obs_stride_r, obs_index, mute, residual, partials);
+    SWFD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(misfit_kernel, dim3(1), dim3(256), 0, st, (int)tiles, partials, misfit);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *swfd_last_error(void) { return g_last_error.c_str(); }
@@ -4103,22 +4165,38 @@ int swfd_residual(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *
                   const int32_t *obs_index, const int32_t *mute, float *residual,
                   float *workspace, float *misfit, void *stream)
 {
-    if (n_steps < 1 || n_shots < 1 || n_rec < 1)
-        return fail(SWFD_EINVAL, "residual: sizes must be positive");
-    if (samples == nullptr || observed == nullptr || obs_index == nullptr ||
-        mute == nullptr || residual == nullptr || workspace == nullptr || misfit == nullptr)
-        return fail(SWFD_EINVAL, "residual: NULL pointer");
+    int rc = check_residual_args(n_steps, n_shots, n_rec, samples, observed, obs_index, mute,
+                                 residual, misfit);
+    if (rc) return rc;
+    if (workspace == nullptr) return fail(SWFD_EINVAL, "residual: NULL pointer");
     const size_t blocks = swfd_residual_workspace_floats(n_steps, n_shots, n_rec);
     if (blocks > 0x7fffffff) return fail(SWFD_EINVAL, "residual: too many tiles");
-    hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n_steps,
-                       n_shots, n_rec, samples, observed, (long long)obs_stride_e,
-                       (long long)obs_stride_r, obs_index, mute, residual, workspace);
-    SWFD_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(misfit_kernel, dim3(1), dim3(256), 0, st, (int)blocks, workspace,
-                       misfit);
-    SWFD_HIP_CHECK(hipGetLastError());
-    return SWFD_OK;
+    return launch_residual(n_steps, n_shots, n_rec, samples, observed, obs_stride_e,
+                           obs_stride_r, obs_index, mute, residual, workspace, blocks, misfit,
+                           as_stream(stream));
+}
+
+int swfd_residual_noalloc(int32_t n_steps, int32_t n_shots, int32_t n_rec, const float *samples,
+                          const float *observed, int64_t obs_stride_e, int64_t obs_stride_r,
+                          const int32_t *obs_index, const int32_t *mute, float *residual,
+                          float *misfit, void *stream)
+{
+    int rc = check_residual_args(n_steps, n_shots, n_rec, samples, observed, obs_index, mute,
+                                 residual, misfit);
+    if (rc) return rc;
+    const size_t tiles = swfd_residual_workspace_floats(n_steps, n_shots, n_rec);
+    if (tiles > (size_t)SWFD_RESIDUAL_SCRATCH_FLOATS)
+        return fail(SWFD_EINVAL,
+                    "residual (no allocation): " + std::to_string(tiles) +
+                        " partial sums needed, the built-in scratch holds " +
+                        std::to_string((size_t)SWFD_RESIDUAL_SCRATCH_FLOATS) +
+                        " (64 x 64 tiles of steps x receivers per shot); use swfd_residual "
+                        "with a workspace");
+    float *partials = nullptr;
+    if ((rc = scratch_on_current_device(&partials))) return rc;
+    return launch_residual(n_steps, n_shots, n_rec, samples, observed, obs_stride_e,
+                           obs_stride_r, obs_index, mute, residual, partials, tiles, misfit,
+                           as_stream(stream));
 }
 
 }  // extern "C"

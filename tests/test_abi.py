@@ -122,3 +122,18 @@ def test_footprint_rows_validation_and_wide_grids():
     assert b"outside the grid" in _native.load_library().swfd_last_error()
     rc, _ = footprint_rows(64, 64, [5], [-1])
     assert rc == _native.SWFD_EINVAL
+
+
+def test_noalloc_residual_rejects_oversized_runs_on_the_host():
+    """swfd_residual_noalloc checks its built-in scratch size before any device access."""
+    import ctypes
+
+    lib = _native.load_library()
+    fake = ctypes.c_void_p(256)  # never dereferenced: the size check comes first
+    n_steps, n_shots, n_rec = 64 * 1000, 300, 256
+    assert lib.swfd_residual_workspace_floats(n_steps, n_shots, n_rec) > \
+        _native.RESIDUAL_SCRATCH_FLOATS
+    rc = lib.swfd_residual_noalloc(n_steps, n_shots, n_rec, fake, fake, 1, 1, fake, fake, fake,
+                                   fake, None)
+    assert rc == _native.SWFD_EINVAL
+    assert "built-in scratch" in lib.swfd_last_error().decode()
