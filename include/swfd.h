@@ -223,6 +223,18 @@ int swfd_scatter(const swfd_grid *grid, const swfd_receivers *rec, int32_t n_sho
 int swfd_laplacian(const swfd_grid *grid, int32_t n_fields, const float *fields, float *out,
                    void *stream);
 
+/* Reduced-precision wavefield histories (store_every > 1 with wave_dtype
+ * float16 / bfloat16, invert_ring.py:443-453): the reference casts each
+ * stored field into wave_dtype (wave[k].copy_(fields), :450-451) and back to
+ * float32 before imaging (u_fwd.to(dtype), torch_backend.py:595).
+ * direction 0 packs n float32 values of src into n 16-bit codes at dst,
+ * direction 1 unpacks codes into float32; type SWFD_HISTORY_F16 (IEEE
+ * binary16) or SWFD_HISTORY_BF16, both rounded to nearest even. */
+#define SWFD_HISTORY_F16 1
+#define SWFD_HISTORY_BF16 2
+int swfd_history_convert(size_t n, int32_t direction, int32_t type, const void *src, void *dst,
+                         void *stream);
+
 /* Multi-step forward run; see swfd_forward_args. */
 int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_receivers *rec,
                  const swfd_forward_args *args, void *stream);

@@ -77,8 +77,12 @@ EXPORTS = (
     "swfd_forward_batch_prepare", "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
     "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
     "swfd_lap_history", "swfd_steps_per_launch", "swfd_footprint_rows",
-    "swfd_residual_noalloc",
+    "swfd_residual_noalloc", "swfd_history_convert",
 )
+
+#: storage codes of swfd_history_convert (include/swfd.h)
+HISTORY_F16 = 1
+HISTORY_BF16 = 2
 
 #: partial sums the built-in scratch of swfd_residual_noalloc holds (include/swfd.h)
 RESIDUAL_SCRATCH_FLOATS = 1 << 20
@@ -122,6 +126,7 @@ def load_library(path=LIBRARY):
                                   ctypes.c_int64, vp, vp, vp, vp, vp, vp]
     lib.swfd_residual_noalloc.argtypes = [_c_int, _c_int, _c_int, vp, vp, ctypes.c_int64,
                                           ctypes.c_int64, vp, vp, vp, vp, vp]
+    lib.swfd_history_convert.argtypes = [ctypes.c_size_t, _c_int, _c_int, vp, vp, vp]
     size_p = P(ctypes.c_size_t)
     for kind, args_t in (("forward", ForwardArgs), ("adjoint", AdjointArgs)):
         getattr(lib, f"swfd_{kind}_batch_table_bytes").argtypes = [
@@ -139,7 +144,7 @@ def load_library(path=LIBRARY):
     lib.swfd_steps_per_launch.argtypes = [_c_int, _c_int, _c_int, _c_int]
     for name in ("swfd_prepare_medium", "swfd_step", "swfd_gather", "swfd_scatter",
                  "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
-                 "swfd_residual_noalloc",
+                 "swfd_residual_noalloc", "swfd_history_convert",
                  "swfd_forward_batch_table_bytes", "swfd_forward_batch_prepare",
                  "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
                  "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune",

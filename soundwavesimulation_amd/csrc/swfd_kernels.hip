@@ -2171,6 +2171,64 @@ __global__ void fill_kernel(float *p, size_t n, float v)
     if (i < n) p[i] = v;
 }
 
+// Reduced-precision history storage: float32 <-> 16-bit codes, both rounded
+// to nearest even. float16 goes through the hardware conversion
+// (v_cvt_f16_f32, IEEE binary16 with subnormals); bfloat16 keeps the upper
+// half of the float32 bits after adding the round-to-nearest-even bias.
+__device__ __forceinline__ uint16_t f32_to_code(float v, int type)
+{
+    if (type == SWFD_HISTORY_F16) {
+        const _Float16 h = (_Float16)v;
+        return __builtin_bit_cast(uint16_t, h);
+    }
+    const uint32_t b = __float_as_uint(v);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);  // quiet NaN
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float code_to_f32(uint16_t c, int type)
+{
+    if (type == SWFD_HISTORY_F16) return (float)__builtin_bit_cast(_Float16, c);
+    return __uint_as_float((uint32_t)c << 16);
+}
+
+// Eight values per lane when every pointer is 16-byte aligned and n % 8 == 0
+// (two float4 <-> one uint4), else one per lane.
+template <bool kPack, int kVec>
+__global__ void __launch_bounds__(256) history_convert_kernel(size_t n, const void *src,
+                                                              void *dst, int type)
+{
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * kVec;
+    if (i >= n) return;
+    if constexpr (kVec == 8) {
+        if constexpr (kPack) {
+            const float4 a = reinterpret_cast<const float4 *>(src)[i / 4];
+            const float4 b = reinterpret_cast<const float4 *>(src)[i / 4 + 1];
+            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            uint32_t w[4];
+            for (int k = 0; k < 4; ++k)
+                w[k] = (uint32_t)f32_to_code(v[2 * k], type) |
+                       ((uint32_t)f32_to_code(v[2 * k + 1], type) << 16);
+            reinterpret_cast<uint4 *>(dst)[i / 8] = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            const uint4 c = reinterpret_cast<const uint4 *>(src)[i / 8];
+            const uint32_t w[4] = {c.x, c.y, c.z, c.w};
+            float v[8];
+            for (int k = 0; k < 4; ++k) {
+                v[2 * k] = code_to_f32((uint16_t)(w[k] & 0xffffu), type);
+                v[2 * k + 1] = code_to_f32((uint16_t)(w[k] >> 16), type);
+            }
+            reinterpret_cast<float4 *>(dst)[i / 4] = make_float4(v[0], v[1], v[2], v[3]);
+            reinterpret_cast<float4 *>(dst)[i / 4 + 1] = make_float4(v[4], v[5], v[6], v[7]);
+        }
+    } else {
+        if constexpr (kPack)
+            static_cast<uint16_t *>(dst)[i] = f32_to_code(static_cast<const float *>(src)[i], type);
+        else
+            static_cast<float *>(dst)[i] = code_to_f32(static_cast<const uint16_t *>(src)[i], type);
+    }
+}
+
 __global__ void scatter_kernel(int n_shots, size_t plane, int n_entries, int n_rec,
                                const float *resid, const int *entry_cell,
                                const int *entry_start, const int *pair_rec,
@@ -4014,6 +4072,33 @@ int swfd_laplacian(const swfd_grid *grid, int32_t n_fields, const float *fields,
     hipLaunchKernelGGL(laplacian_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        as_stream(stream), grid->ny, grid->nx, total, grid->inv_dx2,
                        grid->inv_dy2, grid->center, fields, out);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+int swfd_history_convert(size_t n, int32_t direction, int32_t type, const void *src, void *dst,
+                         void *stream)
+{
+    if (type != SWFD_HISTORY_F16 && type != SWFD_HISTORY_BF16)
+        return fail(SWFD_EINVAL, "history convert: type must be float16 or bfloat16");
+    if (direction != 0 && direction != 1)
+        return fail(SWFD_EINVAL, "history convert: direction must be 0 (pack) or 1 (unpack)");
+    if (n == 0) return SWFD_OK;
+    if (src == nullptr || dst == nullptr || src == dst)
+        return fail(SWFD_EINVAL, "history convert: bad src / dst");
+    const bool vec = n % 8 == 0 && ((uintptr_t)src % 16) == 0 && ((uintptr_t)dst % 16) == 0;
+    const size_t lanes = vec ? n / 8 : n;
+    const size_t blocks = (lanes + 255) / 256;
+    if (blocks > 0x7fffffffu) return fail(SWFD_EINVAL, "history convert: too many values");
+    hipStream_t st = as_stream(stream);
+    const dim3 g((unsigned)blocks), b(256);
+    if (direction == 0) {
+        if (vec) hipLaunchKernelGGL((history_convert_kernel<true, 8>), g, b, 0, st, n, src, dst, type);
+        else hipLaunchKernelGGL((history_convert_kernel<true, 1>), g, b, 0, st, n, src, dst, type);
+    } else {
+        if (vec) hipLaunchKernelGGL((history_convert_kernel<false, 8>), g, b, 0, st, n, src, dst, type);
+        else hipLaunchKernelGGL((history_convert_kernel<false, 1>), g, b, 0, st, n, src, dst, type);
+    }
     SWFD_HIP_CHECK(hipGetLastError());
     return SWFD_OK;
 }

@@ -24,8 +24,9 @@ import numpy as np
 import torch
 
 from .. import _native
-from ..torch_backend import (TorchFDTD2DBatch, _source_values_tensor, adjoint_run,
-                             adjoint_workspace, auto_batch_size, forward_run)
+from ..torch_backend import (TorchFDTD2DBatch, _history_code, _source_values_tensor,
+                             adjoint_run, adjoint_workspace, auto_batch_size, convert_history,
+                             forward_run)
 from .medium import c_from_m, m_from_c
 from .misfit import tikhonov, total_variation
 
@@ -53,19 +54,31 @@ def default_checkpoint_every(n_steps):
     return max(2, int(math.ceil(k / 4.0)) * 4 if k >= 6.0 else int(round(k)))
 
 
+#: float32 staging slots of a reduced-precision history (forward chunks are
+#: packed from them, adjoint chunks unpacked into them)
+HISTORY_STAGING_SLOTS = 4
+
+
 def gradient_bytes_per_shot(n_steps, grid_shape, n_receivers, store_every=1, wave_dtype=None,
                             dtype=_F32, checkpoint_every=None):
     """Device memory one gradient shot needs.
 
     ``store_every == 1`` with full-precision history: checkpointed plan,
-    ``(8 + 2 (n_seg - 1) + K + 1)`` fields plus samples and residuals.
-    Otherwise the reference's stored-history estimate (invert_ring.py:463-473).
+    ``(9 + 2 (n_seg - 1) + K + 1)`` fields (4 forward and 4 adjoint work
+    fields, the shot-group images -- one field per shot for 8 to 32 shots,
+    fewer above, 8 fields in all below -- checkpoints and the segment history)
+    plus samples and residuals.
+    Otherwise the stored history in ``wave_dtype`` (2 B per cell for float16 /
+    bfloat16, held in that dtype) plus 12 float32 fields (6 forward work
+    fields and the staging slots, or 4 adjoint fields, the staging slots and
+    the shot-group images) and four (N, R) trace arrays, as the reference
+    prices it (invert_ring.py:463-473).
     """
     cells = grid_shape[0] * grid_shape[1]
     if int(store_every) == 1 and wave_dtype in (None, _F32):
         K = checkpoint_every or default_checkpoint_every(n_steps)
         _, n_ckpt, n_hist = segment_plan(n_steps, K)
-        return (8 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
+        return (9 + n_ckpt + n_hist) * cells * 4 + 3 * n_steps * n_receivers * 4
     wave_bytes = torch.empty((), dtype=wave_dtype or _F32).element_size()
     return (stored_steps(n_steps, store_every) * cells * wave_bytes + 12 * cells * 4
             + 4 * n_steps * n_receivers * 4)
@@ -230,15 +243,30 @@ def forward_store(solver, sampler, array, pulse, tx, n_steps):
     return samples[:, 0, :].transpose(0, 1).contiguous(), wave[:, 0]
 
 
+def _history_chunks(n_steps, store_every, slots):
+    """``(first_slot, n_slots, step_begin, step_end)`` of each staging chunk:
+    chunk c holds the fields after steps ``step_begin, step_begin + store_every,
+    ...`` (``slots`` of them at most) and the forward advances it over
+    ``[step_begin, step_end)``."""
+    span = slots * store_every
+    for c in range(-(-n_steps // span)):
+        begin = c * span
+        end = min(n_steps, begin + span)
+        yield c * slots, -(-(end - begin) // store_every), begin, end
+
+
 def forward_store_batch(solver, sampler, array, source_values, sources, n_steps, batched,
                         store_every=1, wave_dtype=None):
     """Batched forward keeping every ``store_every``-th field (invert_ring.py:422-455).
 
-    Returns traces (S, R, N) and the stored wavefields (stored, S, ny, nx).
-    Reduced-precision ``wave_dtype`` storage is reproduced by rounding the
-    float32 history through that dtype.
+    Returns traces (S, R, N) and the stored wavefields (stored, S, ny, nx) in
+    ``wave_dtype``. A float16 / bfloat16 history is held in that dtype: the
+    run advances in chunks whose stored fields land in a few float32 staging
+    slots and are packed from there (rounded to nearest even, as the
+    reference's cast is).
     """
     n_steps = int(n_steps)
+    store_every = int(store_every)
     rows_cols = [array.inject_rows_cols(int(tx)) for tx in sources]
     rows = np.asarray([r for r, _ in rows_cols], dtype=np.int64)
     cols = np.asarray([c for _, c in rows_cols], dtype=np.int64)
@@ -246,15 +274,29 @@ def forward_store_batch(solver, sampler, array, source_values, sources, n_steps,
     dev = solver.model.device
     values = _source_values_tensor(None, n_steps, dev, source_values)
     S = len(sources)
+    shape = (S, *solver.model.shape)
     samples = torch.empty((n_steps, S, sampler.n_elements), device=dev, dtype=_F32)
-    wave = torch.empty((stored_steps(n_steps, store_every), S, *solver.model.shape),
-                       device=dev, dtype=_F32)
-    u_old, u = forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, n_steps,
-                           samples=samples, sampler=sampler, hist=wave, hist_first=0,
-                           hist_every=int(store_every))
+    code = _history_code(wave_dtype)
+    wave = torch.empty((stored_steps(n_steps, store_every), *shape), device=dev,
+                       dtype=_F32 if code is None else wave_dtype)
+    if code is None:
+        u_old, u = forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, n_steps,
+                               samples=samples, sampler=sampler, hist=wave, hist_first=0,
+                               hist_every=store_every)
+    else:
+        work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(6)]
+        staging = torch.empty((HISTORY_STAGING_SLOTS, *shape), device=dev, dtype=_F32)
+        work[0].zero_()
+        work[1].zero_()
+        for slot0, n_slots, begin, end in _history_chunks(n_steps, store_every,
+                                                         HISTORY_STAGING_SLOTS):
+            forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, end - begin,
+                        init=(work[0], work[1]), samples=samples, sampler=sampler,
+                        hist=staging, hist_first=begin, hist_every=store_every,
+                        step_begin=begin, work=work)
+            convert_history(staging[:n_slots], wave[slot0:slot0 + n_slots])
+        u_old, u = work[0], work[1]
     batched.u_old, batched.u = u_old, u
-    if wave_dtype not in (None, _F32):
-        wave = wave.to(wave_dtype).to(_F32)
     return samples.permute(1, 2, 0).contiguous(), wave
 
 
@@ -446,22 +488,25 @@ class TorchLeastSquaresFWI:
         return value + tv_value, gradient + tv_gradient
 
     def _adjoint_accumulate(self, wave, residual, g_c2):
-        """Unbatched adjoint of one shot through the single-step API (invert_ring.py:668-681)."""
+        """Unbatched adjoint of one shot through the single-step API (invert_ring.py:668-681):
+        per step the bilinear scatter of ``residual[:, n]``, an adjoint step with that
+        dense source and, for n > 0, ``g_c2 += dt^2 * lam * L[wave[n - 1]]`` in place."""
         self.solver.reset()
         dt2 = self.solver.dt2
         for n in range(self.n_steps - 1, -1, -1):
             source = self.sampler.record_adjoint(residual[:, n])
-            adjoint_field = self.solver.inject_field_and_step(source, adjoint=True)
             if n == 0:
-                continue
-            laplacian = self.model.laplacian(wave[n - 1])
-            g_c2.add_(adjoint_field * laplacian * dt2)
+                self.solver.inject_field_and_step(source, adjoint=True)
+            else:
+                self.solver._adjoint_image_step(source, wave[n - 1], g_c2, dt2)
 
     def _adjoint_accumulate_batch(self, batched, wave, residual, g_c2, residual_layout="srn"):
         """Adjoint over a stored history (invert_ring.py:683-701); adds into ``g_c2``.
 
         ``residual`` is (S, R, N) as the reference passes it, or (N, S, R) with
-        ``residual_layout="nsr"``.
+        ``residual_layout="nsr"``. A float16 / bfloat16 history is unpacked a few
+        stored fields at a time into float32 staging slots, newest first, and
+        the adjoint runs over the steps that image with them.
         """
         S = batched.n_shots
         residual_nsr = (residual.permute(2, 0, 1).contiguous() if residual_layout == "srn"
@@ -469,9 +514,29 @@ class TorchLeastSquaresFWI:
         dev = self.model.device
         shape = (S, *self.model.shape)
         lam = [torch.zeros(shape, device=dev, dtype=_F32) for _ in range(4)]
-        adjoint_run(self.solver, self.sampler, S, residual_nsr, self.n_steps, 0,
-                    state=(lam[0], lam[1]), work=lam, hist=wave.contiguous(), hist_first=0,
-                    hist_every=self.store_every, image_scale=self.store_every * batched.dt2,
-                    grad=g_c2)
+        state = (lam[0], lam[1])
+        scale = self.store_every * batched.dt2
+        wave = wave.contiguous()
+        if _history_code(wave.dtype) is None:
+            adjoint_run(self.solver, self.sampler, S, residual_nsr, self.n_steps, 0,
+                        state=state, work=lam, hist=wave, hist_first=0,
+                        hist_every=self.store_every, image_scale=scale, grad=g_c2)
+        else:
+            staging = torch.empty((HISTORY_STAGING_SLOTS, *shape), device=dev, dtype=_F32)
+            workspace = adjoint_workspace(self.solver, S)
+            chunks = list(_history_chunks(self.n_steps, self.store_every,
+                                          HISTORY_STAGING_SLOTS))
+            for slot0, n_slots, begin, end in reversed(chunks):
+                # steps begin + 1 .. end image with this chunk's fields (step n
+                # images the field after step n - 1; step 0 images nothing)
+                step_hi = min(self.n_steps, end + 1)
+                step_lo = begin + 1 if begin > 0 else 0
+                if step_hi <= step_lo:
+                    continue
+                convert_history(wave[slot0:slot0 + n_slots], staging[:n_slots])
+                adjoint_run(self.solver, self.sampler, S, residual_nsr, step_hi, step_lo,
+                            state=state, work=lam, hist=staging, hist_first=begin,
+                            hist_every=self.store_every, image_scale=scale, grad=g_c2,
+                            workspace=workspace)
         batched.u_old, batched.u = lam[0], lam[1]
         return g_c2

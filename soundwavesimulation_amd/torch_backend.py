@@ -313,6 +313,33 @@ class TorchFDTD2D:
             raise ValueError("Dense source field must match the FDTD grid shape.")
         return self._step(None, source_field.contiguous(), 2, adjoint=adjoint)
 
+    def _adjoint_image_step(self, source_field, forward_field, gradient, scale):
+        """Adjoint step with a dense source, then ``gradient += scale * u' * L[forward_field]``
+        in place in one native call (the reference's ``inject_field_and_step`` +
+        ``laplacian`` + ``addcmul_``, invert_ring.py:668-681); returns the new field."""
+        self._sync_medium()
+        if tuple(source_field.shape) != self.model.shape or \
+                tuple(forward_field.shape) != self.model.shape:
+            raise ValueError("Dense source field must match the FDTD grid shape.")
+        if gradient.dtype != _F32 or not gradient.is_contiguous() or \
+                tuple(gradient.shape) != self.model.shape:
+            raise ValueError("gradient must be a contiguous float32 (ny, nx) tensor")
+        fwd = forward_field
+        if fwd.dtype != _F32:
+            fwd = convert_history(fwd.contiguous(), torch.empty(fwd.shape, device=fwd.device,
+                                                                dtype=_F32))
+        out = torch.empty_like(self.u)
+        args = _native.StepArgs(n_shots=1, adjoint=1, u_old=_native.ptr(self.u_old),
+                                u=_native.ptr(self.u), u_next=_native.ptr(out),
+                                dense=_native.ptr(source_field.contiguous()),
+                                image_field=_native.ptr(fwd.contiguous()),
+                                grad_in=_native.ptr(gradient), grad_out=_native.ptr(gradient),
+                                image_scale=float(scale))
+        _native.check(_native.lib().swfd_step(self._grid, self._medium, None, args,
+                                              _native.stream_of(self.model.device)))
+        self.u_old, self.u = self.u, out
+        return self.u
+
 
 # ---------------------------------------------------------------------------
 # receivers
@@ -454,6 +481,37 @@ class TorchRingSampler:
 # ---------------------------------------------------------------------------
 # multi-step runs
 # ---------------------------------------------------------------------------
+
+def _history_code(wave_dtype):
+    """Storage code of a reduced-precision history dtype, None for float32."""
+    if wave_dtype in (None, _F32):
+        return None
+    if wave_dtype == torch.float16:
+        return _native.HISTORY_F16
+    if wave_dtype == torch.bfloat16:
+        return _native.HISTORY_BF16
+    raise ValueError("wave_dtype must be float32, float16 or bfloat16 on the device.")
+
+
+def convert_history(src, dst):
+    """Pack float32 fields into a float16 / bfloat16 history or unpack them back
+    (the reference's ``wave[k].copy_(fields)`` and ``u_fwd.to(dtype)``), rounding
+    to nearest even, on the device."""
+    if src.numel() != dst.numel() or not src.is_contiguous() or not dst.is_contiguous():
+        raise ValueError("history conversion needs contiguous tensors of one size")
+    if src.dtype == _F32:
+        direction, code = 0, _history_code(dst.dtype)
+    else:
+        direction, code = 1, _history_code(src.dtype)
+        if dst.dtype != _F32:
+            raise ValueError("history conversion unpacks into float32")
+    if code is None:
+        raise ValueError("history conversion needs a float16 or bfloat16 side")
+    _native.check(_native.lib().swfd_history_convert(
+        src.numel(), direction, code, _native.ptr(src), _native.ptr(dst),
+        _native.stream_of(src.device)))
+    return dst
+
 
 def _source_values_tensor(pulse, n_steps, device, source_values=None):
     if source_values is None:
@@ -673,9 +731,11 @@ class TorchFDTD2DBatch:
         """Adjoint step plus ``scale * sum(adjoint * L[forward_field])``; returns a new gradient."""
         args, out = self._new_step_args(True)
         residual = self._residual_tensor(residual, sampler)
-        fwd = torch.as_tensor(forward_field, device=self.model.device).to(_F32).contiguous()
+        fwd = torch.as_tensor(forward_field, device=self.model.device).contiguous()
         if tuple(fwd.shape) != tuple(self.u.shape):
             raise ValueError("forward_field must have shape (shots, ny, nx).")
+        if fwd.dtype != _F32:  # a reduced-precision history (u_fwd.to(dtype) in the reference)
+            fwd = convert_history(fwd, torch.empty(fwd.shape, device=fwd.device, dtype=_F32))
         g_in = torch.as_tensor(gradient, device=self.model.device, dtype=_F32).contiguous()
         g_out = torch.empty_like(g_in)
         args.residual = _native.ptr(residual)
