@@ -37,8 +37,17 @@ def save_traces(path, traces, time_s, array, tx, pulse, dt, c_map=None, device="
 
 
 def load_traces(path):
+    """Every array of a traces archive. An entry stored as a Python object (the
+    reference writes ``sound_speed_m_s=None`` when no map is given) would need
+    unpickling, so it is left out rather than loaded."""
+    out = {}
     with np.load(path, allow_pickle=False) as f:
-        return {k: f[k] for k in f.files if f[k].dtype != object}
+        for k in f.files:
+            try:
+                out[k] = f[k]
+            except ValueError:  # object array: refused without pickle
+                continue
+    return out
 
 
 def save_model(path, c_true, c_est, history, x_m, y_m, array, mask, optimizer):

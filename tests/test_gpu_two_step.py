@@ -253,17 +253,26 @@ def _lap64(f, sp):
                   + pad[..., 2:, 1:-1] - 4.0 * f)
 
 
+@pytest.fixture(params=["wide2", "wide3", "wide4"])
+def wide_form(request):
+    """Full-width adjoint launch forms (Laplacian histories have no two-step
+    block form: the library rejects that pairing)."""
+    from soundwavesimulation_amd import _native
+
+    yield request.param
+    _native.lib().swfd_tune_clear()
+
+
 @pytest.mark.parametrize("ny,nx,K", [(64, 64, 10), (70, 129, 9), (130, 190, 11), (40, 1000, 7),
                                      (5, 7, 6)])
-def test_laplacian_history_matches_field_history(ny, nx, K, adj_form):
+def test_laplacian_history_matches_field_history(ny, nx, K, wide_form):
     """Laplacian histories (forward_run/adjoint_run hist_lap=True): the slots
     hold the zero-padded Laplacians of the field history, and the adjoint run
     that images with them gives the same state bits and gradient as the
     single-step adjoint imaging against the fields."""
     from soundwavesimulation_amd import _native
 
-    if adj_form == "blocks":
-        pytest.skip("Laplacian histories need full-width adjoint launches")
+    adj_form = wide_form
     if not _native.lib().swfd_lap_history(ny, nx):
         pytest.skip("no full-width launches for this grid")
     print(f"Laplacian history ({adj_form}) {ny}x{nx}, segment of {K} steps", flush=True)
