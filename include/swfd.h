@@ -171,6 +171,14 @@ int32_t swfd_lap_history(int32_t ny, int32_t nx);
  * counterpart in the reference; for profiling). */
 int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samples);
 
+/* Launch shape of the imaging adjoint runs of n_shots shots on an ny x nx
+ * grid given work3 (no counterpart in the reference; for profiling): out[0]
+ * steps per launch, out[1] 1 for full-width launches, and for those out[2]
+ * rows per segment, out[3] segments, out[4] shot groups, out[5] shots per
+ * group (zeros otherwise). Host only, but the shape depends on the current
+ * device's compute-unit count. */
+int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *out);
+
 /* Rows and full-width wavefronts that hold receiver footprint cells
  * (host-only, no launch; host pointers; no counterpart in the reference):
  * out[y] (ny entries, overwritten) gets bit w when row y has a footprint cell

@@ -77,7 +77,7 @@ EXPORTS = (
     "swfd_forward_batch_prepare", "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
     "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
     "swfd_lap_history", "swfd_steps_per_launch", "swfd_footprint_rows",
-    "swfd_residual_noalloc", "swfd_history_convert",
+    "swfd_residual_noalloc", "swfd_history_convert", "swfd_adjoint_launch_shape",
 )
 
 #: storage codes of swfd_history_convert (include/swfd.h)
@@ -142,9 +142,10 @@ def load_library(path=LIBRARY):
     lib.swfd_lap_history.argtypes = [_c_int, _c_int]
     lib.swfd_steps_per_launch.restype = _c_int
     lib.swfd_steps_per_launch.argtypes = [_c_int, _c_int, _c_int, _c_int]
+    lib.swfd_adjoint_launch_shape.argtypes = [_c_int, _c_int, _c_int, vp]
     for name in ("swfd_prepare_medium", "swfd_step", "swfd_gather", "swfd_scatter",
                  "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
-                 "swfd_residual_noalloc", "swfd_history_convert",
+                 "swfd_residual_noalloc", "swfd_history_convert", "swfd_adjoint_launch_shape",
                  "swfd_forward_batch_table_bytes", "swfd_forward_batch_prepare",
                  "swfd_forward_batch", "swfd_adjoint_batch_table_bytes",
                  "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune",
@@ -193,3 +194,13 @@ def require_device(device):
     load_library()
     return torch.device("cuda", torch.cuda.current_device() if device.index is None
                         else device.index)
+
+
+def adjoint_launch_shape(ny, nx, n_shots):
+    """``{"steps_per_launch", "full_width", "segment_rows", "segments", "shot_groups",
+    "shots_per_group"}`` of the imaging adjoint runs (needs the device)."""
+    out = (ctypes.c_int32 * 6)()
+    check(lib().swfd_adjoint_launch_shape(ny, nx, n_shots, out))
+    keys = ("steps_per_launch", "full_width", "segment_rows", "segments", "shot_groups",
+            "shots_per_group")
+    return dict(zip(keys, (int(v) for v in out)))

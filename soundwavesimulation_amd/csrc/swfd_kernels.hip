@@ -4220,6 +4220,29 @@ int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samp
     return levels;
 }
 
+int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *out)
+{
+    if (ny < 3 || nx < 3 || n_shots < 1 || out == nullptr)
+        return fail(SWFD_EINVAL, "adjoint launch shape: bad arguments");
+    static const int32_t any_map = 0;
+    swfd_receivers rec{};
+    rec.cell_map = &any_map;
+    static const float any_buffer = 0.0f;
+    float *w3 = const_cast<float *>(&any_buffer);
+    const ForwardPlan plan = adjoint_plan(ny, nx, w3, &rec);
+    for (int i = 0; i < 6; ++i) out[i] = 0;
+    out[0] = plan.levels;
+    out[1] = plan.wide ? 1 : 0;
+    if (plan.wide && plan.levels >= 2) {
+        const AdjwShape a = adjw_shape(ny, nx, n_shots);
+        out[2] = a.acc_rows;
+        out[3] = a.n_seg;
+        out[4] = a.n_groups;
+        out[5] = a.shots_per_group;
+    }
+    return SWFD_OK;
+}
+
 int32_t swfd_lap_history(int32_t ny, int32_t nx)
 {
     if (ny < 3 || nx < 3) return 0;
