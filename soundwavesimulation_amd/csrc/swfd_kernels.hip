@@ -1760,6 +1760,10 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     // with global loads off a scalar base (2 SGPRs each instead of a 4-SGPR
     // buffer descriptor: the adjoint is short of SGPRs)
     const float *emap = (const float *)p.cell_map;
+    // rows where this wavefront holds no receiver footprint cell (the
+    // footprint table's bit `wave` clear) skip the map: one scalar load per row
+    // instead of a vector load per lane (a ring crosses 1-4 of 16 wavefronts)
+    const kci *fpr = (const kci *)p.fp_rows;
     const float *mur = p.mur;
     const int mo = (x == 0) ? 0 : H * 4;
     auto mrow = [&](int yy) -> int {
@@ -1832,7 +1836,9 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
             qb[i] = bload(rb, xo, row_off(y + 1));
             qa[i] = bload(ra, xo, row_off(y));
             qk[i] = bload(rk, xo, row_off(y + 1));
-            qe[i] = gload_off(emap, xo + row_off(y));
+            qe[i] = __int_as_float(-1);
+            if (fpr == nullptr || (((unsigned)fpr[mrow(y) / 4] >> wave) & 1u) != 0u)
+                qe[i] = gload_off(emap, xo + row_off(y));
             qm[i] = 0.0f;
             if constexpr (XE) qm[i] = gload_off(mur, mo + mrow(y));
 #pragma unroll
@@ -3491,6 +3497,7 @@ struct AdjointRun {
         fill = false;
         base_params(p, grid, medium, a->n_shots);
         attach_receivers(p, rec);
+        p.fp_rows = rec->footprint_rows;  // rows without receiver cells skip the map
         p.f_old = p_old;
         p.f_cur = p_cur;
         p.resid = a->residual + (size_t)n * sr;
