@@ -140,12 +140,24 @@ class BatchedAdjoint(_Prepared):
 
 
 class GradientProblem:
-    """One FWI gradient problem of an uneven batch (all ring elements fire)."""
+    """One FWI gradient problem of an uneven batch.
 
-    def __init__(self, solver, sampler, array, pulse, n_steps, observed, checkpoint_every=None):
+    Every ring element fires unless ``sources`` names the transmitting
+    elements (sorted, as the reference's shot sampling keeps them,
+    ``invert_ring.py:579-590``); ``observed`` holds one gather per element
+    ``(n_elements, R, N)``, and each shot's own receiver channel is muted.
+    """
+
+    def __init__(self, solver, sampler, array, pulse, n_steps, observed, checkpoint_every=None,
+                 sources=None):
         self.solver, self.sampler, self.array = solver, sampler, array
         dev = solver.model.device
-        self.S = array.n_elements
+        self.sources = (np.arange(array.n_elements) if sources is None
+                        else np.asarray(sources, dtype=np.int64))
+        if self.sources.ndim != 1 or self.sources.size < 1 or \
+                self.sources.min() < 0 or self.sources.max() >= array.n_elements:
+            raise ValueError("sources must name ring elements.")
+        self.S = int(self.sources.size)
         self.N = int(n_steps)
         self.K = int(checkpoint_every or default_checkpoint_every(self.N))
         self.n_seg, n_ckpt, n_hist = segment_plan(self.N, self.K)
@@ -166,10 +178,10 @@ class GradientProblem:
         self.res_ws = e(int(_native.lib().swfd_residual_workspace_floats(self.N, self.S, R)) + 1)
         self.misfit = e(1)
         self.grad = torch.zeros(solver.model.shape, device=dev, dtype=_F32)
-        rc = [array.inject_rows_cols(i) for i in range(self.S)]
+        rc = [array.inject_rows_cols(int(i)) for i in self.sources]
         self.rows = torch.as_tensor(np.array([r for r, _ in rc], dtype=np.int32), device=dev)
         self.cols = torch.as_tensor(np.array([c for _, c in rc], dtype=np.int32), device=dev)
-        self.index = torch.arange(self.S, dtype=torch.int32, device=dev)
+        self.index = torch.as_tensor(self.sources.astype(np.int32), device=dev)
         self.values = _source_values_tensor(torch.as_tensor(pulse), self.N, dev)
         self.observed = observed.contiguous()
         self.cells = 2.0 * ny * nx * self.N * self.S
@@ -271,4 +283,98 @@ class UnevenGradients:
         return sum(p.cells for p in self.problems)
 
 
-__all__ = ["BatchedForward", "BatchedAdjoint", "GradientProblem", "UnevenGradients"]
+def gradient_problem_bytes(ny, nx, n_shots, n_receivers, n_steps, checkpoint_every=None):
+    """Device bytes a ``GradientProblem`` allocates (work fields, checkpoints,
+    segment history, shot-group images, samples, residual)."""
+    K = int(checkpoint_every or default_checkpoint_every(n_steps))
+    _, n_ckpt, n_hist = segment_plan(n_steps, K)
+    plane = ny * nx
+    fields = 4 + (max(n_ckpt, 2) if n_ckpt > 0 else 0) + n_hist
+    images = max(8, min(32, n_shots))
+    return 4 * (fields * n_shots * plane + images * plane + 2 * n_steps * n_shots * n_receivers)
+
+
+def plan_waves(sizes, budget_bytes, wave_cells):
+    """Group problems into waves that run one after the other.
+
+    ``sizes[i] = (work, bytes, cells_per_step)`` of problem i. Problems are taken
+    in order of increasing work (shortest first, which minimises the mean
+    completion time of a sequence) and a wave is closed when the next problem
+    would take its device memory past ``budget_bytes`` or its cells per step
+    (the parallel width of its launches) past ``wave_cells``. Returns lists of
+    problem indices. Raises if a single problem exceeds the budget.
+    """
+    order = sorted(range(len(sizes)), key=lambda i: (sizes[i][0], i))
+    waves, cur, mem, width = [], [], 0, 0
+    for i in order:
+        _, nbytes, cells = sizes[i]
+        if nbytes > budget_bytes:
+            raise ValueError(f"problem {i} needs {nbytes / 1e9:.1f} GB, more than the budget "
+                             f"of {budget_bytes / 1e9:.1f} GB")
+        if cur and (mem + nbytes > budget_bytes or width + cells > wave_cells):
+            waves.append(cur)
+            cur, mem, width = [], 0, 0
+        cur.append(i)
+        mem += nbytes
+        width += cells
+    if cur:
+        waves.append(cur)
+    return waves
+
+
+class WaveSchedule:
+    """Uneven gradients in memory-bounded waves, shortest work first.
+
+    ``make_problem(i)`` builds problem i (a ``GradientProblem``; its device
+    buffers are allocated then and released after its wave). Each wave is one
+    ``UnevenGradients`` batch; waves run back to back on the current stream.
+    ``run`` returns the gradients and misfits (host copies) of every problem.
+    With ``events`` it records an event after each problem's last launch, and
+    ``spans`` (a list) receives a (start, end) event pair around each wave's
+    launches. ``graph=True`` runs each wave once eagerly on a side stream,
+    captures it as a HIP graph and replays it (the span covers the replay
+    only; results equal the eager run).
+    """
+
+    def __init__(self, make_problem, sizes, budget_bytes, wave_cells):
+        self.make_problem = make_problem
+        self.sizes = list(sizes)
+        self.waves = plan_waves(self.sizes, budget_bytes, wave_cells)
+
+    def run(self, events=None, graph=False, spans=None):
+        results = {}
+        for wave in self.waves:
+            problems = [self.make_problem(i) for i in wave]
+            ug = UnevenGradients(problems)
+            dev = problems[0].solver.model.device
+            span = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            if graph:
+                side = torch.cuda.Stream(dev)
+                side.wait_stream(torch.cuda.current_stream(dev))
+                with torch.cuda.stream(side):  # warm-up on the capture stream
+                    ug.run()
+                torch.cuda.current_stream(dev).wait_stream(side)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    ug.run()
+                span[0].record()
+                g.replay()
+                span[1].record()
+                if events is not None:  # the graph replays as a whole
+                    for i in wave:
+                        events[i].record()
+            else:
+                span[0].record()
+                ug.run(None if events is None else {k: events[i] for k, i in enumerate(wave)})
+                span[1].record()
+            if spans is not None:
+                spans.append(span)
+            for k, i in enumerate(wave):
+                p = problems[k]
+                results[i] = (p.grad.cpu(), float(p.misfit.cpu()))
+            del ug, problems
+        return results
+
+
+__all__ = ["BatchedForward", "BatchedAdjoint", "GradientProblem", "UnevenGradients",
+           "WaveSchedule", "gradient_problem_bytes", "plan_waves"]

@@ -6,16 +6,18 @@ grid from {256^2, 384^2, 512^2, 768^2} at 0.5 mm, a ring of 16-128 elements
 1400-1650 m/s) and a 300 kHz Ricker -- simulates their observed data, tunes
 the launch shapes of every grid size on the device (``autotune.tune``), and
 computes all 24 checkpointed gradients (from a homogeneous 1500 m/s start)
-with ``batch.UnevenGradients``: at every tick one batched launch advances
-every problem, and the whole sequence is captured once as a graph and
-replayed.
+with ``batch.WaveSchedule``: problems in order of increasing work, grouped
+into waves whose device memory stays within the budget (``--budget-gb``, so
+any draw runs) and whose cells per step stay within ``--wave-cells``; each
+wave is one ``UnevenGradients`` batch (one batched launch per kernel variant
+and tick), waves back to back.
 
-Reported (one JSON line): throughput in cell-updates per second over the
-graph replays (2 ny nx N S per problem, summed), and per-problem latency
-percentiles from an eager run with an event after each problem's last
-launch. The draw starts at seed 2 and moves to the next seed while the
-problems' device memory (checkpointed gradient: 4 + 2(n_seg - 1) + K + 1
-fields per shot plus traces) exceeds the budget (``--budget-gb``).
+Reported (one JSON line): throughput in cell-updates per second (2 ny nx N S
+per problem, summed) over the eager run and over per-wave graph replays, the
+makespan, and per-problem latency percentiles from the eager run (an event
+after each problem's last launch, measured from the start of the first wave;
+the set-up of a wave -- buffer allocation and table upload -- lies between
+waves and is included).
 """
 
 from __future__ import annotations
@@ -30,9 +32,8 @@ import torch
 
 from .array import RingArray
 from .autotune import tune
-from .batch import GradientProblem, UnevenGradients
+from .batch import GradientProblem, WaveSchedule, gradient_problem_bytes
 from .experiment import axis_centers, stable_dt
-from .inversion.fwi import default_checkpoint_every, segment_plan
 from .phantoms import ellipse_phantom
 from .sources import ricker
 from .torch_backend import TorchFDTD2D, TorchRingSampler, TorchScalarWave2D, simulate_shots_batch
@@ -44,11 +45,9 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def estimate_bytes(h, n_el, n_steps):
-    K = default_checkpoint_every(n_steps)
-    _, n_ckpt, n_hist = segment_plan(n_steps, K)
-    fields = 4 + n_ckpt + n_hist
-    return fields * n_el * h * h * 4 + 3 * n_steps * n_el * n_el * 4
+def work_of(spec):
+    h, n_el, n_steps = spec
+    return 2.0 * h * h * n_steps * n_el
 
 
 def draw(seed, n=24):
@@ -59,15 +58,8 @@ def draw(seed, n=24):
     return [(int(h), int(r), int(s)) for h, r, s in zip(H, R, N)]
 
 
-def draw_within(budget_bytes, seed=2, n=24):
-    while True:
-        probs = draw(seed, n)
-        if sum(estimate_bytes(*p) for p in probs) <= budget_bytes:
-            return seed, probs
-        seed += 1
-
-
-def build(spec, index, seed, device, spacing=0.5e-3, f_peak=300e3):
+def observed_data(spec, index, seed, device, spacing=0.5e-3, f_peak=300e3):
+    """Ring, pulse and observed gathers (n_el, n_el, N) of problem ``index``."""
     h, n_el, n_steps = spec
     x = axis_centers(h, spacing)
     radius = 0.43 * h * spacing
@@ -84,87 +76,115 @@ def build(spec, index, seed, device, spacing=0.5e-3, f_peak=300e3):
         tr, _ = simulate_shots_batch(truth, sampler, pulse.to(device), array.rows[sl],
                                      array.cols[sl], n_steps)
         observed[sl] = tr
-    del truth
-    solver = TorchFDTD2D(TorchScalarWave2D(np.full((h, h), 1500.0), spacing, device=device), dt)
-    return GradientProblem(solver, sampler, array, pulse, n_steps, observed)
+    return dict(array=array, sampler=sampler, pulse=pulse, dt=dt, observed=observed,
+                spacing=spacing)
+
+
+def problem_from(spec, data, device):
+    """The gradient problem of ``spec`` from a homogeneous 1500 m/s start."""
+    h, _, n_steps = spec
+    solver = TorchFDTD2D(TorchScalarWave2D(np.full((h, h), 1500.0), data["spacing"],
+                                           device=device), data["dt"])
+    return GradientProblem(solver, data["sampler"], data["array"], data["pulse"], n_steps,
+                           data["observed"])
+
+
+def sizes_of(specs):
+    return [(work_of(sp), gradient_problem_bytes(sp[0], sp[0], sp[1], sp[1], sp[2]),
+             sp[1] * sp[0] * sp[0]) for sp in specs]
 
 
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--problems", type=int, default=24)
-    p.add_argument("--budget-gb", type=float, default=265.0)
-    p.add_argument("--replays", type=int, default=2)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--budget-gb", type=float, default=200.0,
+                   help="device memory a wave may hold (observed data aside)")
+    p.add_argument("--wave-cells", type=float, default=48e6,
+                   help="cells per step (shots x grid cells) a wave may hold")
     p.add_argument("--scale", type=float, default=1.0,
                    help="multiply step counts (a quick reduced run: e.g. 0.05)")
     p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--no-graph", action="store_true")
     p.add_argument("--out", default=None)
     a = p.parse_args(argv)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    seed, specs = draw_within(a.budget_gb * 1e9, n=a.problems)
+    specs = draw(a.seed, a.problems)
     if a.scale != 1.0:
         specs = [(h, r, max(8, int(round(s * a.scale)))) for h, r, s in specs]
-    est = sum(estimate_bytes(*s) for s in specs)
-    log(f"seed {seed}: {len(specs)} problems, about {est / 1e9:.0f} GB")
+    sizes = sizes_of(specs)
+    log(f"seed {a.seed}: {len(specs)} problems, {sum(z[1] for z in sizes) / 1e9:.0f} GB of "
+        "gradient buffers in all")
     tuned = {}
     if not a.no_tune:
         for h in sorted({s[0] for s in specs}):
             shots = max(r for hh, r, _ in specs if hh == h)
             tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log)
     t0 = time.perf_counter()
-    problems = [build(s, i, seed, dev) for i, s in enumerate(specs)]
+    data = [observed_data(s, i, a.seed, dev) for i, s in enumerate(specs)]
     torch.cuda.synchronize()
-    log(f"built problems and observed data in {time.perf_counter() - t0:.1f} s")
-    ug = UnevenGradients(problems)
-    cells = ug.cells
-    # eager run with per-problem events
-    start = torch.cuda.Event(enable_timing=True)
-    events = {i: torch.cuda.Event(enable_timing=True) for i in range(len(problems))}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    ug.run(events)
-    torch.cuda.synchronize()
-    t_eager = time.perf_counter() - t0
-    lat = np.array([start.elapsed_time(events[i]) / 1e3 for i in range(len(problems))])
-    log(f"eager: {t_eager:.2f} s, {cells / t_eager / 1e9:.1f} Gcell/s")
-    # graph capture and replays
-    side = torch.cuda.Stream(dev)
-    side.wait_stream(torch.cuda.current_stream(dev))
-    with torch.cuda.stream(side):
-        ug.run()
-    torch.cuda.current_stream(dev).wait_stream(side)
-    torch.cuda.synchronize()
-    graph = torch.cuda.CUDAGraph()
-    t0 = time.perf_counter()
-    with torch.cuda.graph(graph):
-        ug.run()
-    t_capture = time.perf_counter() - t0
-    times = []
-    for r in range(a.replays):
+    log(f"observed data in {time.perf_counter() - t0:.1f} s")
+    cells = sum(z[0] for z in sizes)
+    sched = WaveSchedule(lambda i: problem_from(specs[i], data[i], dev), sizes,
+                         a.budget_gb * 1e9, a.wave_cells)
+    waves = [[int(i) for i in w] for w in sched.waves]
+    log(f"{len(waves)} waves: " + ", ".join(str(len(w)) for w in waves))
+
+    def timed(graph):
+        start = torch.cuda.Event(enable_timing=True)
+        events = {i: torch.cuda.Event(enable_timing=True) for i in range(len(specs))}
+        spans = []
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        graph.replay()
+        t_wall = time.perf_counter()
+        start.record()
+        sched.run(events=events, graph=graph, spans=spans)
         torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        log(f"replay {r}: {times[-1]:.2f} s, {cells / times[-1] / 1e9:.1f} Gcell/s")
-    t_best = min(times)
+        wall = time.perf_counter() - t_wall
+        lat = np.array([start.elapsed_time(events[i]) / 1e3 for i in range(len(specs))])
+        busy = [a.elapsed_time(b) / 1e3 for a, b in spans]
+        # the same on the GPU's launch clock: waves back to back, the set-up
+        # between them (allocation, table upload) left out
+        offsets = np.cumsum([0.0] + busy[:-1])
+        lat_gpu = np.empty(len(specs))
+        for w, wave in enumerate(waves):
+            w0 = start.elapsed_time(spans[w][0]) / 1e3
+            for i in wave:
+                lat_gpu[i] = offsets[w] + lat[i] - w0
+        return wall, lat, busy, lat_gpu
+
+    timed(False)  # warm-up (kernel loading, allocator)
+    wall, lat, busy, lat_gpu = timed(False)
+    makespan = float(lat.max())
+    log(f"eager: {wall:.2f} s wall, {sum(busy):.2f} s of wave launches, "
+        f"{cells / wall / 1e9:.1f} Gcell/s; p50 latency {np.percentile(lat, 50):.2f} s of "
+        f"{makespan:.2f} s")
     report = {
         "workload": f"uneven_{len(specs)}_fwi_gradients",
-        "seed": seed,
+        "seed": a.seed,
         "problems": [{"grid": h, "shots": r, "steps": s} for h, r, s in specs],
+        "waves": waves,
+        "budget_gb": a.budget_gb, "wave_cells": a.wave_cells,
         "cell_updates": cells,
-        "graph_seconds": round(t_best, 3),
-        "gcell_s_graph": round(cells / t_best / 1e9, 2),
-        "eager_seconds": round(t_eager, 3),
-        "gcell_s_eager": round(cells / t_eager / 1e9, 2),
-        "capture_seconds": round(t_capture, 3),
+        "eager_seconds": round(wall, 3),
+        "gcell_s_eager": round(cells / wall / 1e9, 2),
+        "wave_launch_seconds": round(sum(busy), 3),
+        "gcell_s_launches": round(cells / sum(busy) / 1e9, 2),
+        "makespan_s": round(makespan, 3),
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3),
                       "p99": round(float(np.percentile(lat, 99)), 3),
                       "min": round(float(lat.min()), 3), "max": round(float(lat.max()), 3)},
+        "p50_over_makespan": round(float(np.percentile(lat, 50)) / makespan, 3),
+        "latency_launch_clock_s": {"p50": round(float(np.percentile(lat_gpu, 50)), 3),
+                                   "p99": round(float(np.percentile(lat_gpu, 99)), 3),
+                                   "max": round(float(lat_gpu.max()), 3)},
         "tuned": {str(k): v for k, v in tuned.items()},
-        "memory_estimate_gb": round(est / 1e9, 1),
     }
+    if not a.no_graph:
+        _, _, busy_g, _ = timed(True)
+        report["graph_replay_seconds"] = round(sum(busy_g), 3)
+        report["gcell_s_graph"] = round(cells / sum(busy_g) / 1e9, 2)
+        log(f"graphs: {sum(busy_g):.2f} s of replays, {cells / sum(busy_g) / 1e9:.1f} Gcell/s")
     line = json.dumps(report)
     print(line, flush=True)
     if a.out:

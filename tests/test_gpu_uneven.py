@@ -1,0 +1,186 @@
+"""The uneven-batch workload (24 FWI gradient problems on {256, 384, 512,
+768}^2 at 0.5 mm, 16-128 ring elements, 1500-5000 steps) on the device:
+
+* the seeded draw of the workload, with its grids and shot counts as drawn and
+  its step counts scaled by 0.25 to keep the suite short (long enough for every
+  shot to reach other receivers), runs as a wave
+  schedule (shortest work first, device memory per wave bounded); every
+  problem's misfit and gradient equal its own single-problem checkpointed
+  gradient (rel <= 1e-6: only the order of the per-segment image sums
+  differs), and the per-wave graph replays equal the eager run bit for bit;
+* two shots of a 384^2 and a 768^2 problem (6- and 12-wavefront rows) run in
+  one batch against the float64 CPU oracle: traces rel-L2 <= 1e-5, misfit rel
+  <= 1e-5 or the bound the trace error implies (2 |dtraces| / |residual|, the
+  residual being a difference of close traces), gradient rel-L2 <= 5e-4
+  inside the ring;
+* one problem per grid width (a ring of a quarter of the grid width, steps
+  for a wave to cross it), batched together: central finite differences of
+  the float32 device misfit along a smooth random direction, Richardson-
+  extrapolated from two steps, against the adjoint gradient: rel <= 1e-3.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from soundwavesimulation_amd import RingArray, axis_centers, ricker, stable_dt
+from soundwavesimulation_amd.batch import (GradientProblem, UnevenGradients, WaveSchedule,
+                                           gradient_problem_bytes)
+from soundwavesimulation_amd.inversion import interior_mask
+from soundwavesimulation_amd.inversion.fwi import CheckpointedGradient, residual_and_misfit
+from soundwavesimulation_amd.phantoms import ellipse_phantom
+from soundwavesimulation_amd.torch_backend import (TorchFDTD2D, TorchRingSampler,
+                                                   TorchScalarWave2D, forward_run,
+                                                   simulate_shots_batch)
+from soundwavesimulation_amd.uneven import draw, observed_data, problem_from, sizes_of
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda", 0)
+SP = 0.5e-3
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_workload_draw_in_waves_matches_single_problem_gradients():
+    specs = [(h, r, max(8, int(round(s * 0.25)))) for h, r, s in draw(2)]
+    print(f"uneven workload, seed 2: {len(specs)} problems, steps scaled by 0.25", flush=True)
+    sizes = sizes_of(specs)
+    data = [observed_data(s, i, 2, DEV) for i, s in enumerate(specs)]
+    budget = 40e9
+    sched = WaveSchedule(lambda i: problem_from(specs[i], data[i], DEV), sizes, budget, 48e6)
+    assert len(sched.waves) > 2 and sorted(i for w in sched.waves for i in w) == list(range(24))
+    for w in sched.waves:
+        assert sum(sizes[i][1] for i in w) <= budget
+    eager = sched.run()
+    graphed = sched.run(graph=True)
+    worst = 0.0
+    for i, spec in enumerate(specs):
+        g_e, J_e = eager[i]
+        g_g, J_g = graphed[i]
+        assert torch.equal(g_g, g_e) and J_g == J_e, i
+        p = problem_from(spec, data[i], DEV)
+        eng = CheckpointedGradient(p.solver, p.sampler, p.S, p.N, checkpoint_every=p.K)
+        samples = eng.forward(p.rows, p.cols, p.values)
+        residual, misfit = residual_and_misfit(samples, p.observed, p.index, p.index)
+        g = torch.zeros_like(p.grad)
+        eng.backward(residual, g, p.solver.dt2)
+        J = float(misfit.cpu())
+        err = float((g_e - g.cpu()).norm() / g.cpu().norm())
+        worst = max(worst, err)
+        assert J > 0.0 and abs(J_e - J) <= 1e-6 * J, i
+        assert err <= 1e-6, (i, spec, err)
+        del p, eng, samples, residual, g
+    print(f"  worst gradient rel vs single-problem runs {worst:.1e}")
+
+
+def _ring_problem(h, n_el, n_steps, seed, sources=None, radius_frac=0.43):
+    x = axis_centers(h, SP)
+    radius = radius_frac * h * SP
+    dt = stable_dt(1650.0, SP, SP, cfl=0.3)
+    pulse = ricker(dt, 300e3)
+    array = RingArray(n_el, radius, x, x)
+    sampler = TorchRingSampler(array, device=DEV)
+    c_true = ellipse_phantom(h, h, SP, 8, seed, radius)
+    truth = TorchFDTD2D(TorchScalarWave2D(c_true, SP, device=DEV), dt)
+    observed, _ = simulate_shots_batch(truth, sampler, torch.as_tensor(pulse, dtype=torch.float32),
+                                       array.rows, array.cols, n_steps)
+    c0 = np.full((h, h), 1500.0)
+    solver = TorchFDTD2D(TorchScalarWave2D(c0, SP, device=DEV), dt)
+    prob = GradientProblem(solver, sampler, array, torch.as_tensor(pulse), n_steps, observed,
+                           sources=sources)
+    return prob, dict(c0=c0, dt=dt, pulse=pulse, array=array, x=x, radius=radius,
+                      observed=observed)
+
+
+def test_two_shots_at_384_and_768_match_the_oracle():
+    n_el = 64
+    probs, meta = [], []
+    # a ring of a quarter of the grid width and steps for a wave to cross it, so
+    # that the residual carries the ellipses' scattering (a residual of close
+    # traces only would leave the gradient at the traces' float32 noise)
+    for h, seed, n_steps in ((384, 11, 1000), (768, 12, 2000)):
+        p, m = _ring_problem(h, n_el, n_steps, seed, sources=[0, n_el // 2], radius_frac=0.25)
+        probs.append(p)
+        meta.append(m)
+    ug = UnevenGradients(probs)
+    ug.run()
+    torch.cuda.synchronize()
+    for p, m in zip(probs, meta):
+        h = m["c0"].shape[0]
+        print(f"two shots at {h}^2 vs the float64 oracle", flush=True)
+        r0, c0i, w = m["array"].footprint()
+        src = p.sources
+        obs = m["observed"][torch.as_tensor(src)].double().cpu().numpy()
+        J64, g64, tr64 = oracle.gradient(m["c0"], SP, SP, m["dt"], m["array"].rows[src],
+                                         m["array"].cols[src], m["pulse"], p.N, r0, c0i, w,
+                                         obs, src)
+        traces = p.samples.permute(1, 2, 0).cpu().numpy()
+        mask = interior_mask(m["x"], m["x"], m["radius"], 2 * SP)
+        e_t = rel(traces, tr64)
+        e_J = abs(float(p.misfit.cpu()) - J64) / J64
+        e_g = rel(p.grad.cpu().numpy()[mask], g64[mask])
+        # J = |r|^2 / 2 with r = traces - observed (observed shared): a trace error
+        # dr moves J by at most |r| |dr|, i.e. J by 2 |dr| / |r| relative
+        r64 = tr64 - obs
+        for k, tx in enumerate(src):
+            r64[k, tx] = 0.0
+        bound = 2.0 * e_t * np.linalg.norm(tr64) / np.linalg.norm(r64)
+        print(f"  traces {e_t:.2e}, misfit {e_J:.2e} (bound from the traces {bound:.2e}), "
+              f"gradient {e_g:.2e}")
+        assert e_t <= 1e-5 and e_J <= max(1e-5, 1.1 * bound) and e_g <= 5e-4
+
+
+def test_directional_finite_differences_per_grid_width():
+    from scipy.ndimage import gaussian_filter
+
+    # rings of a quarter of the grid width, steps for a wave to cross them
+    specs = [(256, 24, 700, 21), (384, 20, 1000, 22), (512, 16, 1340, 23), (768, 16, 2000, 24)]
+    probs, meta = [], []
+    for h, n_el, n_steps, seed in specs:
+        p, m = _ring_problem(h, n_el, n_steps, seed, radius_frac=0.25)
+        probs.append(p)
+        meta.append(m)
+    ug = UnevenGradients(probs)
+    ug.run()
+    torch.cuda.synchronize()
+    for p, m, (h, n_el, n_steps, seed) in zip(probs, meta, specs):
+        mask = interior_mask(m["x"], m["x"], m["radius"], 2 * SP)
+        g = p.grad.double().cpu().numpy()
+        d = gaussian_filter(np.random.default_rng(seed).standard_normal((h, h)), 6.0)
+        d[~mask] = 0.0
+        c2 = m["c0"] ** 2
+        d /= np.abs(d).max()
+        slope = float(np.sum(g * d))
+        J0 = float(p.misfit.cpu())
+        # a step moving the misfit by about 1 % (float32 rounding of the traces
+        # stays far below the difference), at most 2 % of c^2
+        step = min(0.01 * J0 / abs(slope), 0.02 * c2.max())
+
+        def misfit(c2_new):
+            solver = TorchFDTD2D(TorchScalarWave2D(np.sqrt(c2_new), SP, device=DEV), m["dt"])
+            samples = torch.empty((n_steps, p.S, n_el), device=DEV)
+            forward_run(solver, p.S, p.rows, p.cols, p.values, n_steps, samples=samples,
+                        sampler=p.sampler, final=False)
+            _, J = residual_and_misfit(samples, p.observed, p.index, p.index)
+            return float(J.cpu())
+
+        fd = []
+        for eps in (step, 0.5 * step):
+            fd.append((misfit(c2 + eps * d) - misfit(c2 - eps * d)) / (2 * eps))
+        richardson = (4.0 * fd[1] - fd[0]) / 3.0
+        err = abs(richardson - slope) / abs(slope)
+        print(f"{h}^2 x {n_el} shots x {n_steps} steps: <g, d> {slope:.6e}, FD {richardson:.6e}, "
+              f"rel {err:.1e}", flush=True)
+        assert err <= 1e-3
+
+
+def test_problem_bytes_match_the_allocation():
+    p, _ = _ring_problem(256, 20, 300, 3)
+    ny, nx = p.solver.model.shape
+    assert gradient_problem_bytes(ny, nx, p.S, p.sampler.n_elements, p.N, p.K) == \
+        p.memory_bytes

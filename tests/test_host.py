@@ -216,3 +216,24 @@ def test_auto_batch_size_never_admits_more_than_fits(monkeypatch):
     full = gradient_bytes_per_shot(6000, (1024, 1024), 256, 4, torch.float32)
     half = gradient_bytes_per_shot(6000, (1024, 1024), 256, 4, torch.float16)
     assert full - half == 1500 * 1024 * 1024 * 2
+
+
+def test_wave_plan_takes_shortest_work_first_within_memory_and_width():
+    from soundwavesimulation_amd.batch import plan_waves
+    from soundwavesimulation_amd.uneven import draw, sizes_of
+
+    sizes = sizes_of(draw(2))
+    budget, width = 60e9, 48e6
+    waves = plan_waves(sizes, budget, width)
+    flat = [i for w in waves for i in w]
+    assert sorted(flat) == list(range(24))
+    work = [sizes[i][0] for i in flat]
+    assert work == sorted(work)  # shortest work first
+    for w in waves:
+        assert sum(sizes[i][1] for i in w) <= budget
+        assert len(w) == 1 or sum(sizes[i][2] for i in w) <= width
+    # any draw runs within a budget that holds its largest problem
+    big = max(z[1] for z in sizes)
+    assert all(sum(sizes[i][1] for i in w) <= big for w in plan_waves(sizes, big, 1e12))
+    with pytest.raises(ValueError, match="more than the budget"):
+        plan_waves(sizes, 0.5 * big, 1e12)
