@@ -100,7 +100,7 @@ def main(argv=None):
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--budget-gb", type=float, default=200.0,
                    help="device memory a wave may hold (observed data aside)")
-    p.add_argument("--wave-cells", type=float, default=48e6,
+    p.add_argument("--wave-cells", type=float, default=96e6,
                    help="cells per step (shots x grid cells) a wave may hold")
     p.add_argument("--scale", type=float, default=1.0,
                    help="multiply step counts (a quick reduced run: e.g. 0.05)")
