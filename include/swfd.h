@@ -78,7 +78,14 @@ typedef struct swfd_receivers {
  * Imaging (optional, image_field != NULL):
  *   grad_out = grad_in + image_scale * sum_s u'[s] * Lap(image_field[s]).
  * samples (optional): gather of u' after the boundary, (S, R).
- * u_next must not alias u_old or u; u_next_copy (optional) receives u' too. */
+ * u_next must not alias u_old or u; u_next_copy (optional) receives u' too.
+ * src_weights (optional, (S, 4), no counterpart in the reference's solver; the
+ * reference's suggested reciprocity fix, desmond/docs/upstream_suggestions.md:
+ * 65-71): a bilinear source instead of the point source: u'[s, r + dy, c + dx]
+ * += src_values[src_index] * src_weights[s][2 dy + dx] for dy, dx in {0, 1},
+ * (r, c) = (src_rows[s], src_cols[s]) the transmitter's footprint corner and
+ * the weights the receivers' bilinear weights (order w00, w10, w01, w11);
+ * the footprint must lie inside the grid. */
 typedef struct swfd_step_args {
     int32_t n_shots;
     int32_t adjoint;
@@ -94,6 +101,7 @@ typedef struct swfd_step_args {
     float *grad_out;
     float image_scale;
     float *samples;
+    const float *src_weights; /* optional (S, 4): bilinear source (see above) */
 } swfd_step_args;
 
 /* A run of forward steps [step_begin, step_begin + n_steps) with point
@@ -134,6 +142,7 @@ typedef struct swfd_forward_args {
     int32_t ckpt_every, ckpt_slots;
     int32_t hist_lap; /* 1: history slots receive Laplacians (see above) */
     float *work4, *work5; /* optional: with samples, up to 4 steps per launch */
+    const float *src_weights; /* optional (S, 4): bilinear sources, as in swfd_step_args */
 } swfd_forward_args;
 
 /* A run of adjoint steps n = step_hi - 1 down to step_lo with residual

@@ -117,8 +117,10 @@ def step(c, dx, dy, dt, u_old, u, adjoint=False, point=None, dense=None):
 
 
 def forward(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0, weights,
-            keep_wave=False):
-    """Forward shots. Returns traces (S, R, N) and, optionally, wave (N, S, ny, nx)."""
+            keep_wave=False, src_weights=None):
+    """Forward shots. Returns traces (S, R, N) and, optionally, wave (N, S, ny, nx).
+    ``src_weights`` (S, 4): bilinear transmitters with footprint corners
+    (src_rows, src_cols)."""
     c = _f64(c)
     ny, nx = c.shape
     rows, cols = _i32(src_rows), _i32(src_cols)
@@ -132,15 +134,24 @@ def forward(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0, 
     rc = load().oracle_forward(ny, nx, ctypes.c_double(dx), ctypes.c_double(dy),
                                ctypes.c_double(dt), _d(c), S, _i(rows), _i(cols), _d(vals),
                                int(n_steps), R, _i(r0), _i(c0), _d(w), _d(traces),
-                               _d(wave) if keep_wave else None)
+                               _d(wave) if keep_wave else None, _src_w(src_weights, S))
     if rc:
         raise MemoryError("oracle_forward allocation failed")
     return traces, wave
 
 
+def _src_w(src_weights, n_shots):
+    """Pointer to a float64 (S, 4) copy (the pointer keeps it alive); None for
+    point sources."""
+    if src_weights is None:
+        return None
+    return _d(np.ascontiguousarray(np.asarray(src_weights, dtype=np.float64).reshape(n_shots, 4)))
+
+
 def gradient(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0, weights,
-             observed, tx, store_every=1):
-    """Misfit J, gradient w.r.t. c^2 (ny, nx) and predicted traces (S, R, N)."""
+             observed, tx, store_every=1, src_weights=None):
+    """Misfit J, gradient w.r.t. c^2 (ny, nx) and predicted traces (S, R, N).
+    ``src_weights``: bilinear transmitters, as for :func:`forward`."""
     c = _f64(c)
     ny, nx = c.shape
     rows, cols = _i32(src_rows), _i32(src_cols)
@@ -159,7 +170,8 @@ def gradient(c, dx, dy, dt, src_rows, src_cols, src_values, n_steps, row0, col0,
     rc = load().oracle_gradient(ny, nx, ctypes.c_double(dx), ctypes.c_double(dy),
                                 ctypes.c_double(dt), _d(c), S, _i(rows), _i(cols), _d(vals),
                                 int(n_steps), R, _i(r0), _i(c0), _d(w), _d(obs), _i(txs),
-                                int(store_every), ctypes.byref(J), _d(g), _d(traces))
+                                int(store_every), ctypes.byref(J), _d(g), _d(traces),
+                                _src_w(src_weights, S))
     if rc:
         raise MemoryError("oracle_gradient allocation failed")
     return float(J.value), g, traces

@@ -21,7 +21,12 @@
  *     L*[lam] = Lap(c^2 lam) with the *forward* Mur operator
  *     (torch_backend.py:525-537), imaging g += K dt^2 lam' * Lap(u_fwd[n-1]) on
  *     steps with (n-1) % K == 0 (invert_ring.py:683-701, torch_backend.py:583-598);
- *   - misfit J = 1/2 sum residual^2 (invert_ring.py:651).
+ *   - misfit J = 1/2 sum residual^2 (invert_ring.py:651);
+ *   - optional bilinear transmitters (the reference's suggested reciprocity
+ *     fix, desmond/docs/upstream_suggestions.md:65-71, applied in
+ *     desmond/src/phase2_numerics.py:105-120): w * value added on the four
+ *     footprint nodes with the receivers' bilinear weights, in place of the
+ *     nearest-node point source.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -201,12 +206,27 @@ int oracle_step(int ny, int nx, double dx, double dy, double dt, const double *c
     return 0;
 }
 
+/* The transmitter of shot s at step value `val`: a point source at
+ * (sr, sc), or with src_w (S, 4) the bilinear source w * val on the nodes
+ * (sr + dy, sc + dx), weight index 2 dy + dx (w00, w10, w01, w11). */
+static inline void add_transmitter(int nx, double *un, int s, int sr, int sc, double val,
+                                   const double *src_w)
+{
+    if (!src_w) {
+        un[IDX(sr, sc)] += val;
+        return;
+    }
+    for (int k = 0; k < 4; ++k) un[IDX(sr + (k >> 1), sc + (k & 1))] += val * src_w[4 * s + k];
+}
+
 /* Forward shots. traces: (S, R, N) double. wave (optional): (N, S, ny, nx)
- * double, the field after every step. */
+ * double, the field after every step. src_w (optional, (S, 4)): bilinear
+ * transmitters with footprint corners (src_rows, src_cols). */
 int oracle_forward(int ny, int nx, double dx, double dy, double dt, const double *c,
                    int n_shots, const int *src_rows, const int *src_cols,
                    const double *src_values, int n_steps, int n_rec, const int *row0,
-                   const int *col0, const double *w, double *traces, double *wave)
+                   const int *col0, const double *w, double *traces, double *wave,
+                   const double *src_w)
 {
     const size_t n = (size_t)ny * nx;
     double *c2 = malloc(n * sizeof(double));
@@ -225,7 +245,7 @@ int oracle_forward(int ny, int nx, double dx, double dy, double dt, const double
         memset(buf, 0, 3 * n * sizeof(double));
         for (int t = 0; t < n_steps; ++t) {
             leapfrog(&g, uo, u, un, 0, NULL);
-            un[IDX(src_rows[s], src_cols[s])] += src_values[t];
+            add_transmitter(nx, un, s, src_rows[s], src_cols[s], src_values[t], src_w);
             mur(&g, u, un);
             for (int r = 0; r < n_rec; ++r)
                 traces[((size_t)s * n_rec + r) * n_steps + t] =
@@ -246,13 +266,14 @@ int oracle_forward(int ny, int nx, double dx, double dy, double dt, const double
 /* Misfit and adjoint-state gradient with respect to c^2, summed over shots.
  * observed: (S, R, N) double, tx[s]: muted receiver of shot s.
  * store_every K >= 1: imaging on steps with (n-1) % K == 0, weight K dt^2.
- * Outputs: *J, g_c2 (ny, nx), optional traces (S, R, N). */
+ * Outputs: *J, g_c2 (ny, nx), optional traces (S, R, N). src_w: as for
+ * oracle_forward. */
 int oracle_gradient(int ny, int nx, double dx, double dy, double dt, const double *c,
                     int n_shots, const int *src_rows, const int *src_cols,
                     const double *src_values, int n_steps, int n_rec, const int *row0,
                     const int *col0, const double *w, const double *observed,
                     const int *tx, int store_every, double *J_out, double *g_c2,
-                    double *traces_out)
+                    double *traces_out, const double *src_w)
 {
     const size_t n = (size_t)ny * nx;
     double *c2 = malloc(n * sizeof(double));
@@ -281,7 +302,7 @@ int oracle_gradient(int ny, int nx, double dx, double dy, double dt, const doubl
         memset(buf, 0, 3 * n * sizeof(double));
         for (int t = 0; t < n_steps; ++t) {
             leapfrog(&g, uo, u, un, 0, NULL);
-            un[IDX(src_rows[s], src_cols[s])] += src_values[t];
+            add_transmitter(nx, un, s, src_rows[s], src_cols[s], src_values[t], src_w);
             mur(&g, u, un);
             for (int r = 0; r < n_rec; ++r)
                 tr[(size_t)r * n_steps + t] = gather(nx, un, row0[r], col0[r], w, n_rec, r);

@@ -46,7 +46,7 @@ class StepArgs(ctypes.Structure):
                 ("u_next", _fp), ("u_next_copy", _fp), ("src_rows", _fp), ("src_cols", _fp),
                 ("src_values", _fp), ("src_index", _c_int), ("dense", _fp), ("residual", _fp),
                 ("image_field", _fp), ("grad_in", _fp), ("grad_out", _fp),
-                ("image_scale", _c_float), ("samples", _fp)]
+                ("image_scale", _c_float), ("samples", _fp), ("src_weights", _fp)]
 
 
 class ForwardArgs(ctypes.Structure):
@@ -56,7 +56,7 @@ class ForwardArgs(ctypes.Structure):
                 ("src_cols", _fp), ("src_values", _fp), ("samples", _fp), ("hist", _fp),
                 ("hist_first", _c_int), ("hist_every", _c_int), ("hist_slots", _c_int),
                 ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int),
-                ("hist_lap", _c_int), ("work4", _fp), ("work5", _fp)]
+                ("hist_lap", _c_int), ("work4", _fp), ("work5", _fp), ("src_weights", _fp)]
 
 
 class AdjointArgs(ctypes.Structure):

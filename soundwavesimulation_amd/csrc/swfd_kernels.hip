@@ -75,6 +75,9 @@ struct StepParams {
     const int *src_rows, *src_cols;
     const float *src_values;
     int src_index;
+    // bilinear source (NULL: point source): (S, 4) weights of the 2 x 2 cells
+    // from (src_rows[s], src_cols[s]), order w00, w10, w01, w11
+    const float *src_w;
     // dense source (S, ny, nx)
     const float *dense;
     // bilinear residual injection
@@ -128,13 +131,42 @@ __device__ __forceinline__ float lap5(float w, float e, float s, float n, float 
     return __fmaf_rn(cen, c, __fmaf_rn(idy2, s + n, idx2 * (w + e)));
 }
 
+// The transmitter's contribution to cell (rho, x) of shot s, given the
+// source value `val` of the step: a point source adds val at (sr, sc); a
+// bilinear source (p.src_w set) adds val * w at the 2 x 2 cells from (sr, sc)
+// with the receivers' bilinear weights, so that source and receiver are
+// adjoint (the reference's suggested fix for reciprocity, injecting
+// w * pulse[n] on the four footprint nodes). The weights are read with scalar
+// loads only on the footprint's two rows.
+__device__ __forceinline__ float add_transmitter(const StepParams &p, int s, float v, int rho,
+                                                 int x, int sr, int sc, float val)
+{
+    if (p.src_w == nullptr) return (rho == sr && x == sc) ? v + val : v;
+    const int dy = rho - sr;
+    if (dy != 0 && dy != 1) return v;
+    typedef const float __attribute__((address_space(4))) kw;
+    const kw *w = (const kw *)p.src_w + (size_t)s * 4 + 2 * dy;
+    const float w0 = w[0], w1 = w[1];
+    const int dx = x - sc;
+    if (dx == 0) return v + val * w0;
+    if (dx == 1) return v + val * w1;
+    return v;
+}
+
+// Rows the transmitter of a step touches, beyond its first: 1 for a bilinear
+// source, 0 for a point source.
+__device__ __forceinline__ int transmitter_rows(const StepParams &p)
+{
+    return p.src_w != nullptr ? 1 : 0;
+}
+
 // Sources added to the interior value of cell (y, x) of shot s, in the order
-// point, dense, residual scatter.
+// point (or bilinear), dense, residual scatter.
 __device__ __forceinline__ float add_sources(const StepParams &p, int s, size_t soff,
                                              size_t ci, int y, int x, float v)
 {
-    if (p.src_rows != nullptr && y == p.src_rows[s] && x == p.src_cols[s])
-        v += p.src_values[p.src_index];
+    if (p.src_rows != nullptr)
+        v = add_transmitter(p, s, v, y, x, p.src_rows[s], p.src_cols[s], p.src_values[p.src_index]);
     if (p.dense != nullptr) v += p.dense[soff + ci];
     if (p.resid != nullptr) {
         const int e = p.cell_map[ci];
@@ -525,11 +557,11 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
         }
         if constexpr ((F & kPoint) != 0) {
             const int sr = p.src_rows[s], sc = p.src_cols[s];
-            if (sr >= y0 && sr < y0 + RB && x == sc) {
+            const int extra = transmitter_rows(p);
+            if (sr + extra >= y0 && sr < y0 + RB && (unsigned)(x - sc) <= (unsigned)extra) {
                 const float val = p.src_values[p.src_index];
 #pragma unroll
-                for (int i = 0; i < RB; ++i)
-                    if (y0 + i == sr) v[i] += val;
+                for (int i = 0; i < RB; ++i) v[i] = add_transmitter(p, s, v[i], y0 + i, x, sr, sc, val);
             }
         }
         if constexpr ((F & kDense) != 0) {
@@ -845,9 +877,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
                 const float bw = from_west(b0, q0), be = from_east(b0, q0);
                 c0 = __fmaf_rn(k0, lap5(bw, be, bm, bp, b0, p.idx2, p.idy2, p.cen),
                                2.0f * b0 - a0);
-                if constexpr ((F & kPoint) != 0) {
-                    if (y == sr && x == sc) c0 += s1;
-                }
+                if constexpr ((F & kPoint) != 0) c0 = add_transmitter(p, s, c0, y, x, sr, sc, s1);
                 if constexpr (EDGE) {
                     if (y >= 1 && y <= H - 2) c0 = mur_sides(c0, b0, x, W, s0);
                     if (y == H - 1) c0 = mur_row(mur, W, H, x, kt_x, b0, bm, cm1, 2);
@@ -860,9 +890,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
                 const float he = hl ? b0 : qw;
                 hc = __fmaf_rn(hk0, lap5(hw, he, qm1, qp, q0, p.idx2, p.idy2, p.cen),
                                2.0f * q0 - ha0);
-                if constexpr ((F & kPoint) != 0) {
-                    if (y == sr && xh == sc) hc += s1;
-                }
+                if constexpr ((F & kPoint) != 0) hc = add_transmitter(p, s, hc, y, xh, sr, sc, s1);
                 if constexpr (EDGE) {
                     // right halo on column W-1: Mur from lane 63's own block
                     // values at column W-2 (b0, c0); never the left halo
@@ -907,9 +935,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
                 const float cw = from_west(cm1, hcm1), ce = from_east(cm1, hcm1);
                 d = __fmaf_rn(km1, lap5(cw, ce, cm2, c0, cm1, p.idx2, p.idy2, p.cen),
                               2.0f * cm1 - bm);
-                if constexpr ((F & kPoint) != 0) {
-                    if (r == sr && x == sc) d += s2;
-                }
+                if constexpr ((F & kPoint) != 0) d = add_transmitter(p, s, d, r, x, sr, sc, s2);
                 if constexpr (EDGE) {
                     if (r >= 1 && r <= H - 2) d = mur_sides(d, cm1, x, W, sm1);
                     if (r == H - 1) d = mur_row(mur, W, H, x, kt_x, cm1, cm2, dm1, 2);
@@ -1065,6 +1091,7 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                               plane_bytes);
         }
         const int sr = p.src_rows[s], sc = p.src_cols[s];
+        const int src_extra = transmitter_rows(p);
         float sv[KL];
 #pragma unroll
         for (int j = 0; j < KL; ++j) sv[j] = p.src_values[p.src_index + j];
@@ -1147,7 +1174,8 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                 const float uw = from_west(u, hw->v[j - 1]), ue = from_east(u, he->v[j - 1]);
                 const float lp = lap5(uw, ue, us, un, u, p.idx2, p.idy2, p.cen);
                 float v = __fmaf_rn(kh[j - 1], lp, 2.0f * u - old);
-                if (__builtin_expect(rho == sr, 0)) v = (x == sc) ? v + sv[j - 1] : v;
+                if (__builtin_expect((unsigned)(rho - sr) <= (unsigned)src_extra, 0))
+                    v = add_transmitter(p, s, v, rho, x, sr, sc, sv[j - 1]);
                 if constexpr (XE) {
                     // rows of segments away from the top and bottom are interior
                     if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
@@ -3265,6 +3293,7 @@ struct ForwardRun {
         p.src_rows = a->src_rows;
         p.src_cols = a->src_cols;
         p.src_values = a->src_values;
+        p.src_w = a->src_weights;
         p.src_index = n;
         int gathers = 0;
         if (a->samples != nullptr && n_pend > 0 && !sep_gather) {
@@ -4014,6 +4043,7 @@ int swfd_step(const swfd_grid *grid, const swfd_medium *medium, const swfd_recei
     p.src_cols = a->src_cols;
     p.src_values = a->src_values;
     p.src_index = a->src_index;
+    p.src_w = a->src_rows != nullptr ? a->src_weights : nullptr;
     p.dense = a->dense;
     p.resid = a->residual;
     if ((rc = launch_step(p, a->adjoint != 0, as_stream(stream), 0))) return rc;

@@ -147,16 +147,19 @@ class CheckpointedGradient:
         self.rows = None
         self.cols = None
         self.values = None
+        self.src_weights = None
 
-    def forward(self, rows_i32, cols_i32, values):
-        """All forward steps; samples land in ``self.samples`` (N, S, R)."""
+    def forward(self, rows_i32, cols_i32, values, src_weights=None):
+        """All forward steps; samples land in ``self.samples`` (N, S, R).
+        ``src_weights`` (S, 4): bilinear sources (see ``forward_run``)."""
         self.rows, self.cols, self.values = rows_i32, cols_i32, values
+        self.src_weights = src_weights
         # the adjoint's fields are idle here: they keep the intermediate steps of
         # each launch for the gather
         forward_run(self.solver, self.n_shots, rows_i32, cols_i32, values, self.n_steps,
                     samples=self.samples, sampler=self.sampler, work=self.work + self.lam[:2],
                     ckpt=self.ckpt, ckpt_every=self.K if self.ckpt is not None else 0,
-                    final=False)
+                    final=False, src_weights=src_weights)
         return self.samples
 
     def lap_history(self):
@@ -195,7 +198,7 @@ class CheckpointedGradient:
                 forward_run(self.solver, self.n_shots, self.rows, self.cols, self.values, n_re,
                             init=init, hist=self.hist,
                             hist_first=n0 - 2, hist_every=1, step_begin=n0, work=self.work,
-                            final=False)
+                            final=False, src_weights=self.src_weights)
             adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
                         state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
                         hist_every=1, image_scale=image_scale, grad=grad,
@@ -218,7 +221,8 @@ class CheckpointedGradient:
                 init = (self.ckpt[2 * (j - 1)], self.ckpt[2 * (j - 1) + 1])
             forward_run(self.solver, self.n_shots, self.rows, self.cols, self.values, n1 - n0,
                         init=init, hist=self.hist, hist_first=n0 - 2, hist_every=1,
-                        step_begin=n0, work=self.work, final=False, hist_lap=True)
+                        step_begin=n0, work=self.work, final=False, hist_lap=True,
+                        src_weights=self.src_weights)
             adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
                         state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
                         hist_every=1, image_scale=image_scale, grad=grad,
@@ -256,7 +260,7 @@ def _history_chunks(n_steps, store_every, slots):
 
 
 def forward_store_batch(solver, sampler, array, source_values, sources, n_steps, batched,
-                        store_every=1, wave_dtype=None):
+                        store_every=1, wave_dtype=None, bilinear_source=False):
     """Batched forward keeping every ``store_every``-th field (invert_ring.py:422-455).
 
     Returns traces (S, R, N) and the stored wavefields (stored, S, ny, nx) in
@@ -267,11 +271,16 @@ def forward_store_batch(solver, sampler, array, source_values, sources, n_steps,
     """
     n_steps = int(n_steps)
     store_every = int(store_every)
-    rows_cols = [array.inject_rows_cols(int(tx)) for tx in sources]
-    rows = np.asarray([r for r, _ in rows_cols], dtype=np.int64)
-    cols = np.asarray([c for _, c in rows_cols], dtype=np.int64)
-    batched.set_sources(rows, cols)
     dev = solver.model.device
+    src_w = None
+    if bilinear_source:
+        rows, cols, w = array.bilinear_sources(sources)
+        src_w = torch.as_tensor(w.astype(np.float32)).to(dev).contiguous()
+    else:
+        rows_cols = [array.inject_rows_cols(int(tx)) for tx in sources]
+        rows = np.asarray([r for r, _ in rows_cols], dtype=np.int64)
+        cols = np.asarray([c for _, c in rows_cols], dtype=np.int64)
+    batched.set_sources(rows, cols)
     values = _source_values_tensor(None, n_steps, dev, source_values)
     S = len(sources)
     shape = (S, *solver.model.shape)
@@ -282,7 +291,7 @@ def forward_store_batch(solver, sampler, array, source_values, sources, n_steps,
     if code is None:
         u_old, u = forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, n_steps,
                                samples=samples, sampler=sampler, hist=wave, hist_first=0,
-                               hist_every=store_every)
+                               hist_every=store_every, src_weights=src_w)
     else:
         work = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(6)]
         staging = torch.empty((HISTORY_STAGING_SLOTS, *shape), device=dev, dtype=_F32)
@@ -293,7 +302,7 @@ def forward_store_batch(solver, sampler, array, source_values, sources, n_steps,
             forward_run(solver, S, batched._rows_i32, batched._cols_i32, values, end - begin,
                         init=(work[0], work[1]), samples=samples, sampler=sampler,
                         hist=staging, hist_first=begin, hist_every=store_every,
-                        step_begin=begin, work=work)
+                        step_begin=begin, work=work, src_weights=src_w)
             convert_history(staging[:n_slots], wave[slot0:slot0 + n_slots])
         u_old, u = work[0], work[1]
     batched.u_old, batched.u = u_old, u
@@ -306,8 +315,11 @@ class TorchLeastSquaresFWI:
     def __init__(self, solver, sampler, array, pulse, n_steps, n_shots, observed, mask,
                  alpha=0.0, tv=0.0, tv_eps=1.0e-9, c_min=1400.0, c_max=2000.0,
                  c_background=1500.0, shot_seed=0, parallel_shots=16, batched_solvers=None,
-                 store_every=1, wave_dtype=None, checkpoint_every=None):
+                 store_every=1, wave_dtype=None, checkpoint_every=None, bilinear_source=False):
         self.solver = solver
+        # transmitters inject on their bilinear footprints (reciprocal data);
+        # the reference fires from the nearest node
+        self.bilinear_source = bool(bilinear_source)
         self.model = solver.model
         self.sampler = sampler
         self.array = array
@@ -413,9 +425,19 @@ class TorchLeastSquaresFWI:
         return residual
 
     def _rows_cols(self, shot_batch):
-        rc = [self.array.inject_rows_cols(int(tx)) for tx in shot_batch]
         dev = self.model.device
+        if self.bilinear_source:
+            rows, cols, _ = self.array.bilinear_sources(shot_batch)
+            return _i32(rows, dev), _i32(cols, dev)
+        rc = [self.array.inject_rows_cols(int(tx)) for tx in shot_batch]
         return (_i32([r for r, _ in rc], dev), _i32([c for _, c in rc], dev))
+
+    def _source_weights(self, shot_batch):
+        """(S, 4) device weights of bilinear transmitters, None for point sources."""
+        if not self.bilinear_source:
+            return None
+        _, _, w = self.array.bilinear_sources(shot_batch)
+        return torch.as_tensor(w.astype(np.float32)).to(self.model.device).contiguous()
 
     def _exact(self):
         return self.store_every == 1 and self.wave_dtype == _F32
@@ -434,7 +456,8 @@ class TorchLeastSquaresFWI:
             samples = torch.empty((self.n_steps, S, self.sampler.n_elements), device=dev,
                                   dtype=_F32)
             forward_run(self.solver, S, rows, cols, self.source_values, self.n_steps,
-                        samples=samples, sampler=self.sampler, final=False)
+                        samples=samples, sampler=self.sampler, final=False,
+                        src_weights=self._source_weights(shot_batch))
             _, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
                                           _i32(self._mute_indices(shot_batch), dev))
             total = total + part
@@ -464,7 +487,7 @@ class TorchLeastSquaresFWI:
         dev = self.model.device
         engine = self._engine(len(shot_batch))
         rows, cols = self._rows_cols(shot_batch)
-        samples = engine.forward(rows, cols, self.source_values)
+        samples = engine.forward(rows, cols, self.source_values, self._source_weights(shot_batch))
         residual, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
                                              _i32(self._mute_indices(shot_batch), dev))
         engine.backward(residual, g_c2, self.solver.dt2)
@@ -475,7 +498,8 @@ class TorchLeastSquaresFWI:
         batched = self._batched_solver(len(shot_batch))
         predicted, wave = forward_store_batch(self.solver, self.sampler, self.array,
                                               self.source_values, shot_batch, self.n_steps,
-                                              batched, self.store_every, self.wave_dtype)
+                                              batched, self.store_every, self.wave_dtype,
+                                              self.bilinear_source)
         samples = predicted.permute(2, 0, 1).contiguous()
         residual, part = residual_and_misfit(samples, self.observed, _i32(shot_batch, dev),
                                              _i32(self._mute_indices(shot_batch), dev))

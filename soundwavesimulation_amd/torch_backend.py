@@ -199,6 +199,7 @@ class TorchFDTD2D:
         _native.lib().swfd_medium_view(self._grid, _native.ptr(self._coef), self._medium)
         self._medium_version = None
         self._src_rc = torch.zeros(2, device=dev, dtype=torch.int32)
+        self._src_w = None
         self._refresh_boundaries()
         if self.cfl >= 1.0:
             raise ValueError(f"Unstable time step: CFL = {self.cfl:.3f} >= 1.")
@@ -260,6 +261,19 @@ class TorchFDTD2D:
         self.source_col = int(col)
         self._src_rc = torch.tensor([self.source_row, self.source_col],
                                     dtype=torch.int32).to(self.model.device)
+        self._src_w = None
+
+    def set_bilinear_source(self, row0: int, col0: int, weights):
+        """Inject ``w * value`` on the 2 x 2 nodes from (row0, col0) with the
+        weights w00, w10, w01, w11 (``RingArray.bilinear_sources``) instead of
+        ``value`` at one node: source and receivers are then adjoint, which
+        makes the simulation reciprocal (no counterpart in the reference's
+        solver; its suggested fix, desmond/docs/upstream_suggestions.md:65-71)."""
+        ny, nx = self.model.shape
+        if not (0 <= int(row0) < ny - 1 and 0 <= int(col0) < nx - 1):
+            raise ValueError("Bilinear source footprint must lie inside the grid.")
+        self.set_source(row0, col0)
+        self._src_w = _weights_tensor(weights, 1, self.model.device)
 
     def set_c(self, c):
         self.model.set_c(c)
@@ -294,6 +308,7 @@ class TorchFDTD2D:
             args.src_cols = _native.ptr(self._src_rc[1:2])
             args.src_values = _native.ptr(vt)
             args.src_index = 0
+            args.src_weights = _native.ptr(self._src_w)
         elif source_mode == 2:
             args.dense = _native.ptr(source_field)
         _native.check(_native.lib().swfd_step(self._grid, self._medium, None, args,
@@ -526,9 +541,26 @@ def _source_values_tensor(pulse, n_steps, device, source_values=None):
     return source_values.to(device=device, dtype=_F32).contiguous()
 
 
+def _check_src_weights(src_weights, n_shots):
+    if src_weights is None:
+        return None
+    if src_weights.dtype != _F32 or not src_weights.is_contiguous() or \
+            tuple(src_weights.shape) != (n_shots, 4):
+        raise ValueError("Bilinear source weights must be a contiguous float32 (shots, 4) tensor.")
+    return src_weights
+
+
+def _weights_tensor(weights, n_shots, device):
+    if weights is None:
+        return None
+    t = torch.as_tensor(np.asarray(weights, dtype=np.float32).reshape(n_shots, 4)).to(device)
+    return t.contiguous()
+
+
 def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=None,
                 samples=None, sampler=None, hist=None, hist_first=0, hist_every=1, ckpt=None,
-                ckpt_every=0, step_begin=0, work=None, final=True, hist_lap=False):
+                ckpt_every=0, step_begin=0, work=None, final=True, hist_lap=False,
+                src_weights=None):
     """Run forward steps in one native call; returns the final ``(u_old, u)``.
 
     ``init`` is the state before ``step_begin`` (zeros if None); ``samples``
@@ -538,6 +570,8 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
     (with samples: two, four with six fields).
     ``hist_lap``: the history receives the Laplacians of the fields (of
     ``init`` and every output but the last) for ``adjoint_run(hist_lap=True)``.
+    ``src_weights`` (S, 4) float32 device tensor: bilinear sources, the rows
+    and columns then being the footprint corners (``RingArray.bilinear_sources``).
     """
     model = solver.model
     dev = model.device
@@ -564,7 +598,8 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
         ckpt=_native.ptr(ckpt), ckpt_every=ckpt_every,
         ckpt_slots=0 if ckpt is None else ckpt.shape[0], hist_lap=1 if hist_lap else 0,
         work4=_native.ptr(work[4]) if len(work) > 5 else None,
-        work5=_native.ptr(work[5]) if len(work) > 5 else None)
+        work5=_native.ptr(work[5]) if len(work) > 5 else None,
+        src_weights=_native.ptr(_check_src_weights(src_weights, n_shots)))
     rec = None
     if samples is not None:
         if sampler is None:
@@ -613,10 +648,16 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
 
 
 def simulate_shot(solver: TorchFDTD2D, sampler: TorchRingSampler, pulse, source_row: int,
-                  source_col: int, n_steps: int, snapshot_stride: int = 0):
-    """One shot; returns traces (R, N) on the device and host snapshots."""
+                  source_col: int, n_steps: int, snapshot_stride: int = 0, source_weights=None):
+    """One shot; returns traces (R, N) on the device and host snapshots.
+
+    ``source_weights`` (4,): a bilinear source whose footprint corner is
+    (source_row, source_col) (``RingArray.bilinear_sources``)."""
     solver.reset()
-    solver.set_source(source_row, source_col)
+    if source_weights is None:
+        solver.set_source(source_row, source_col)
+    else:
+        solver.set_bilinear_source(source_row, source_col, source_weights)
     dev = solver.model.device
     n_steps = int(n_steps)
     if snapshot_stride:
@@ -633,7 +674,7 @@ def simulate_shot(solver: TorchFDTD2D, sampler: TorchRingSampler, pulse, source_
     rc = torch.tensor([[source_row], [source_col]], dtype=torch.int32).to(dev)
     samples = torch.empty((n_steps, 1, sampler.n_elements), device=dev, dtype=_F32)
     u_old, u = forward_run(solver, 1, rc[0], rc[1], values, n_steps, samples=samples,
-                           sampler=sampler)
+                           sampler=sampler, src_weights=solver._src_w)
     solver.u_old = u_old[0].clone()
     solver.u = u[0].clone()
     return samples[:, 0, :].transpose(0, 1).contiguous(), []
@@ -760,8 +801,11 @@ class TorchFDTD2DBatch:
 
 def simulate_shots_batch(solver: TorchFDTD2D, sampler: TorchRingSampler, pulse, source_rows,
                          source_cols, n_steps: int, batched: TorchFDTD2DBatch | None = None,
-                         source_values=None):
-    """Independent shots with a shared waveform; returns traces (S, R, N) and the final u."""
+                         source_values=None, source_weights=None):
+    """Independent shots with a shared waveform; returns traces (S, R, N) and the final u.
+
+    ``source_weights`` (S, 4): bilinear sources whose footprint corners are the
+    rows and columns given (``RingArray.bilinear_sources``)."""
     if batched is None:
         batched = TorchFDTD2DBatch(solver, len(source_rows))
     elif batched.n_shots != len(source_rows):
@@ -772,7 +816,8 @@ def simulate_shots_batch(solver: TorchFDTD2D, sampler: TorchRingSampler, pulse, 
     values = _source_values_tensor(pulse, n_steps, dev, source_values)
     samples = torch.empty((n_steps, batched.n_shots, sampler.n_elements), device=dev, dtype=_F32)
     u_old, u = forward_run(solver, batched.n_shots, batched._rows_i32, batched._cols_i32,
-                           values, n_steps, samples=samples, sampler=sampler)
+                           values, n_steps, samples=samples, sampler=sampler,
+                           src_weights=_weights_tensor(source_weights, batched.n_shots, dev))
     batched.u_old, batched.u = u_old, u
     traces = samples.permute(1, 2, 0).contiguous()
     return traces, batched.u
