@@ -160,3 +160,103 @@ def test_two_shots_match_float64_oracle(key):
     print(f"{key}^2 gradient vs float64 oracle: J {j_err:.2e}, gradient {g_err:.2e}")
     assert j_err <= 1e-5
     assert g_err <= 5e-4
+
+
+def test_1024_gradient_cell_by_cell_against_the_oracle():
+    """The headline grid cell by cell: 1024^2 at 0.25 mm, the 256-element ring of
+    radius 110 mm, two shots for 1500 steps (waves reach the ellipses and
+    return), blurred start, default checkpoints; traces rel-L2 <= 1e-5, misfit
+    rel <= 1e-5 or the bound the trace error implies (2 |dtraces| / |residual|),
+    and the gradient inside the ring rel-L2 <= 5e-4 against the float64 oracle
+    (which keeps every forward field: 25 GB of host memory here)."""
+    w = workload("1024")
+    n, h, n_el = w["n"], w["h"], w["n_el"]
+    n_steps = 1500
+    array = w["array"]
+    r0, c0i, wts = array.footprint()
+    shots = np.array([0, n_el // 2])
+    rows, cols = array.rows[shots], array.cols[shots]
+    print(f"1024^2, shots {shots.tolist()}, {n_steps} steps: observed data", flush=True)
+    obs64, _ = oracle.forward(w["c_true"], h, h, w["dt"], rows, cols, w["pulse"], n_steps, r0,
+                              c0i, wts)
+    c_start = c_from_m(w["m0"])
+    sampler = TorchRingSampler(array, device=DEV)
+    solver = TorchFDTD2D(TorchScalarWave2D(c_start, h, device=DEV), w["dt"])
+    c32 = solver.model.c.double().cpu().numpy()
+    J_ref, g_ref, tr_ref = oracle.gradient(c32, h, h, w["dt"], rows, cols, w["pulse"], n_steps,
+                                           r0, c0i, wts, obs64.astype(np.float32), shots)
+    engine = CheckpointedGradient(solver, sampler, len(shots), n_steps)
+    pulse_t = torch.as_tensor(w["pulse"], dtype=torch.float32, device=DEV)
+    values = torch.zeros(n_steps, device=DEV)
+    values[:pulse_t.numel()] = pulse_t
+    samples = engine.forward(torch.as_tensor(rows.astype(np.int32)).to(DEV),
+                             torch.as_tensor(cols.astype(np.int32)).to(DEV), values)
+    observed = torch.zeros((n_el, n_el, n_steps), device=DEV)
+    observed[torch.as_tensor(shots)] = torch.as_tensor(obs64, dtype=torch.float32).to(DEV)
+    idx = torch.as_tensor(shots.astype(np.int32)).to(DEV)
+    residual, misfit = residual_and_misfit(samples, observed, idx, idx)
+    grad = torch.zeros((n, n), device=DEV)
+    engine.backward(residual, grad, solver.dt2)
+    torch.cuda.synchronize()
+    J = float(misfit.item())
+    mask = w["mask"]
+    g_err = rel(grad.cpu().numpy()[mask], g_ref[mask])
+    j_err = abs(J - J_ref) / abs(J_ref)
+    # J = |r|^2 / 2, r = traces - observed (observed shared): a trace error dr moves
+    # J by at most 2 |dr| / |r| relative, and the blurred start leaves r small
+    predicted = samples.permute(1, 2, 0).double().cpu().numpy()
+    t_err = rel(predicted, tr_ref)
+    r_ref = tr_ref - obs64.astype(np.float32)
+    for k, tx in enumerate(shots):
+        r_ref[k, tx] = 0.0
+    bound = 2.0 * t_err * np.linalg.norm(tr_ref) / np.linalg.norm(r_ref)
+    print(f"1024^2 gradient vs float64 oracle: traces {t_err:.2e}, J {j_err:.2e} (bound from "
+          f"the traces {bound:.2e}), gradient {g_err:.2e} (checkpoints every {engine.K} steps)")
+    assert t_err <= 1e-5
+    assert j_err <= max(1e-5, 1.1 * bound)
+    assert g_err <= 5e-4
+
+
+def test_512_gradient_on_16_seeded_pixels():
+    """The batched-gradient workload's pixel check on the device: 512^2, 128
+    shots, 4000 steps, blurred start; 16 seeded pixels inside the ring, each
+    moved by +-20 and +-10 m/s (Richardson-extrapolated central differences of
+    the float32 misfit; a 0.1 m/s step of one pixel moves the misfit by less
+    than the traces' float32 rounding, so that size is checked in float64 on
+    the oracle) against dJ/dc = -2 g_m / c^3 from the adjoint gradient:
+    rel-L2 over the 16 pixels <= 1e-2."""
+    w = workload("512")
+    solver = TorchFDTD2D(TorchScalarWave2D(w["c_true"], w["h"], device=DEV), w["dt"])
+    sampler = TorchRingSampler(w["array"], device=DEV)
+    pulse_t = torch.as_tensor(w["pulse"], dtype=torch.float32, device=DEV)
+    n_el, n_steps = w["n_el"], w["n_steps"]
+    observed, _ = simulate_shots_batch(solver, sampler, pulse_t, w["array"].rows,
+                                       w["array"].cols, n_steps)
+    problem = TorchLeastSquaresFWI(solver, sampler, w["array"], pulse_t, n_steps, n_el, observed,
+                                   w["mask"], c_min=1400.0, c_max=1650.0, c_background=1500.0,
+                                   parallel_shots=w["per_batch"], checkpoint_every=w["every"])
+    m0, mask = w["m0"], w["mask"]
+    _, g = problem.misfit_and_grad(m0)
+    c0 = c_from_m(m0)
+    rng = np.random.default_rng(16)
+    candidates = np.argwhere(mask & (c0 > 1425.0) & (c0 < 1625.0))
+    pixels = candidates[rng.choice(len(candidates), 16, replace=False)]
+    fd, adj = [], []
+    for (py, px) in pixels:
+        est = []
+        for hstep in (20.0, 10.0):
+            js = []
+            for sign in (1.0, -1.0):
+                c = c0.copy()
+                c[py, px] += sign * hstep
+                m = 1.0 / c ** 2
+                np.testing.assert_array_equal(problem.project(m), m)
+                js.append(problem.misfit(m))
+            est.append((js[0] - js[1]) / (2.0 * hstep))
+        fd.append((4.0 * est[1] - est[0]) / 3.0)
+        adj.append(-2.0 * g[py, px] / c0[py, px] ** 3)
+        print(f"  pixel ({py}, {px}): adjoint {adj[-1]:.6e}, finite differences {fd[-1]:.6e}",
+              flush=True)
+    err = rel(fd, adj)
+    print(f"16 pixels at 512^2: rel-L2 {err:.2e}")
+    assert err <= 1e-2
