@@ -125,6 +125,14 @@ struct StepParams {
     const unsigned *fp_rows;
 };
 
+// 2u - u_old of the leapfrog update as one fused operation: 2u is exact, so
+// the single rounding of fma(2, u, -u_old) equals that of the separate
+// product and difference.
+__device__ __forceinline__ float leap(float u, float u_old)
+{
+    return __fmaf_rn(2.0f, u, -u_old);
+}
+
 __device__ __forceinline__ float lap5(float w, float e, float s, float n, float c,
                                       float idx2, float idy2, float cen)
 {
@@ -198,7 +206,7 @@ __device__ float interior_value(const StepParams &p, int s, size_t soff, int y, 
     float v;
     if (!ADJ) {
         v = __fmaf_rn(p.c2dt2[i], lap5(uw, ue, us, un, uc, p.idx2, p.idy2, p.cen),
-                      2.0f * uc - uo);
+                      leap(uc, uo));
     } else {
         const float *c2 = p.c2;
         const float pw = x > 0 ? c2[i - 1] * uw : 0.0f;
@@ -206,7 +214,7 @@ __device__ float interior_value(const StepParams &p, int s, size_t soff, int y, 
         const float ps = y > 0 ? c2[i - W] * us : 0.0f;
         const float pn = y < H - 1 ? c2[i + W] * un : 0.0f;
         v = __fmaf_rn(p.dt2, lap5(pw, pe, ps, pn, c2[i] * uc, p.idx2, p.idy2, p.cen),
-                      2.0f * uc - uo);
+                      leap(uc, uo));
     }
     return add_sources(p, s, soff, i, y, x, v);
 }
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
                 v[i] = __fmaf_rn(coef[i],
                                  lap5(uw, ue, col[i], col[i + 2], col[i + 1], p.idx2, p.idy2,
                                       p.cen),
-                                 2.0f * col[i + 1] - old[i]);
+                                 leap(col[i + 1], old[i]));
             }
         } else {
             float pc[RB + 2];
@@ -552,7 +560,7 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams p)
                 const float pe = from_east(pc[i + 1], ph);
                 v[i] = __fmaf_rn(p.dt2,
                                  lap5(pw, pe, pc[i], pc[i + 2], pc[i + 1], p.idx2, p.idy2, p.cen),
-                                 2.0f * col[i + 1] - old[i]);
+                                 leap(col[i + 1], old[i]));
             }
         }
         if constexpr ((F & kPoint) != 0) {
@@ -876,7 +884,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
             if (row_in) {
                 const float bw = from_west(b0, q0), be = from_east(b0, q0);
                 c0 = __fmaf_rn(k0, lap5(bw, be, bm, bp, b0, p.idx2, p.idy2, p.cen),
-                               2.0f * b0 - a0);
+                               leap(b0, a0));
                 if constexpr ((F & kPoint) != 0) c0 = add_transmitter(p, s, c0, y, x, sr, sc, s1);
                 if constexpr (EDGE) {
                     if (y >= 1 && y <= H - 2) c0 = mur_sides(c0, b0, x, W, s0);
@@ -889,7 +897,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
                 const float hw = hl ? qe : b0;
                 const float he = hl ? b0 : qw;
                 hc = __fmaf_rn(hk0, lap5(hw, he, qm1, qp, q0, p.idx2, p.idy2, p.cen),
-                               2.0f * q0 - ha0);
+                               leap(q0, ha0));
                 if constexpr ((F & kPoint) != 0) hc = add_transmitter(p, s, hc, y, xh, sr, sc, s1);
                 if constexpr (EDGE) {
                     // right halo on column W-1: Mur from lane 63's own block
@@ -934,7 +942,7 @@ __device__ __forceinline__ void fwd2_wave(const StepParams &p, int lane, int grp
             if (r >= r0 && r < r1) {
                 const float cw = from_west(cm1, hcm1), ce = from_east(cm1, hcm1);
                 d = __fmaf_rn(km1, lap5(cw, ce, cm2, c0, cm1, p.idx2, p.idy2, p.cen),
-                              2.0f * cm1 - bm);
+                              leap(cm1, bm));
                 if constexpr ((F & kPoint) != 0) d = add_transmitter(p, s, d, r, x, sr, sc, s2);
                 if constexpr (EDGE) {
                     if (r >= 1 && r <= H - 2) d = mur_sides(d, cm1, x, W, sm1);
@@ -1035,7 +1043,8 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
     constexpr bool LAP = (F & kLap) != 0;
     const int W = p.nx, H = p.ny;
     const int x = wave * kLanes - p.wide_off + lane;
-    const bool owned = (x >= 0) & (x < W);
+    // only the first and last wavefronts of a row (XE) reach past the grid
+    const bool owned = !XE || ((x >= 0) & (x < W));
     // Laplacian histories need values outside the grid to be zero: lanes
     // beyond the side columns and rows beyond the top and bottom (they only
     // feed boundary cells, which the Mur updates replace, so the fields do
@@ -1173,7 +1182,7 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                 // that the lane shifts overwrite in place
                 const float uw = from_west(u, hw->v[j - 1]), ue = from_east(u, he->v[j - 1]);
                 const float lp = lap5(uw, ue, us, un, u, p.idx2, p.idy2, p.cen);
-                float v = __fmaf_rn(kh[j - 1], lp, 2.0f * u - old);
+                float v = __fmaf_rn(kh[j - 1], lp, leap(u, old));
                 if (__builtin_expect((unsigned)(rho - sr) <= (unsigned)src_extra, 0))
                     v = add_transmitter(p, s, v, rho, x, sr, sc, sv[j - 1]);
                 if constexpr (XE) {
@@ -1550,7 +1559,7 @@ __device__ __forceinline__ void adj2_wave(const StepParams &p, int lane, int wav
                 const float Pm = km * bm, Pp = kp * bp;
                 const float pw = from_west(P0, ph0), pe = from_east(P0, ph0);
                 c0 = __fmaf_rn(dt2, lap5(pw, pe, Pm, Pp, P0, p.idx2, p.idy2, p.cen),
-                               2.0f * b0 - a0);
+                               leap(b0, a0));
                 if (inject && e0 >= 0) c0 += injected(res1, e0);
                 if constexpr (EDGE) {
                     if (y >= 1 && y <= H - 2) c0 = mur_sides(c0, b0, x, W, s0);
@@ -1561,7 +1570,7 @@ __device__ __forceinline__ void adj2_wave(const StepParams &p, int lane, int wav
                 const float he = hl ? P0 : phw;
                 hc = __fmaf_rn(dt2, lap5(hw, he, kqm1 * qm1, kqp * qp, ph0, p.idx2, p.idy2,
                                          p.cen),
-                               2.0f * q0 - ha0);
+                               leap(q0, ha0));
                 if (inject && eh0 >= 0) hc += injected(res1, eh0);
                 if constexpr (EDGE) {
                     if (h_right_edge) {
@@ -1599,7 +1608,7 @@ __device__ __forceinline__ void adj2_wave(const StepParams &p, int lane, int wav
             if (r >= r0 && r < r1) {
                 const float pw = from_west(pcm1, hpcm1), pe = from_east(pcm1, hpcm1);
                 d = __fmaf_rn(dt2, lap5(pw, pe, pcm2, pc0, pcm1, p.idx2, p.idy2, p.cen),
-                              2.0f * cm1 - bm);
+                              leap(cm1, bm));
                 if (inject && em1 >= 0) d += injected(res2, em1);
                 if constexpr (EDGE) {
                     if (r >= 1 && r <= H - 2) d = mur_sides(d, cm1, x, W, sm1);
@@ -1767,7 +1776,8 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     constexpr bool FIELD = IMG == 1 || IMG == 3;
     const int W = p.nx, H = p.ny;
     const int x = wave * kLanes - p.wide_off + lane;
-    const bool owned = (x >= 0) & (x < W);
+    // only the first and last wavefronts of a row (XE) reach past the grid
+    const bool owned = !XE || ((x >= 0) & (x < W));
     const int xo = min(max(x, 0), W - 1) * 4;
     const int so = owned ? x * 4 : kDropOffset;
     int r0, r1;
@@ -1924,7 +1934,8 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
             for (int j = 1; j <= KL; ++j) {
                 if constexpr (FIELD) {
                     const int fr = y - j + 1;  // row of level j's newest imaging field
-                    fh[j][0] = (owned & (fr >= 0) & (fr < H)) ? f[j - 1] : 0.0f;
+                    // segments away from the top and bottom (!YE) image inside the grid
+                    fh[j][0] = (owned & (!YE || ((fr >= 0) & (fr < H)))) ? f[j - 1] : 0.0f;
                 } else {
                     fh[j][0] = f[j - 1];
                 }
@@ -1942,7 +1953,7 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 const float Pn = kh[j - 1] * lv[j - 1][0];
                 const float pw = from_west(P0, hw->v[j - 1].x), pe = from_east(P0, he->v[j - 1].x);
                 float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, p.idx2, p.idy2, p.cen),
-                                    2.0f * u - old);
+                                    leap(u, old));
                 const int e = eh[j - 1];
                 v = inject_lanes(p, p.lv_resid[j - 1], res_off, e, lane, v);
                 if constexpr (XE) {
