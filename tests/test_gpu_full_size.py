@@ -17,10 +17,14 @@ Both use a 300 kHz Ricker and the truth blurred by 10 mm as the start.
   and gradient (interior mask) rel-L2 <= 5e-4 (the oracle keeps every
   forward field, which at 1024^2 x 6000 steps would take 100 GB of host
   memory).
-
-The pixel-by-pixel check (16 seeded pixels, h = 0.1 m/s) runs in float64 on
-the oracle (tests/test_oracle.py): in float32 a 0.1 m/s change of one pixel
-moves the misfit by less than the rounding of the traces.
+* The 1024^2 gradient cell by cell against the oracle: two shots for 1500
+  steps (25 GB of oracle history), traces, misfit and gradient inside the
+  ring.
+* 16 seeded pixels of the 512^2 gradient against central differences of the
+  device misfit with +-20 / +-10 m/s steps (rel-L2 <= 1e-2). The same check
+  at h = 0.1 m/s runs in float64 on the oracle (tests/test_oracle.py): in
+  float32 a 0.1 m/s change of one pixel moves the misfit by less than the
+  rounding of the traces.
 """
 
 import numpy as np
