@@ -143,6 +143,14 @@ typedef struct swfd_forward_args {
     int32_t hist_lap; /* 1: history slots receive Laplacians (see above) */
     float *work4, *work5; /* optional: with samples, up to 4 steps per launch */
     const float *src_weights; /* optional (S, 4): bilinear sources, as in swfd_step_args */
+    /* optional boundary ring (no counterpart in the reference): the boundary
+     * cells of the field after step j, for j in [bnd_first, bnd_first +
+     * bnd_slots), go to slot j - bnd_first, each slot (S, swfd_boundary_floats)
+     * floats per shot: row 0, row ny-1, column 0 (rows 1..ny-2), column nx-1
+     * (rows 1..ny-2). Lets swfd_adjoint reconstruct the forward fields. Not in
+     * batched runs. */
+    float *bnd;
+    int32_t bnd_first, bnd_slots;
 } swfd_forward_args;
 
 /* A run of adjoint steps n = step_hi - 1 down to step_lo with residual
@@ -168,7 +176,33 @@ typedef struct swfd_adjoint_args {
     float *grad;
     float *workspace; /* swfd_adjoint_workspace_floats() floats, needed with hist */
     int32_t hist_lap; /* 1: slots hold Lap(field): the imaging adds lam' * slot */
+    /* Optional, instead of hist (no counterpart in the reference: replaces its
+     * stored wavefield, invert_ring.py:442-455, with a reconstruction): the
+     * forward fields are rebuilt backwards in time inside the adjoint launches,
+     * U(j-2) = 2 U(j-1) - U(j) + dt2 c^2 Lap(U(j-1)) + source of step j in the
+     * interior and the boundary ring (see swfd_forward_args) on the boundary,
+     * starting from recon_next / recon_cur = the forward fields after steps
+     * step_hi-1 / step_hi-2 (read only); step n >= 1 images against the
+     * reconstructed field after step n-1. recon_work0..3: four fields for the
+     * reconstructed pairs; src_*: the forward run's sources. Needs full-width
+     * launches (see swfd_steps_per_launch) and work3; not in batched runs. */
+    const float *recon_next, *recon_cur;
+    float *recon_work0, *recon_work1, *recon_work2, *recon_work3;
+    const float *bnd;
+    int32_t bnd_first, bnd_slots;
+    const int32_t *src_rows, *src_cols;
+    const float *src_values;
+    const float *src_weights;
 } swfd_adjoint_args;
+
+/* Floats per shot in one slot of the boundary ring of an ny x nx grid:
+ * 2 nx + 2 (ny - 2) (no counterpart in the reference). */
+size_t swfd_boundary_floats(int32_t ny, int32_t nx);
+
+/* 1 when swfd_adjoint can reconstruct the forward fields on an ny x nx grid
+ * (full-width launches with work3), else 0 (no counterpart in the
+ * reference). */
+int32_t swfd_adjoint_recon(int32_t ny, int32_t nx);
 
 /* 1 when runs on an ny x nx grid (with work3) support Laplacian histories
  * (hist_lap), else 0 (no counterpart in the reference). */

@@ -56,7 +56,8 @@ class ForwardArgs(ctypes.Structure):
                 ("src_cols", _fp), ("src_values", _fp), ("samples", _fp), ("hist", _fp),
                 ("hist_first", _c_int), ("hist_every", _c_int), ("hist_slots", _c_int),
                 ("ckpt", _fp), ("ckpt_every", _c_int), ("ckpt_slots", _c_int),
-                ("hist_lap", _c_int), ("work4", _fp), ("work5", _fp), ("src_weights", _fp)]
+                ("hist_lap", _c_int), ("work4", _fp), ("work5", _fp), ("src_weights", _fp),
+                ("bnd", _fp), ("bnd_first", _c_int), ("bnd_slots", _c_int)]
 
 
 class AdjointArgs(ctypes.Structure):
@@ -65,7 +66,12 @@ class AdjointArgs(ctypes.Structure):
                 ("work2", _fp), ("work3", _fp), ("final_old", _fp), ("final_u", _fp), ("residual", _fp),
                 ("hist", _fp), ("hist_first", _c_int), ("hist_every", _c_int),
                 ("hist_slots", _c_int), ("image_scale", _c_float), ("grad", _fp),
-                ("workspace", _fp), ("hist_lap", _c_int)]
+                ("workspace", _fp), ("hist_lap", _c_int),
+                ("recon_next", _fp), ("recon_cur", _fp), ("recon_work0", _fp),
+                ("recon_work1", _fp), ("recon_work2", _fp), ("recon_work3", _fp),
+                ("bnd", _fp), ("bnd_first", _c_int), ("bnd_slots", _c_int),
+                ("src_rows", _fp), ("src_cols", _fp), ("src_values", _fp),
+                ("src_weights", _fp)]
 
 
 #: every symbol include/swfd.h declares
@@ -78,6 +84,7 @@ EXPORTS = (
     "swfd_adjoint_batch_prepare", "swfd_adjoint_batch", "swfd_tune", "swfd_tune_clear",
     "swfd_lap_history", "swfd_steps_per_launch", "swfd_footprint_rows",
     "swfd_residual_noalloc", "swfd_history_convert", "swfd_adjoint_launch_shape",
+    "swfd_boundary_floats", "swfd_adjoint_recon",
 )
 
 #: storage codes of swfd_history_convert (include/swfd.h)
@@ -143,6 +150,10 @@ def load_library(path=LIBRARY):
     lib.swfd_steps_per_launch.restype = _c_int
     lib.swfd_steps_per_launch.argtypes = [_c_int, _c_int, _c_int, _c_int]
     lib.swfd_adjoint_launch_shape.argtypes = [_c_int, _c_int, _c_int, vp]
+    lib.swfd_boundary_floats.restype = ctypes.c_size_t
+    lib.swfd_boundary_floats.argtypes = [_c_int, _c_int]
+    lib.swfd_adjoint_recon.restype = _c_int
+    lib.swfd_adjoint_recon.argtypes = [_c_int, _c_int]
     for name in ("swfd_prepare_medium", "swfd_step", "swfd_gather", "swfd_scatter",
                  "swfd_laplacian", "swfd_forward", "swfd_adjoint", "swfd_residual",
                  "swfd_residual_noalloc", "swfd_history_convert", "swfd_adjoint_launch_shape",

@@ -123,7 +123,26 @@ struct StepParams {
     // per row: bit w set when full-width wavefront w holds a receiver
     // footprint cell there (NULL: intermediate levels stored everywhere)
     const unsigned *fp_rows;
+    // boundary ring: the boundary cells of the forward field after step q in
+    // slot q - bnd_first (forward launches write it, reconstructing adjoint
+    // launches read it; NULL: none)
+    float *bnd;
+    int bnd_first, bnd_slots;
+    // reconstructing adjoint launches (first step n = src_index): the forward
+    // fields after steps n and n - 1, each level's reconstructed field (NULL:
+    // not stored), the first step that images, and the rows of the staged
+    // boundary columns
+    const float *r_next, *r_cur;
+    float *lv_rout[4];
+    int img_from, bcol_rows;
 };
+
+// Floats per shot in a slot of the boundary ring: row 0, row H - 1, then
+// columns 0 and W - 1 of rows 1 .. H - 2.
+__host__ __device__ __forceinline__ size_t bnd_floats(int H, int W)
+{
+    return 2 * (size_t)W + 2 * (size_t)(H - 2);
+}
 
 // 2u - u_old of the leapfrog update as one fused operation: 2u is exact, so
 // the single rounding of fma(2, u, -u_old) equals that of the separate
@@ -337,6 +356,7 @@ enum : unsigned {
     kOut2 = 16u,
     kGather = 32u,
     kLap = 64u,  // multi-step forward: store each level's input Laplacian
+    kBnd = 128u, // multi-step forward: footprint stores keep the boundary cells too
 };
 
 // Neighbour exchange inside a wavefront through DPP wavefront shifts: lane i
@@ -1018,6 +1038,18 @@ struct alignas(16) Edge4 {
 // [iteration parity][wavefront slot: 0 and nw + 1 are zero pads][west, east]
 typedef Edge4 EdgeRow[kMaxWideWaves + 2][2];
 
+// A prefetched slot value copied into a register of its own (early clobber):
+// the slot's register is then free for the refill, so a march keeps each slot
+// in one register across its loop and the latch needs no copy of a value the
+// refill just loaded (such a copy waits there for that load and, the counter
+// being shared, for every store before it).
+__device__ __forceinline__ float take(float v)
+{
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=&v"(r) : "v"(v));
+    return r;
+}
+
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, int voff, int soff)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
@@ -1041,7 +1073,9 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
                                           int seg, EdgeRow *ex)
 {
     constexpr bool LAP = (F & kLap) != 0;
+    constexpr bool BND = (F & kBnd) != 0;
     const int W = p.nx, H = p.ny;
+
     const int x = wave * kLanes - p.wide_off + lane;
     // only the first and last wavefronts of a row (XE) reach past the grid
     const bool owned = !XE || ((x >= 0) & (x < W));
@@ -1144,6 +1178,16 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
             if constexpr (SM == kStoreFootprint) {
                 // row and wavefront are uniform: a scalar load of the row's mask
                 stored = j >= KL - 2 || (((unsigned)((const kci *)p.fp_rows)[rho] >> wave) & 1u) != 0;
+                if constexpr (BND) {
+                    // the boundary cells too (the ring is taken from the fields):
+                    // rows 0 and H - 1 whole, columns 0 and W - 1 by their lanes
+                    if (!stored && YE && ((rho == 0) | (rho == H - 1))) stored = true;
+                    if (!stored && XE) {
+                        bstore(ro[j], v, ((x == 0) | (x == W - 1)) ? so : kDropOffset,
+                               rho * row_bytes);
+                        return;
+                    }
+                }
             }
             if (stored) bstore(ro[j], v, so, rho * row_bytes);
             if constexpr ((F & kOut2) != 0)
@@ -1798,10 +1842,6 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
     // with global loads off a scalar base (2 SGPRs each instead of a 4-SGPR
     // buffer descriptor: the adjoint is short of SGPRs)
     const float *emap = (const float *)p.cell_map;
-    // rows where this wavefront holds no receiver footprint cell (the
-    // footprint table's bit `wave` clear) skip the map: one scalar load per row
-    // instead of a vector load per lane (a ring crosses 1-4 of 16 wavefronts)
-    const kci *fpr = (const kci *)p.fp_rows;
     const float *mur = p.mur;
     const int mo = (x == 0) ? 0 : H * 4;
     auto mrow = [&](int yy) -> int {
@@ -1874,9 +1914,10 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
             qb[i] = bload(rb, xo, row_off(y + 1));
             qa[i] = bload(ra, xo, row_off(y));
             qk[i] = bload(rk, xo, row_off(y + 1));
-            qe[i] = __int_as_float(-1);
-            if (fpr == nullptr || (((unsigned)fpr[mrow(y) / 4] >> wave) & 1u) != 0u)
-                qe[i] = gload_off(emap, xo + row_off(y));
+            // every iteration issues the same loads, so the compiler's waits
+            // count exactly (a conditional load makes them wait for loads
+            // issued after it)
+            qe[i] = gload_off(emap, xo + row_off(y));
             qm[i] = 0.0f;
             if constexpr (XE) qm[i] = gload_off(mur, mo + mrow(y));
 #pragma unroll
@@ -1917,13 +1958,11 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                 mh[j] = mh[j - 1];
                 eh[j] = eh[j - 1];
             }
-            float b1 = qb[i], a0 = qa[i], k1 = qk[i], e0 = qe[i], m0 = qm[i];
+            const float b1 = take(qb[i]), a0 = take(qa[i]), k1 = take(qk[i]), e0 = take(qe[i]);
+            const float m0 = XE ? take(qm[i]) : 0.0f;
             float f[KL];
 #pragma unroll
-            for (int j = 0; j < KL; ++j) f[j] = qf[i][j];
-            asm volatile("" : "+v"(b1), "+v"(a0), "+v"(k1), "+v"(e0), "+v"(m0));
-#pragma unroll
-            for (int j = 0; j < KL; ++j) asm volatile("" : "+v"(f[j]));
+            for (int j = 0; j < KL; ++j) f[j] = FIELD || IMG == 2 ? take(qf[i][j]) : 0.0f;
             __builtin_amdgcn_sched_barrier(0);
             fill(i, min(y + kPfA, y_end));
             lv[0][0] = b1;
@@ -1968,8 +2007,12 @@ __device__ __forceinline__ void adjm_wave(const StepParams &p, int lane, int wav
                     }
                 }
                 lv[j][0] = v;
-                if ((j >= KL - 1) & (rho >= r0) & (rho < r1) & (rho > 0))
-                    bstore(ro[j - 1], v, so, rho * row_bytes);
+                if (j >= KL - 1) {
+                    // stores of rows outside the segment are dropped, not
+                    // branched around (the same operations every iteration)
+                    const bool st = (rho >= r0) & (rho < r1) & (rho > 0);
+                    bstore(ro[j - 1], v, st ? so : kDropOffset, st ? rho * row_bytes : 0);
+                }
                 if constexpr (IMG != 0) {
                     // row rho - 1 = y - j: its term goes to racc[j - 1]
                     if (IMG == 3 || p.lv_in[j - 1] != nullptr) {
@@ -2079,6 +2122,371 @@ __global__ void __launch_bounds__(1024) adjm_batch_kernel(const StepParams *tabl
     const int b = (int)blockIdx.x;
     const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
     adjm_block<KL, IMG>(batch_params(table, e), b - first[e], ex);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint launches that rebuild the forward field backwards in time
+// ---------------------------------------------------------------------------
+//
+// The leapfrog is reversible in the interior: U(m-2) = 2 U(m-1) - U(m) +
+// dt^2 c^2 Lap(U(m-1)) + s(m), s(m) the transmitter's value of step m. The Mur
+// update of the boundary cells is not (solved for the older value it divides
+// by |k| < 1 every step), so the forward run keeps the boundary cells of every
+// step in a ring and the reconstruction takes them from there. Adjoint step m
+// images lam' against Lap(U(m-1)), the Laplacian that the reconstruction of
+// U(m-2) takes anyway: a launch of KL adjoint steps reads and writes a pair of
+// forward fields instead of reading KL stored ones, and no forward
+// recomputation is needed. The reconstruction rounds differently from the
+// forward run (about 1e-6 relative after thousands of steps); the adjoint
+// states are those of adjm_kernel bit for bit.
+//
+// Level j (1..KL) of a launch whose first step is n runs adjoint step
+// m = n - j + 1 exactly as adjm_kernel does and reconstructs uv[j] = U(n-j-1)
+// in the same row pipeline: at iteration y it takes row y - j + 1 from the
+// three newest rows of uv[j - 1] (uv[0] = U(n-1), marched as rows) and the row
+// of uv[j - 2] (U(n) for level 1, read per row). The imaging term of row
+// y - j (one row late, as in adjm_kernel) uses the Laplacian kept from the
+// previous iteration. Lanes and rows outside the grid hold zeros, so the
+// imaging Laplacian sees the zero padding the stored fields had. Boundary
+// cells come from the ring: the two columns staged in LDS per shot (rows of
+// the segment plus the pipeline halo, every level), rows 0 and ny - 1 read
+// when the level reaches them.
+
+// (first adjoint step of the launch, level j) -> index of the reconstructed
+// field U(n - j - 1)
+__device__ __forceinline__ int recon_field(int n, int j) { return n - j - 1; }
+
+template <int KL, bool XE, bool YE, bool EVERY>
+__device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wave, int grp,
+                                          int seg, AdjEdgeRow *ex, float *bcol, float *brow)
+{
+    // staged rows: [0][level][x] row 0, [brow_second][level][x] row H - 1 (the
+    // same slot unless a segment's march can reach both: H <= rows + 3 KL)
+    const int brow_second = p.ny <= p.acc_rows + 3 * KL ? 1 : 0;
+    const int W = p.nx, H = p.ny;
+    const int x = wave * kLanes - p.wide_off + lane;
+    const bool owned = !XE || ((x >= 0) & (x < W));
+    const int xo = min(max(x, 0), W - 1) * 4;
+    const int so = owned ? x * 4 : kDropOffset;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_begin = max(r0 - (KL - 1), 0);
+    // level KL reaches row r1 - 1; imaging one row late needs one more
+    const int y_end = r1 + KL - 1;
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int row_bytes = W * 4;
+    auto row_off = [&](int yy) -> int {
+        if constexpr (YE) return min(max(yy, 0), H - 1) * row_bytes;
+        else return yy * row_bytes;
+    };
+    // forward-field rows outside the grid are zero
+    auto in_rows = [&](int yy) -> bool {
+        if constexpr (YE) return (yy >= 0) & (yy < H);
+        else return true;
+    };
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
+    const float *emap = (const float *)p.cell_map;
+    const float *mur = p.mur;
+    const int mo = (x == 0) ? 0 : H * 4;
+    auto mrow = [&](int yy) -> int {
+        if constexpr (YE) return min(max(yy, 0), H - 1) * 4;
+        else return yy * 4;
+    };
+    const float dt2 = p.dt2;
+    float kb_x = 0.0f, kt_x = 0.0f;
+    if constexpr (YE) {
+        kb_x = mur[2 * H + xo / 4];
+        kt_x = mur[2 * H + W + xo / 4];
+    }
+    const int w_slot = lane == 0 ? wave : wave + 1;
+    const int w_side = lane == 0 ? 1 : 0;
+    const int e_slot = lane == kLanes - 1 ? wave + 2 : wave + 1;
+    const int e_side = lane == kLanes - 1 ? 0 : 1;
+    const bool edge_lane = (lane == 0) | (lane == kLanes - 1);
+    const int my_side = lane == kLanes - 1 ? 1 : 0;
+    const int n_top = p.src_index;
+    const size_t bnd_n = bnd_floats(H, W);
+    // staged boundary columns: bcol[(j * 2 + side) * nb + row - yb] for level
+    // j + 1, side 0 = column 0, 1 = column W - 1
+    const int yb = y_begin - KL + 1;
+    const int nb = p.bcol_rows;
+    const int col_side = (x == 0) ? 0 : 1;
+    const bool side_lane = (x == 0) | (x == W - 1);
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    float *acc = adjm_acc;
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        // boundary columns of this shot's levels (every thread of the
+        // workgroup; the barrier before the march orders the writes)
+        for (int i = (int)threadIdx.x; i < KL * 2 * nb; i += (int)blockDim.x) {
+            const int r = i % nb, side = (i / nb) & 1, j = i / (2 * nb);
+            const int row = yb + r;
+            const int q = recon_field(n_top, j + 1);
+            const int k = q - p.bnd_first;
+            float v = 0.0f;
+            if ((row >= 1) & (row <= H - 2) & (q >= 0) & (k >= 0) & (k < p.bnd_slots))
+                v = p.bnd[((size_t)k * p.n_shots + s) * bnd_n + 2 * W + (size_t)side * (H - 2) +
+                          row - 1];
+            bcol[i] = v;
+        }
+        // rows 0 and H - 1 of the levels, for segments that reach them (staged
+        // too: a conditional vector load in the march would make the compiler
+        // wait for every outstanding load after it)
+        if constexpr (YE) {
+            for (int which = 0; which < 2; ++which) {
+                const bool need = which == 0 ? (yb <= 0) : (y_end + 1 >= H - 1);
+                if (!need) continue;  // uniform
+                float *dst = brow + (size_t)(which == 0 ? 0 : brow_second) * KL * W;
+                for (int i = (int)threadIdx.x; i < KL * W; i += (int)blockDim.x) {
+                    const int xx = i % W, j = i / W;
+                    const int q = recon_field(n_top, j + 1);
+                    const int k = q - p.bnd_first;
+                    float v = 0.0f;
+                    if ((q >= 0) & (k >= 0) & (k < p.bnd_slots))
+                        v = p.bnd[((size_t)k * p.n_shots + s) * bnd_n + (size_t)which * W + xx];
+                    dst[i] = v;
+                }
+            }
+        }
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rua = make_rsrc(p.r_next + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rub = make_rsrc(p.r_cur + soff, plane_bytes);
+        __amdgpu_buffer_rsrc_t ro[KL], rou[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            // integer arithmetic: the pointers may be NULL (not stored)
+            ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
+                              plane_bytes);
+            rou[j] = make_rsrc((const float *)((uintptr_t)p.lv_rout[j] + soff * sizeof(float)),
+                               plane_bytes);
+        }
+        const size_t res_off = (size_t)s * p.n_rec;
+        const int sr = p.src_rows[s], sc = p.src_cols[s];
+        const int src_extra = transmitter_rows(p);
+        // lam and U of each level's newest three rows (level 0: the inputs,
+        // newest row y + 1 at iteration y); P = c^2 lam formed where needed
+        float lv[KL + 1][3], uv[KL + 1][3];
+        // c^2 of rows y + 1, ..., y - KL; scatter entry and Mur side
+        // coefficient of rows y, ..., y - KL + 1
+        float kh[KL + 2], mh[KL];
+        int eh[KL];
+        // imaging terms of rows y - 1 .. y - KL summed over the levels so far,
+        // and each level's Laplacian of the previous iteration's row
+        float racc[KL], lsave[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            racc[j] = lsave[j] = 0.0f;
+            mh[j] = 0.0f;
+            eh[j] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j <= KL; ++j)
+            lv[j][0] = lv[j][1] = lv[j][2] = uv[j][0] = uv[j][1] = uv[j][2] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KL + 2; ++j) kh[j] = 0.0f;
+        lv[0][0] = bload(rb, xo, row_off(y_begin));
+        lv[0][1] = bload(rb, xo, row_off(y_begin - 1));
+        {
+            const float u0 = bload(rub, xo, row_off(y_begin));
+            const float um = bload(rub, xo, row_off(y_begin - 1));
+            uv[0][0] = (owned & in_rows(y_begin)) ? u0 : 0.0f;
+            uv[0][1] = (owned & in_rows(y_begin - 1)) ? um : 0.0f;
+        }
+        kh[0] = bload(rk, xo, row_off(y_begin));
+        kh[1] = bload(rk, xo, row_off(y_begin - 1));
+        float qb[kPfA], qa[kPfA], qk[kPfA], qe[kPfA], qm[kPfA], qu[kPfA], qv[kPfA];
+        auto fill = [&](int i, int y) {
+            qb[i] = bload(rb, xo, row_off(y + 1));
+            qa[i] = bload(ra, xo, row_off(y));
+            qk[i] = bload(rk, xo, row_off(y + 1));
+            qu[i] = bload(rub, xo, row_off(y + 1));
+            qv[i] = bload(rua, xo, row_off(y));
+            // every iteration issues the same loads, so the compiler's waits
+            // count exactly (a conditional load makes them wait for loads
+            // issued after it)
+            qe[i] = gload_off(emap, xo + row_off(y));
+            qm[i] = 0.0f;
+            if constexpr (XE) qm[i] = gload_off(mur, mo + mrow(y));
+        };
+#pragma unroll
+        for (int i = 0; i < kPfA; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            fill(i, y_begin + i);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (edge_lane) {
+            AdjEdge e;
+            e.v[0] = make_float2(kh[0] * lv[0][0], uv[0][0]);
+#pragma unroll
+            for (int j = 1; j < kMaxLevels; ++j) e.v[j] = make_float2(0.0f, 0.0f);
+            ex[y_begin & 1][wave + 1][my_side] = e;
+        }
+        __syncthreads();
+        auto iteration = [&](int y, int i) {
+#pragma unroll
+            for (int j = 0; j <= KL; ++j) {
+                lv[j][2] = lv[j][1];
+                lv[j][1] = lv[j][0];
+                uv[j][2] = uv[j][1];
+                uv[j][1] = uv[j][0];
+            }
+#pragma unroll
+            for (int j = KL + 1; j >= 1; --j) kh[j] = kh[j - 1];
+#pragma unroll
+            for (int j = KL - 1; j >= 1; --j) {
+                mh[j] = mh[j - 1];
+                eh[j] = eh[j - 1];
+            }
+            const float b1 = take(qb[i]), a0 = take(qa[i]), k1 = take(qk[i]), e0 = take(qe[i]),
+                        u1 = take(qu[i]), ua = take(qv[i]);
+            const float m0 = XE ? take(qm[i]) : 0.0f;
+            __builtin_amdgcn_sched_barrier(0);
+            fill(i, min(y + kPfA, y_end));
+            lv[0][0] = b1;
+            uv[0][0] = (owned & in_rows(y + 1)) ? u1 : 0.0f;
+            kh[0] = k1;
+            eh[0] = owned ? __float_as_int(e0) : -1;
+            mh[0] = m0;
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                const int rho = y - j + 1;
+                const int m = n_top - j + 1;  // adjoint step of the level
+                const AdjEdge *hw = &ex[y & 1][w_slot][w_side];
+                const AdjEdge *he = &ex[y & 1][e_slot][e_side];
+                // adjoint step m (adjm_kernel's arithmetic)
+                const float u = lv[j - 1][1];
+                const float old = j == 1 ? a0 : lv[j - 2][2];
+                const float Ps = kh[j + 1] * lv[j - 1][2];
+                const float P0 = kh[j] * u;
+                const float Pn = kh[j - 1] * lv[j - 1][0];
+                const float pw = from_west(P0, hw->v[j - 1].x), pe = from_east(P0, he->v[j - 1].x);
+                float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, p.idx2, p.idy2, p.cen),
+                                    leap(u, old));
+                v = inject_lanes(p, p.lv_resid[j - 1], res_off, eh[j - 1], lane, v);
+                if constexpr (XE) {
+                    if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
+                }
+                if constexpr (YE) {
+                    if (rho == H - 1) v = mur_row(mur, W, H, x, kt_x, u, lv[j - 1][2], lv[j][1], 2);
+                    if (rho == 1) {
+                        const float v0 = mur_row(mur, W, H, x, kb_x, lv[j - 1][2], u, v, 0);
+                        lv[j][1] = v0;
+                        if ((j >= KL - 1) & (r0 == 0)) bstore(ro[j - 1], v0, so, 0);
+                    }
+                }
+                lv[j][0] = v;
+                if (j >= KL - 1) {
+                    // stores of rows outside the segment are dropped, not
+                    // branched around (the same operations every iteration)
+                    const bool st = (rho >= r0) & (rho < r1) & (rho > 0);
+                    bstore(ro[j - 1], v, st ? so : kDropOffset, st ? rho * row_bytes : 0);
+                }
+                // the forward field U(m - 2) from U(m - 1) (uv[j - 1]) and U(m)
+                const float U0 = uv[j - 1][1];
+                const float Uold = j == 1 ? ua : uv[j - 2][2];
+                const float Uw = from_west(U0, hw->v[j - 1].y), Ue = from_east(U0, he->v[j - 1].y);
+                const float Lr = lap5(Uw, Ue, uv[j - 1][2], uv[j - 1][0], U0, p.idx2, p.idy2, p.cen);
+                float w = __fmaf_rn(dt2 * kh[j], Lr, leap(U0, Uold));
+                if (__builtin_expect((unsigned)(rho - sr) <= (unsigned)src_extra, 0))
+                    w = add_transmitter(p, s, w, rho, x, sr, sc, ((const kcf *)p.src_values)[m]);
+                const int q = recon_field(n_top, j);
+                if constexpr (XE) {
+                    const float bc = bcol[((j - 1) * 2 + col_side) * nb + rho - yb];
+                    if (side_lane & (rho >= 1) & (rho <= H - 2)) w = bc;
+                }
+                if constexpr (YE) {
+                    if ((rho == 0) | (rho == H - 1)) {  // uniform
+                        const int which = rho == 0 ? 0 : brow_second;
+                        w = brow[((size_t)which * KL + j - 1) * W + xo / 4];
+                    }
+                    w = in_rows(rho) ? w : 0.0f;
+                }
+                if constexpr (XE) w = owned ? w : 0.0f;
+                if (q < 0) w = 0.0f;  // U(-1) = 0
+                uv[j][0] = w;
+                if (j >= KL - 1) {
+                    const bool st = (rho >= r0) & (rho < r1);
+                    bstore(rou[j - 1], w, st ? so : kDropOffset, st ? rho * row_bytes : 0);
+                }
+                // imaging of row rho - 1: lam' of step m against Lap(U(m - 1))
+                if (EVERY || m >= p.img_from) racc[j - 1] += lv[j][1] * lsave[j - 1];
+                lsave[j - 1] = Lr;
+            }
+            // row y - KL has the terms of every level: add it to the image
+            const int Rf = y - KL;
+            if ((Rf >= r0) & (Rf < r1) & owned) acc[(Rf - r0) * W + x] += racc[KL - 1];
+#pragma unroll
+            for (int k = KL - 1; k >= 1; --k) racc[k] = racc[k - 1];
+            racc[0] = 0.0f;
+            if (edge_lane) {
+                AdjEdge e;
+#pragma unroll
+                for (int j = 0; j < kMaxLevels; ++j)
+                    e.v[j] = j < KL ? make_float2(kh[j] * lv[j][0], uv[j][0]) : make_float2(0.0f, 0.0f);
+                ex[(y + 1) & 1][wave + 1][my_side] = e;
+            }
+            __syncthreads();
+        };
+        for (int y = y_begin; y <= y_end; y += kPfA) {
+#pragma unroll
+            for (int i = 0; i < kPfA; ++i) iteration(y + i, i);
+        }
+    }
+}
+
+// Workgroup b (row segment, shot group) of a reconstructing adjoint launch.
+template <int KL, bool EVERY>
+__device__ __forceinline__ void adjr_block(const StepParams &p, int b, AdjEdgeRow *ex)
+{
+    const int wblk = (b % kXcds) * p.chunk + b / kXcds;
+    if (wblk >= p.n_work) return;  // uniform over the workgroup
+    const int t = (int)threadIdx.x;
+    if (t < 2 * 2 * kMaxLevels) {  // zero pads at both ends, both parities
+        const int par = t / (2 * kMaxLevels), side = (t / kMaxLevels) & 1, j = t % kMaxLevels;
+        ex[par][0][side].v[j] = make_float2(0.0f, 0.0f);
+        ex[par][p.wide_waves + 1][side].v[j] = make_float2(0.0f, 0.0f);
+    }
+    const int lane = t & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(t / kLanes);
+    const int grp = wblk % p.n_groups;
+    const int seg = wblk / p.n_groups;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int W = p.nx;
+    const int n_acc = (r1 - r0) * W;
+    for (int i = t; i < n_acc; i += (int)blockDim.x) adjm_acc[i] = 0.0f;
+    float *bcol = adjm_acc + (size_t)p.acc_rows * W;
+    float *brow = bcol + (size_t)KL * 2 * p.bcol_rows;
+    // (the first barrier of the march orders these writes before any use)
+    const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
+    const bool ye = (r0 < 2 * KL) || (r1 + KL >= p.ny);
+    if (xe) {
+        if (ye) adjr_wave<KL, true, true, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
+        else adjr_wave<KL, true, false, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
+    } else {
+        if (ye) adjr_wave<KL, false, true, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
+        else adjr_wave<KL, false, false, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
+    }
+    __syncthreads();
+    // partial[grp] += scale * segment image, per cell
+    const int x = wave * kLanes - p.wide_off + lane;
+    if ((x >= 0) & (x < W)) {
+        gf *gp = gstore_ptr(p.g_part) + (size_t)grp * ((size_t)p.ny * W);
+        for (int i = 0; i < r1 - r0; ++i) {
+            gf *cell = gp + (size_t)(r0 + i) * W + x;
+            *cell = __fmaf_rn(p.img_scale, adjm_acc[i * W + x], *cell);
+        }
+    }
+}
+
+template <int KL, bool EVERY>
+__global__ void __launch_bounds__(1024) adjr_kernel(StepParams p)
+{
+    __shared__ AdjEdgeRow ex[2];
+    adjr_block<KL, EVERY>(p, (int)blockIdx.x, ex);
 }
 
 // g[i] += sum over groups of g_part[k][i], in group order.
@@ -2214,6 +2622,32 @@ __global__ void fill_kernel(float *p, size_t n, float v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Boundary cells of `field` (S, H, W) into a slot of the boundary ring (S,
+// bnd_floats): after launches that store their fields in full.
+__global__ void bnd_extract_kernel(const float *field, int S, int H, int W, float *slot)
+{
+    const size_t P = bnd_floats(H, W);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)S * P) return;
+    const size_t s = i / P;
+    const int k = (int)(i % P);
+    int row, col;
+    if (k < W) {
+        row = 0;
+        col = k;
+    } else if (k < 2 * W) {
+        row = H - 1;
+        col = k - W;
+    } else if (k < 2 * W + H - 2) {
+        row = k - 2 * W + 1;
+        col = 0;
+    } else {
+        row = k - 2 * W - (H - 2) + 1;
+        col = W - 1;
+    }
+    slot[i] = field[s * H * W + (size_t)row * W + col];
 }
 
 // Reduced-precision history storage: float32 <-> 16-bit codes, both rounded
@@ -2697,6 +3131,18 @@ template <int KL, typename Fn>
 int fwdm_variant(int sm, unsigned f, Fn &&fn)
 {
 #define SWFD_FWDM(SMV, FV) fn(fwdm_kernel<KL, SMV, FV>, fwdm_batch_kernel<KL, SMV, FV>)
+    if ((f & kBnd) != 0) {
+        // boundary cells kept by footprint stores (point sources, optional
+        // checkpoint copies, gathers after the launch)
+        if constexpr (KL > 2) {
+            if (sm == kStoreFootprint && (f & ~kOut2) == (kPoint | kBnd)) {
+                if ((f & kOut2) != 0) SWFD_FWDM(kStoreFootprint, kPoint | kOut2 | kBnd);
+                else SWFD_FWDM(kStoreFootprint, kPoint | kBnd);
+                return SWFD_OK;
+            }
+        }
+        return fail(SWFD_EINVAL, "boundary cells: footprint stores of 3-4 levels only");
+    }
     if ((f & kLap) != 0) {
         // Laplacian histories: the fields of the last two levels are stored
         if (f != (kPoint | kLap) || sm != kStoreLastTwo)
@@ -2782,10 +3228,14 @@ int multi_shape(StepParams &p, int gather_pairs_count, int &threads, unsigned &f
     }
     if (gather_pairs_count > 0) f |= kGather;
     if (p.hist_lap) f |= kLap;
+
     const bool footprint = all && p.fp_rows != nullptr && p.levels > 2;
     sm = (p.hist_lap && last_two) ? kStoreLastTwo
          : (footprint ? kStoreFootprint
                       : (all ? kStoreAll : (last_two ? kStoreLastTwo : kStoreAny)));
+    // a ring is taken from the stored fields: levels kept only where receivers
+    // sit keep their boundary cells too
+    if (p.bnd != nullptr && sm == kStoreFootprint) f |= kBnd;
     return blocks;
 }
 
@@ -3073,6 +3523,88 @@ int launch_adj_multi(StepParams &p, hipStream_t stream)
     return SWFD_OK;
 }
 
+// Dynamic LDS a reconstructing adjoint workgroup may use: the segment image
+// and the staged boundary columns (one such workgroup per CU).
+constexpr size_t kMaxAdjrLds = 156 * 1024;
+
+// Shape of a reconstructing adjoint launch of p.levels (1..4) steps (the
+// geometry of the full-width adjoint); returns the workgroup count (negative
+// error code on failure), the threads per workgroup and the LDS bytes.
+int adjr_shape(StepParams &p, int &threads, size_t &lds)
+{
+    const int L = p.levels;
+    if (L < 1 || L > kMaxLevels || !wide_fits(p.nx) || p.cell_map == nullptr)
+        return -fail(SWFD_EINVAL, "reconstructing adjoint: 1-4 levels, nx <= 1023, scatter table");
+    if ((size_t)p.ny * p.nx * sizeof(float) >= (size_t)1 << 31)
+        return -fail(SWFD_EINVAL, "reconstructing adjoint: a field plane must be under 2 GiB");
+    for (int j = 0; j < L; ++j)
+        if (p.lv_resid[j] == nullptr) return -fail(SWFD_EINVAL, "reconstructing adjoint: residual");
+    const int lo = std::max(L - 2, 0);
+    for (int j = lo; j < L; ++j)
+        if (p.lv_out[j] == nullptr || p.lv_rout[j] == nullptr)
+            return -fail(SWFD_EINVAL, "reconstructing adjoint needs the last outputs");
+    if (p.g_part == nullptr || p.r_next == nullptr || p.r_cur == nullptr || p.bnd == nullptr ||
+        p.src_rows == nullptr || p.src_cols == nullptr || p.src_values == nullptr)
+        return -fail(SWFD_EINVAL, "reconstructing adjoint: fields, ring, sources or images missing");
+    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots);
+    p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
+    p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
+    p.seg = a.seg;
+    p.n_seg = a.n_seg;
+    p.n_groups = a.n_groups;
+    p.shots_per_group = a.shots_per_group;
+    p.acc_rows = a.acc_rows;
+    p.bcol_rows = a.acc_rows + 3 * L;
+    p.n_work = p.n_seg * p.n_groups;
+    p.chunk = (p.n_work + kXcds - 1) / kXcds;
+    p.n_stencil_blocks = p.chunk * kXcds;
+    // image, staged boundary columns, staged rows 0 / H - 1 (both only when a
+    // segment's march can reach both)
+    const size_t brows = (p.ny <= a.acc_rows + 3 * L ? 2 : 1) * (size_t)L * p.nx;
+    lds = ((size_t)a.acc_rows * p.nx + (size_t)L * 2 * p.bcol_rows + brows) * sizeof(float);
+    if (lds > kMaxAdjrLds)
+        return -fail(SWFD_EINVAL, "reconstructing adjoint: row segments too long for the LDS");
+    threads = p.wide_waves * kLanes;
+    return p.n_stencil_blocks;
+}
+
+template <int KL, bool EVERY>
+int adjr_launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StepParams &p)
+{
+    static bool big = false;
+    if (!big) {
+        SWFD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&adjr_kernel<KL, EVERY>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kMaxAdjrLds));
+        big = true;
+    }
+    hipLaunchKernelGGL((adjr_kernel<KL, EVERY>), grid, block, lds, st, p);
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
+}
+
+// p.levels (1..4) adjoint steps with the forward field rebuilt in one
+// full-width launch; every level whose step is at least p.img_from images.
+int launch_adj_recon(StepParams &p, hipStream_t st)
+{
+    int threads = 0;
+    size_t lds = 0;
+    const int blocks = adjr_shape(p, threads, lds);
+    if (blocks < 0) return -blocks;
+    const dim3 grid((unsigned)blocks), block((unsigned)threads);
+    const bool every = p.src_index - p.levels + 1 >= p.img_from;
+    switch (p.levels * 2 + (every ? 1 : 0)) {
+        case 2: return adjr_launch<1, false>(grid, block, lds, st, p);
+        case 3: return adjr_launch<1, true>(grid, block, lds, st, p);
+        case 4: return adjr_launch<2, false>(grid, block, lds, st, p);
+        case 5: return adjr_launch<2, true>(grid, block, lds, st, p);
+        case 6: return adjr_launch<3, false>(grid, block, lds, st, p);
+        case 7: return adjr_launch<3, true>(grid, block, lds, st, p);
+        case 8: return adjr_launch<4, false>(grid, block, lds, st, p);
+        default: return adjr_launch<4, true>(grid, block, lds, st, p);
+    }
+}
+
 // Number of imaging partial slices a step of this shape uses.
 int image_groups(int ny, int nx, int n_shots)
 {
@@ -3258,6 +3790,17 @@ struct ForwardRun {
         lap = a->hist_lap != 0;
         if (lap && batched)
             return fail(SWFD_EINVAL, "Laplacian history: not in batched runs");
+        if (a->bnd != nullptr) {
+            if (batched || lap)
+                return fail(SWFD_EINVAL, "boundary ring: not in batched runs or with Laplacian "
+                                         "histories");
+            if (a->bnd_first > a->step_begin ||
+                (long)a->bnd_first + a->bnd_slots < (long)a->step_begin + a->n_steps)
+                return fail(SWFD_EINVAL, "boundary ring: its slots must cover the run's steps");
+            // the ring is taken from each level's stored field
+            if (wide && max_levels > 2 && (a->work4 == nullptr || a->work5 == nullptr))
+                return fail(SWFD_EINVAL, "boundary ring needs work4 and work5");
+        }
         if (lap && (!wide || max_levels < 2))
             return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step launches "
                                      "(see swfd_lap_history) and work3");
@@ -3306,6 +3849,9 @@ struct ForwardRun {
         p.src_values = a->src_values;
         p.src_w = a->src_weights;
         p.src_index = n;
+        p.bnd = a->bnd;
+        p.bnd_first = a->bnd_first;
+        p.bnd_slots = a->bnd_slots;
         int gathers = 0;
         if (a->samples != nullptr && n_pend > 0 && !sep_gather) {
             p.gat_field = pend_field[0];
@@ -3344,7 +3890,9 @@ struct ForwardRun {
                 out = free_work(work, p_old, p_cur, taken);
                 taken = out;
             }
-            if (out == nullptr && sep_gather) out = j == 0 ? a->work4 : a->work5;
+            // a ring is taken from the stored fields: every level needs an output
+            if (out == nullptr && (sep_gather || a->bnd != nullptr))
+                out = j == 0 ? a->work4 : a->work5;
             p.lv_out[j] = out;
             p.lv_out2[j] = ckpt_out(n + j);
             if (sep_gather) p.lv_samples[j] = a->samples + (size_t)(n + j) * sr;
@@ -3361,6 +3909,36 @@ struct ForwardRun {
             p.f_out2_b = p.lv_out2[1];
         }
         return gathers;
+    }
+    // boundary ring of the steps of a launch, from its stored fields (every
+    // level of a full-width launch has an output; footprint stores keep the
+    // boundary cells)
+    int launch_boundary(const StepParams &p, hipStream_t st) const
+    {
+        if (a->bnd == nullptr) return SWFD_OK;
+        const int n = a->step_begin + k;
+        const float *outs[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
+        int L = 1;
+        if (next_is_wide()) {
+            L = p.levels;
+            for (int j = 0; j < L; ++j) outs[j] = p.lv_out[j];
+        } else {
+            outs[0] = p.f_out;
+            if (next_is_pair()) {
+                outs[1] = p.f_out_b;
+                L = 2;
+            }
+        }
+        const size_t total = (size_t)a->n_shots * bnd_floats(grid->ny, grid->nx);
+        for (int j = 0; j < L; ++j) {
+            const int slot = n + j - a->bnd_first;
+            if (outs[j] == nullptr) return fail(SWFD_EINVAL, "boundary ring: a level has no output");
+            hipLaunchKernelGGL(bnd_extract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                               0, st, outs[j], a->n_shots, grid->ny, grid->nx,
+                               a->bnd + (size_t)slot * total);
+            SWFD_HIP_CHECK(hipGetLastError());
+        }
+        return SWFD_OK;
     }
     // the gather of a launch's levels when they are gathered after it
     int launch_gather(const StepParams &p, hipStream_t st) const
@@ -3481,6 +4059,11 @@ struct AdjointRun {
     bool partials_used = false;
     const float *p_old = nullptr, *p_cur = nullptr;
     int n = 0;  // next step to run (descending)
+    // forward fields rebuilt in the launches: the pair (after step n, after
+    // step n - 1) and the buffers the reconstructed pairs rotate through
+    bool recon = false;
+    float *rwork[4] = {nullptr, nullptr, nullptr, nullptr};
+    const float *r_next = nullptr, *r_cur = nullptr;
 
     // batched: lockstep tables (no Laplacian histories)
     int init(const swfd_grid *g, const swfd_medium *m, const swfd_receivers *r,
@@ -3514,13 +4097,43 @@ struct AdjointRun {
         if (lap && (!wide || max_levels < 2))
             return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step adjoint "
                                      "launches (see swfd_lap_history) and work3");
+        recon = a->recon_next != nullptr || a->recon_cur != nullptr;
+        if (recon) {
+            if (batched) return fail(SWFD_EINVAL, "reconstruction: not in batched runs");
+            if (a->recon_next == nullptr || a->recon_cur == nullptr || a->recon_work0 == nullptr ||
+                a->recon_work1 == nullptr || a->recon_work2 == nullptr ||
+                a->recon_work3 == nullptr || a->bnd == nullptr || a->bnd_slots < 1 ||
+                a->src_rows == nullptr || a->src_cols == nullptr || a->src_values == nullptr)
+                return fail(SWFD_EINVAL, "reconstruction: fields, work buffers, ring or sources "
+                                         "missing");
+            if (a->hist != nullptr || lap)
+                return fail(SWFD_EINVAL, "reconstruction replaces the field history");
+            if (a->grad == nullptr || a->workspace == nullptr)
+                return fail(SWFD_EINVAL, "reconstruction: imaging needs grad and the workspace");
+            if (!wide || max_levels < 2)
+                return fail(SWFD_EINVAL, "reconstruction needs full-width adjoint launches (see "
+                                         "swfd_adjoint_recon) and work3");
+            rwork[0] = a->recon_work0;
+            rwork[1] = a->recon_work1;
+            rwork[2] = a->recon_work2;
+            rwork[3] = a->recon_work3;
+            r_next = a->recon_next;
+            r_cur = a->recon_cur;
+        }
         p_old = a->init_old;
         p_cur = a->init_u;
         n = a->step_hi - 1;
         return SWFD_OK;
     }
     bool done() const { return n < a->step_lo; }
-    int next_levels() const { return std::min(max_levels, n - a->step_lo + 1); }
+    int next_levels() const
+    {
+        const int rem = n - a->step_lo + 1;
+        int L = std::min(max_levels, rem);
+        // reconstructing runs end with two or more steps where they can
+        if (recon && L > 2 && rem - L == 1) --L;
+        return L;
+    }
     bool next_is_pair() const { return next_levels() >= 2; }
     bool next_is_wide() const { return wide && next_levels() >= 2; }
     long image_slot(int m) const
@@ -3537,7 +4150,6 @@ struct AdjointRun {
         fill = false;
         base_params(p, grid, medium, a->n_shots);
         attach_receivers(p, rec);
-        p.fp_rows = rec->footprint_rows;  // rows without receiver cells skip the map
         p.f_old = p_old;
         p.f_cur = p_cur;
         p.resid = a->residual + (size_t)n * sr;
@@ -3550,6 +4162,35 @@ struct AdjointRun {
         };
         const int L = next_levels();
         p.levels = L;
+        if (recon) {
+            float *out_c = free_work(work, p_old, p_cur);
+            float *out_d = free_work(work, p_old, p_cur, out_c);
+            float *rout_c = free_work(rwork, r_next, r_cur);
+            float *rout_d = free_work(rwork, r_next, r_cur, rout_c);
+            if (out_c == nullptr || out_d == nullptr || rout_c == nullptr || rout_d == nullptr)
+                return fail(SWFD_EINVAL, "adjoint: work buffers overlap state");
+            for (int j = 0; j < L; ++j) p.lv_resid[j] = a->residual + (size_t)(n - j) * sr;
+            const int lo = std::max(L - 2, 0);
+            p.lv_out[lo] = out_c;
+            p.lv_rout[lo] = rout_c;
+            if (L >= 2) {
+                p.lv_out[L - 1] = out_d;
+                p.lv_rout[L - 1] = rout_d;
+            }
+            p.src_index = n;
+            p.r_next = r_next;
+            p.r_cur = r_cur;
+            p.bnd = const_cast<float *>(a->bnd);
+            p.bnd_first = a->bnd_first;
+            p.bnd_slots = a->bnd_slots;
+            p.src_rows = a->src_rows;
+            p.src_cols = a->src_cols;
+            p.src_values = a->src_values;
+            p.src_w = a->src_weights;
+            p.img_from = 1;  // step 0 images U(-1) = 0
+            use_partials();
+            return SWFD_OK;
+        }
         if (L >= 2) {
             // level j is step n - j, imaged against the forward field before it
             float *out_c = free_work(work, p_old, p_cur);
@@ -3595,6 +4236,22 @@ struct AdjointRun {
     void advance(const StepParams &p)
     {
         const int L = p.levels;
+        if (recon) {
+            if (L >= 2) {
+                p_old = p.lv_out[L - 2];
+                p_cur = p.lv_out[L - 1];
+                r_next = p.lv_rout[L - 2];
+                r_cur = p.lv_rout[L - 1];
+            } else {
+                p_old = p_cur;
+                p_cur = p.lv_out[0];
+                r_next = r_cur;
+                r_cur = p.lv_rout[0];
+            }
+            last_multi = true;  // every launch imaged its own steps
+            n -= L;
+            return;
+        }
         if (L >= 2) {
             p_old = p.lv_out[L - 2];
             p_cur = p.lv_out[L - 1];
@@ -4164,7 +4821,7 @@ int swfd_forward(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         if (run.next_is_wide()) rc = launch_multi(p, st, gathers);
         else if (run.next_is_pair()) rc = launch_pair(p, st, gathers);
         else rc = launch_step(p, false, st, gathers);
-        if (rc || (rc = run.launch_gather(p, st))) return rc;
+        if (rc || (rc = run.launch_gather(p, st)) || (rc = run.launch_boundary(p, st))) return rc;
         run.advance(p);
     }
     return run.finish(stream);
@@ -4182,7 +4839,8 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
         bool fill = false;
         if ((rc = run.build(p, fill))) return rc;
         if (fill && (rc = run.launch_fill(st))) return rc;
-        if (run.next_is_wide()) rc = launch_adj_multi(p, st);
+        if (run.recon) rc = launch_adj_recon(p, st);
+        else if (run.next_is_wide()) rc = launch_adj_multi(p, st);
         else if (run.next_is_pair()) rc = launch_adj_pair(p, st);
         else rc = launch_step(p, true, st, 0);
         if (rc) return rc;
@@ -4289,6 +4947,24 @@ int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *
         out[5] = a.shots_per_group;
     }
     return SWFD_OK;
+}
+
+size_t swfd_boundary_floats(int32_t ny, int32_t nx)
+{
+    if (ny < 3 || nx < 3) return 0;
+    return bnd_floats(ny, nx);
+}
+
+int32_t swfd_adjoint_recon(int32_t ny, int32_t nx)
+{
+    if (ny < 3 || nx < 3) return 0;
+    static const int32_t any_map = 0;
+    swfd_receivers rec{};
+    rec.cell_map = &any_map;
+    static const float any_buffer = 0.0f;
+    float *w3 = const_cast<float *>(&any_buffer);
+    const ForwardPlan a = adjoint_plan(ny, nx, w3, &rec);
+    return (a.wide && a.levels >= 2) ? 1 : 0;
 }
 
 int32_t swfd_lap_history(int32_t ny, int32_t nx)

@@ -11,6 +11,12 @@ step; the reverse pass re-runs each segment from its checkpoint into a
 segment history and runs the adjoint over it. Recomputation reproduces the
 forward fields bit for bit, so the result equals the store-everything
 gradient while holding ``2 N/K + K + 1`` fields instead of ``N``.
+With ``reconstruct`` (the default of :class:`TorchLeastSquaresFWI` where the
+grid allows full-width launches) the reverse pass instead rebuilds the
+forward fields backwards in time inside the adjoint launches, from the
+checkpoints and a ring of the boundary cells of every step: no recomputation
+and no segment history; the rebuilt fields differ from the forward ones by
+rounding (about 1e-6 relative).
 The reference's approximate options (``store_every > 1``, reduced-precision
 ``wave_dtype``) keep its stored-history algorithm.
 """
@@ -25,8 +31,8 @@ import torch
 
 from .. import _native
 from ..torch_backend import (TorchFDTD2DBatch, _history_code, _source_values_tensor,
-                             adjoint_run, adjoint_workspace, auto_batch_size, convert_history,
-                             forward_run)
+                             adjoint_run, adjoint_workspace, auto_batch_size, boundary_floats,
+                             convert_history, forward_run)
 from .medium import c_from_m, m_from_c
 from .misfit import tikhonov, total_variation
 
@@ -57,6 +63,31 @@ def default_checkpoint_every(n_steps):
 #: float32 staging slots of a reduced-precision history (forward chunks are
 #: packed from them, adjoint chunks unpacked into them)
 HISTORY_STAGING_SLOTS = 4
+
+#: steps between the checkpoints a reconstructing gradient restarts its
+#: backward rebuild of the forward field from (the rebuild drifts from the
+#: forward fields by rounding only, about 1e-6 relative over thousands of steps)
+RECON_EVERY = 1024
+
+
+def reconstruction_available(grid_shape):
+    """Whether the adjoint can rebuild the forward fields on this grid
+    (full-width launches), and SWFD_RECON is not 0."""
+    if os.environ.get("SWFD_RECON", "1") == "0":
+        return False
+    return bool(_native.lib().swfd_adjoint_recon(int(grid_shape[0]), int(grid_shape[1])))
+
+
+def recon_bytes_per_shot(n_steps, grid_shape, n_receivers, checkpoint_every=None):
+    """Device memory one shot of a reconstructing gradient needs: 12 work fields
+    (forward, adjoint, rebuilt pairs), the shot-group images (one field per shot
+    at most), checkpoints, the boundary ring of every step, samples and
+    residuals."""
+    cells = grid_shape[0] * grid_shape[1]
+    K = int(checkpoint_every or RECON_EVERY)
+    n_seg = -(-int(n_steps) // K)
+    ring = int(n_steps) * (2 * grid_shape[1] + 2 * (grid_shape[0] - 2))
+    return (13 + 2 * (n_seg - 1)) * cells * 4 + ring * 4 + 3 * n_steps * n_receivers * 4
 
 
 def gradient_bytes_per_shot(n_steps, grid_shape, n_receivers, store_every=1, wave_dtype=None,
@@ -125,12 +156,20 @@ class CheckpointedGradient:
     gradient with respect to c^2.
     """
 
-    def __init__(self, solver, sampler, n_shots, n_steps, checkpoint_every=None):
+    def __init__(self, solver, sampler, n_shots, n_steps, checkpoint_every=None,
+                 reconstruct=False):
         self.solver = solver
         self.sampler = sampler
         self.n_shots = int(n_shots)
         self.n_steps = int(n_steps)
-        self.K = int(checkpoint_every or default_checkpoint_every(n_steps))
+        # reconstruct: the adjoint rebuilds the forward fields backwards in time
+        # from the checkpoints and a ring of the boundary cells of every step
+        # (no recomputation, no field history)
+        self.reconstruct = bool(reconstruct)
+        if self.reconstruct and not reconstruction_available(solver.model.shape):
+            raise ValueError("this grid has no reconstructing adjoint (full-width launches).")
+        default = RECON_EVERY if self.reconstruct else default_checkpoint_every(n_steps)
+        self.K = int(checkpoint_every or default)
         if self.K < 2:
             raise ValueError("checkpoint_every must be at least 2.")
         self.n_seg, n_ckpt, n_hist = segment_plan(self.n_steps, self.K)
@@ -140,7 +179,14 @@ class CheckpointedGradient:
         self.lam = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
         self.ckpt = (torch.empty((max(n_ckpt, 2), *shape), device=dev, dtype=_F32)
                      if n_ckpt > 0 else None)
-        self.hist = torch.empty((n_hist, *shape), device=dev, dtype=_F32)
+        if self.reconstruct:
+            self.hist = None
+            self.rwork = [torch.empty(shape, device=dev, dtype=_F32) for _ in range(4)]
+            self.ring = torch.empty((self.n_steps, self.n_shots,
+                                     boundary_floats(solver.model.shape)), device=dev,
+                                    dtype=_F32)
+        else:
+            self.hist = torch.empty((n_hist, *shape), device=dev, dtype=_F32)
         self.workspace = adjoint_workspace(solver, self.n_shots)
         self.samples = torch.empty((self.n_steps, self.n_shots, sampler.n_elements),
                                    device=dev, dtype=_F32)
@@ -155,11 +201,13 @@ class CheckpointedGradient:
         self.rows, self.cols, self.values = rows_i32, cols_i32, values
         self.src_weights = src_weights
         # the adjoint's fields are idle here: they keep the intermediate steps of
-        # each launch for the gather
+        # each launch for the gather (and the ring); a reconstructing run keeps
+        # the final pair (fields after steps N-2, N-1) in work[0], work[1]
         forward_run(self.solver, self.n_shots, rows_i32, cols_i32, values, self.n_steps,
                     samples=self.samples, sampler=self.sampler, work=self.work + self.lam[:2],
                     ckpt=self.ckpt, ckpt_every=self.K if self.ckpt is not None else 0,
-                    final=False, src_weights=src_weights)
+                    final=self.reconstruct, src_weights=src_weights,
+                    bnd=self.ring if self.reconstruct else None)
         return self.samples
 
     def lap_history(self):
@@ -179,6 +227,8 @@ class CheckpointedGradient:
         for t in self.lam[:2]:
             t.zero_()
         state = (self.lam[0], self.lam[1])
+        if self.reconstruct:
+            return self._backward_recon(residual_nsr, grad, image_scale, state)
         if self.lap_history():
             return self._backward_lap(residual_nsr, grad, image_scale, state)
         for j in range(self.n_seg - 1, -1, -1):
@@ -203,6 +253,26 @@ class CheckpointedGradient:
                         state=state, work=self.lam, hist=self.hist, hist_first=n0 - 2,
                         hist_every=1, image_scale=image_scale, grad=grad,
                         workspace=self.workspace)
+        return grad
+
+    def _backward_recon(self, residual_nsr, grad, image_scale, state):
+        # Each segment rebuilds the forward fields backwards from the pair
+        # after its last two steps: the final state for the last segment, else
+        # the checkpoint before the next segment (fields after n1-2 and n1-1).
+        K, N = self.K, self.n_steps
+        sources = (self.rows, self.cols, self.values, self.src_weights)
+        for j in range(self.n_seg - 1, -1, -1):
+            n0 = j * K
+            n1 = min(n0 + K, N)
+            if n1 == N:
+                pair = (self.work[1], self.work[0])
+            else:
+                c = n1 // K - 1
+                pair = (self.ckpt[2 * c + 1], self.ckpt[2 * c])
+            adjoint_run(self.solver, self.sampler, self.n_shots, residual_nsr, n1, n0,
+                        state=state, work=self.lam, image_scale=image_scale, grad=grad,
+                        workspace=self.workspace, recon=pair, recon_work=self.rwork,
+                        bnd=self.ring, sources=sources)
         return grad
 
     def _backward_lap(self, residual_nsr, grad, image_scale, state):
@@ -315,7 +385,8 @@ class TorchLeastSquaresFWI:
     def __init__(self, solver, sampler, array, pulse, n_steps, n_shots, observed, mask,
                  alpha=0.0, tv=0.0, tv_eps=1.0e-9, c_min=1400.0, c_max=2000.0,
                  c_background=1500.0, shot_seed=0, parallel_shots=16, batched_solvers=None,
-                 store_every=1, wave_dtype=None, checkpoint_every=None, bilinear_source=False):
+                 store_every=1, wave_dtype=None, checkpoint_every=None, bilinear_source=False,
+                 reconstruct=None):
         self.solver = solver
         # transmitters inject on their bilinear footprints (reciprocal data);
         # the reference fires from the nearest node
@@ -339,14 +410,22 @@ class TorchLeastSquaresFWI:
         if self.store_every < 1:
             raise ValueError("store_every must be positive.")
         self.wave_dtype = _F32 if wave_dtype is None else wave_dtype
-        self.checkpoint_every = (int(checkpoint_every) if checkpoint_every
-                                 else default_checkpoint_every(self.n_steps))
+        # the exact gradient rebuilds the forward fields in the adjoint where the
+        # grid allows (None: automatic), else recomputes them from checkpoints
+        if reconstruct is None:
+            reconstruct = self._exact() and reconstruction_available(self.model.shape)
+        self.reconstruct = bool(reconstruct) and self._exact()
+        default = RECON_EVERY if self.reconstruct else default_checkpoint_every(self.n_steps)
+        self.checkpoint_every = int(checkpoint_every) if checkpoint_every else default
         if parallel_shots == "auto":
-            parallel_shots = auto_batch_size(
-                dev, self.n_shots,
-                gradient_bytes_per_shot(self.n_steps, self.model.shape, sampler.n_elements,
-                                        self.store_every, self.wave_dtype, _F32,
-                                        self.checkpoint_every))
+            if self.reconstruct:
+                per_shot = recon_bytes_per_shot(self.n_steps, self.model.shape,
+                                                sampler.n_elements, self.checkpoint_every)
+            else:
+                per_shot = gradient_bytes_per_shot(self.n_steps, self.model.shape,
+                                                   sampler.n_elements, self.store_every,
+                                                   self.wave_dtype, _F32, self.checkpoint_every)
+            parallel_shots = auto_batch_size(dev, self.n_shots, per_shot)
         self.parallel_shots = int(parallel_shots)
         if self.parallel_shots < 1:
             raise ValueError("parallel_shots must be positive.")
@@ -408,7 +487,7 @@ class TorchLeastSquaresFWI:
         engine = self._engines.get(batch_size)
         if engine is None:
             engine = CheckpointedGradient(self.solver, self.sampler, batch_size, self.n_steps,
-                                          self.checkpoint_every)
+                                          self.checkpoint_every, reconstruct=self.reconstruct)
             self._engines[batch_size] = engine
         return engine
 

@@ -560,7 +560,7 @@ def _weights_tensor(weights, n_shots, device):
 def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=None,
                 samples=None, sampler=None, hist=None, hist_first=0, hist_every=1, ckpt=None,
                 ckpt_every=0, step_begin=0, work=None, final=True, hist_lap=False,
-                src_weights=None):
+                src_weights=None, bnd=None, bnd_first=0):
     """Run forward steps in one native call; returns the final ``(u_old, u)``.
 
     ``init`` is the state before ``step_begin`` (zeros if None); ``samples``
@@ -572,6 +572,9 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
     ``init`` and every output but the last) for ``adjoint_run(hist_lap=True)``.
     ``src_weights`` (S, 4) float32 device tensor: bilinear sources, the rows
     and columns then being the footprint corners (``RingArray.bilinear_sources``).
+    ``bnd`` (slots, S, swfd_boundary_floats) float32: boundary ring, the
+    boundary cells of the field after step j going to slot ``j - bnd_first``
+    (full-width runs with more than two steps per launch need six work fields).
     """
     model = solver.model
     dev = model.device
@@ -599,7 +602,9 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
         ckpt_slots=0 if ckpt is None else ckpt.shape[0], hist_lap=1 if hist_lap else 0,
         work4=_native.ptr(work[4]) if len(work) > 5 else None,
         work5=_native.ptr(work[5]) if len(work) > 5 else None,
-        src_weights=_native.ptr(_check_src_weights(src_weights, n_shots)))
+        src_weights=_native.ptr(_check_src_weights(src_weights, n_shots)),
+        bnd=_native.ptr(_check_ring(bnd, n_shots, model.shape)), bnd_first=bnd_first,
+        bnd_slots=0 if bnd is None else bnd.shape[0])
     rec = None
     if samples is not None:
         if sampler is None:
@@ -612,6 +617,21 @@ def forward_run(solver, n_shots, rows_i32, cols_i32, values, n_steps, *, init=No
     return (work[0], work[1]) if final else None
 
 
+def boundary_floats(shape):
+    """Floats per shot in a slot of the boundary ring of a (ny, nx) grid."""
+    return int(_native.lib().swfd_boundary_floats(int(shape[0]), int(shape[1])))
+
+
+def _check_ring(bnd, n_shots, shape):
+    if bnd is None:
+        return None
+    if (bnd.dtype != _F32 or bnd.dim() != 3 or bnd.shape[1] != n_shots
+            or bnd.shape[2] != boundary_floats(shape) or not bnd.is_contiguous()):
+        raise ValueError("boundary ring must be a contiguous float32 (slots, shots, "
+                         "boundary_floats) tensor.")
+    return bnd
+
+
 def adjoint_workspace(solver, n_shots):
     ny, nx = solver.model.shape
     n = int(_native.lib().swfd_adjoint_workspace_floats(ny, nx, n_shots))
@@ -620,7 +640,8 @@ def adjoint_workspace(solver, n_shots):
 
 def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, state, work,
                 hist=None, hist_first=0, hist_every=1, image_scale=0.0, grad=None,
-                workspace=None, hist_lap=False):
+                workspace=None, hist_lap=False, recon=None, recon_work=None, bnd=None,
+                bnd_first=0, sources=None):
     """Adjoint steps ``step_hi-1 .. step_lo`` in one native call.
 
     ``state`` = (lam_old, lam) before the run; the state after the run is
@@ -628,11 +649,30 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
     scratch fields (may include the state tensors); with four the run takes
     several steps per launch. ``hist_lap``: the history holds Laplacians
     (``forward_run(hist_lap=True)``).
+
+    Instead of ``hist``, ``recon`` = (field after step step_hi-1, field after
+    step step_hi-2) of the forward run (read only) makes the launches rebuild
+    the forward fields backwards in time and image against them; it needs
+    ``recon_work`` (four fields), the forward run's boundary ring ``bnd`` and
+    its ``sources`` = (rows_i32, cols_i32, values, src_weights or None).
     """
     dev = solver.model.device
     solver._sync_medium()
-    if hist is not None and workspace is None:
+    if (hist is not None or recon is not None) and workspace is None:
         workspace = adjoint_workspace(solver, n_shots)
+    rk = {}
+    if recon is not None:
+        if recon_work is None or len(recon_work) != 4 or bnd is None or sources is None:
+            raise ValueError("reconstruction needs four recon_work fields, the ring and the "
+                             "sources.")
+        rows, cols, values, src_w = sources
+        rk = dict(recon_next=_native.ptr(recon[0]), recon_cur=_native.ptr(recon[1]),
+                  recon_work0=_native.ptr(recon_work[0]), recon_work1=_native.ptr(recon_work[1]),
+                  recon_work2=_native.ptr(recon_work[2]), recon_work3=_native.ptr(recon_work[3]),
+                  bnd=_native.ptr(_check_ring(bnd, n_shots, solver.model.shape)),
+                  bnd_first=bnd_first, bnd_slots=bnd.shape[0], src_rows=_native.ptr(rows),
+                  src_cols=_native.ptr(cols), src_values=_native.ptr(values),
+                  src_weights=_native.ptr(_check_src_weights(src_w, n_shots)))
     args = _native.AdjointArgs(
         n_shots=n_shots, step_hi=step_hi, step_lo=step_lo,
         init_old=_native.ptr(state[0]), init_u=_native.ptr(state[1]),
@@ -642,7 +682,7 @@ def adjoint_run(solver, sampler, n_shots, residual_nsr, step_hi, step_lo, *, sta
         residual=_native.ptr(residual_nsr), hist=_native.ptr(hist), hist_first=hist_first,
         hist_every=hist_every, hist_slots=0 if hist is None else hist.shape[0],
         image_scale=float(image_scale), grad=_native.ptr(grad),
-        workspace=_native.ptr(workspace), hist_lap=1 if hist_lap else 0)
+        workspace=_native.ptr(workspace), hist_lap=1 if hist_lap else 0, **rk)
     _native.check(_native.lib().swfd_adjoint(solver._grid, solver._medium, sampler._rec, args,
                                              _native.stream_of(dev)))
 

@@ -19,7 +19,9 @@ Both use a 300 kHz Ricker and the truth blurred by 10 mm as the start.
   memory).
 * The 1024^2 gradient cell by cell against the oracle: two shots for 1500
   steps (25 GB of oracle history), traces, misfit and gradient inside the
-  ring.
+  ring, with recomputed forward fields and with fields rebuilt backwards in
+  the adjoint (TorchLeastSquaresFWI's default, so the finite-difference
+  checks above run that path).
 * 16 seeded pixels of the 512^2 gradient against central differences of the
   device misfit with +-20 / +-10 m/s steps (rel-L2 <= 1e-2). The same check
   at h = 0.1 m/s runs in float64 on the oracle (tests/test_oracle.py): in
@@ -189,36 +191,40 @@ def test_1024_gradient_cell_by_cell_against_the_oracle():
     c32 = solver.model.c.double().cpu().numpy()
     J_ref, g_ref, tr_ref = oracle.gradient(c32, h, h, w["dt"], rows, cols, w["pulse"], n_steps,
                                            r0, c0i, wts, obs64.astype(np.float32), shots)
-    engine = CheckpointedGradient(solver, sampler, len(shots), n_steps)
     pulse_t = torch.as_tensor(w["pulse"], dtype=torch.float32, device=DEV)
     values = torch.zeros(n_steps, device=DEV)
     values[:pulse_t.numel()] = pulse_t
-    samples = engine.forward(torch.as_tensor(rows.astype(np.int32)).to(DEV),
-                             torch.as_tensor(cols.astype(np.int32)).to(DEV), values)
     observed = torch.zeros((n_el, n_el, n_steps), device=DEV)
     observed[torch.as_tensor(shots)] = torch.as_tensor(obs64, dtype=torch.float32).to(DEV)
     idx = torch.as_tensor(shots.astype(np.int32)).to(DEV)
-    residual, misfit = residual_and_misfit(samples, observed, idx, idx)
-    grad = torch.zeros((n, n), device=DEV)
-    engine.backward(residual, grad, solver.dt2)
-    torch.cuda.synchronize()
-    J = float(misfit.item())
     mask = w["mask"]
-    g_err = rel(grad.cpu().numpy()[mask], g_ref[mask])
-    j_err = abs(J - J_ref) / abs(J_ref)
-    # J = |r|^2 / 2, r = traces - observed (observed shared): a trace error dr moves
-    # J by at most 2 |dr| / |r| relative, and the blurred start leaves r small
-    predicted = samples.permute(1, 2, 0).double().cpu().numpy()
-    t_err = rel(predicted, tr_ref)
-    r_ref = tr_ref - obs64.astype(np.float32)
-    for k, tx in enumerate(shots):
-        r_ref[k, tx] = 0.0
-    bound = 2.0 * t_err * np.linalg.norm(tr_ref) / np.linalg.norm(r_ref)
-    print(f"1024^2 gradient vs float64 oracle: traces {t_err:.2e}, J {j_err:.2e} (bound from "
-          f"the traces {bound:.2e}), gradient {g_err:.2e} (checkpoints every {engine.K} steps)")
-    assert t_err <= 1e-5
-    assert j_err <= max(1e-5, 1.1 * bound)
-    assert g_err <= 5e-4
+    # recomputed forward fields, then fields rebuilt backwards in the adjoint
+    for recon in (False, True):
+        engine = CheckpointedGradient(solver, sampler, len(shots), n_steps, reconstruct=recon)
+        samples = engine.forward(torch.as_tensor(rows.astype(np.int32)).to(DEV),
+                                 torch.as_tensor(cols.astype(np.int32)).to(DEV), values)
+        residual, misfit = residual_and_misfit(samples, observed, idx, idx)
+        grad = torch.zeros((n, n), device=DEV)
+        engine.backward(residual, grad, solver.dt2)
+        torch.cuda.synchronize()
+        J = float(misfit.item())
+        g_err = rel(grad.cpu().numpy()[mask], g_ref[mask])
+        j_err = abs(J - J_ref) / abs(J_ref)
+        # J = |r|^2 / 2, r = traces - observed (observed shared): a trace error dr
+        # moves J by at most 2 |dr| / |r| relative, and the blurred start leaves r small
+        predicted = samples.permute(1, 2, 0).double().cpu().numpy()
+        t_err = rel(predicted, tr_ref)
+        r_ref = tr_ref - obs64.astype(np.float32)
+        for k, tx in enumerate(shots):
+            r_ref[k, tx] = 0.0
+        bound = 2.0 * t_err * np.linalg.norm(tr_ref) / np.linalg.norm(r_ref)
+        print(f"1024^2 gradient vs float64 oracle ({'rebuilt' if recon else 'recomputed'} "
+              f"fields, checkpoints every {engine.K} steps): traces {t_err:.2e}, J {j_err:.2e} "
+              f"(bound from the traces {bound:.2e}), gradient {g_err:.2e}")
+        assert t_err <= 1e-5
+        assert j_err <= max(1e-5, 1.1 * bound)
+        assert g_err <= 5e-4
+        del engine
 
 
 def test_512_gradient_on_16_seeded_pixels():
