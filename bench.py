@@ -4,27 +4,34 @@ Workload (default): 1024 x 1024 grid at 0.25 mm, 256-element ring of radius
 110 mm, all 256 shots per gradient, 6000 steps, 300 kHz Ricker, true medium
 of 12 seeded ellipses (c 1400-1650 m/s), start = truth blurred by 10 mm.
 A "step" is one call of ``TorchLeastSquaresFWI.misfit_and_grad`` (forward
-with checkpoints, residual and misfit, segment recomputation, adjoint with
-imaging, host chain rule to slowness squared). Inputs are synthetic and
+with checkpoints and the boundary ring, residual and misfit, adjoint with
+imaging that rebuilds the forward field backwards in time -- or, with
+SWFD_RECON=0, segment recomputation and an adjoint over the recomputed
+fields --, host chain rule to slowness squared). Inputs are synthetic and
 resident in HBM before the timed region.
 
 Printed JSON (one line): Gcell/s = 2 * ny * nx * steps * shots / seconds per
 gradient, plus
   roofline      -- the fused adjoint kernel (L steps per launch, L from the
                    library): `achieved` = the launch's algorithmic bytes (its
-                   inputs read once and outputs written once: lam_old, lam and
-                   the L forward fields of every shot, c^2, the residual rows
-                   and the gradient; lam of the last two levels written) over
-                   its average launch time, timed with HIP events over at
-                   least 50 launches on the stream they run on; `traffic` =
-                   the HBM bytes per launch that rocprofv3's FETCH_SIZE /
-                   WRITE_SIZE counters measured for the same launch shape
-                   (profiles/traffic_adjoint_<grid>.json, keyed on kernel,
-                   grid, steps, shots, segment rows and shot groups; null
-                   with the reason when the shape differs), `traffic_frac`
-                   the rate those bytes imply; `algorithmic_equiv` restates
-                   the launch at 20 B per cell-update (a single step's
-                   streams), a figure that multi-step launches exceed;
+                   inputs read once and outputs written once) over its average
+                   launch time, timed with HIP events over at least 50
+                   launches on the stream they run on. Reconstructing
+                   (adjr_kernel): per shot the adjoint pair and the forward
+                   pair read and written, the L steps' boundary cells; c^2,
+                   the residual rows and the gradient. Recomputing
+                   (adjm_kernel): lam_old, lam and the L forward fields of
+                   every shot read, lam of the last two levels written, c^2,
+                   the residual rows and the gradient. `traffic` = the HBM
+                   bytes per launch that rocprofv3's FETCH_SIZE / WRITE_SIZE
+                   counters measured for the same launch shape
+                   (profiles/traffic_adjoint[_recon]_<grid>.json, keyed on
+                   kernel, grid, steps, shots, segment rows and shot groups;
+                   null with the reason when the shape differs),
+                   `traffic_frac` the rate those bytes imply;
+                   `algorithmic_equiv` restates the launch at 20 B per
+                   cell-update (a single step's streams), a figure that
+                   multi-step launches exceed;
   cpu_baseline  -- the float64 C oracle (OpenMP, all host cores) on a
                    bounded sample of the same workload, with the CPU model;
   torch_reference(_compiled) -- the reference's op sequence (conv2d
@@ -131,14 +138,19 @@ def build_problem(a, dev):
 
 
 def time_adjoint_kernel(problem, min_launches=50):
-    """Average launch time of the fused adjoint kernel (residual + imaging).
+    """Average launch time of the gradient's adjoint kernel (residual
+    injection + imaging, and with reconstruction the backward rebuild of the
+    forward field).
 
-    A run advances ``L`` adjoint steps per launch (``swfd_steps_per_launch``);
-    runs of the segment length (the history holds that many fields) are
-    repeated until at least ``min_launches`` launches are timed, with HIP events
-    on torch's current stream, where the library launches. Each run also
-    clears and sums the shot groups' partial images once (two small kernels,
-    about 0.2 % of the timed region here).
+    A run advances ``L`` adjoint steps per launch (``swfd_steps_per_launch``).
+    Recomputing gradients: runs of the segment length (the history holds that
+    many fields) from step 0; reconstructing gradients: runs of 16 launches
+    over the last steps of the record (every level images), rebuilding from
+    the forward run's final pair and ring. Runs repeat until at least
+    ``min_launches`` launches are timed, with HIP events on torch's current
+    stream, where the library launches. Each run also clears and sums the shot
+    groups' partial images once (two small kernels, about 0.2 % of the timed
+    region here).
     """
     import soundwavesimulation_amd._native as native
     from soundwavesimulation_amd.torch_backend import adjoint_run
@@ -147,25 +159,35 @@ def time_adjoint_kernel(problem, min_launches=50):
     engine = problem._engine(S)
     ny, nx = problem.model.shape
     L = int(native.lib().swfd_steps_per_launch(1, ny, nx, 0)) if len(engine.lam) > 3 else 1
-    n_steps = max(L, (engine.K // L) * L)  # imaging reads history slots up to n_steps
-    per_run = n_steps // L
-    runs = -(-min_launches // per_run)
     dev = problem.model.device
     R = problem.sampler.n_elements
-    residual = torch.randn((n_steps, S, R), device=dev) * 1e-3
     grad = torch.zeros(problem.model.shape, device=dev)
     for t in engine.lam[:2]:
         t.zero_()
-    kw = dict(state=(engine.lam[0], engine.lam[1]), work=engine.lam, hist=engine.hist,
-              hist_first=-2, hist_every=1, image_scale=problem.solver.dt2, grad=grad,
-              workspace=engine.workspace)
-    adjoint_run(problem.solver, problem.sampler, S, residual, n_steps, 0, **kw)  # warm
+    kw = dict(state=(engine.lam[0], engine.lam[1]), work=engine.lam,
+              image_scale=problem.solver.dt2, grad=grad, workspace=engine.workspace)
+    if engine.reconstruct:
+        n_steps = 16 * L
+        hi, lo = engine.n_steps, engine.n_steps - n_steps
+        rows, cols = problem._rows_cols(np.arange(S))
+        kw.update(recon=(engine.work[1], engine.work[0]), recon_work=engine.rwork,
+                  bnd=engine.ring, sources=(rows, cols, problem.source_values,
+                                            problem._source_weights(np.arange(S))))
+    else:
+        n_steps = max(L, (engine.K // L) * L)  # imaging reads history slots up to n_steps
+        hi, lo = n_steps, 0
+        kw.update(hist=engine.hist, hist_first=-2, hist_every=1)
+    residual = torch.randn((engine.n_steps if engine.reconstruct else n_steps, S, R),
+                           device=dev) * 1e-3
+    per_run = n_steps // L
+    runs = -(-min_launches // per_run)
+    adjoint_run(problem.solver, problem.sampler, S, residual, hi, lo, **kw)  # warm
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(runs):
-        adjoint_run(problem.solver, problem.sampler, S, residual, n_steps, 0, **kw)
+        adjoint_run(problem.solver, problem.sampler, S, residual, hi, lo, **kw)
     stop.record(stream)
     torch.cuda.synchronize()
     n_launch = runs * per_run
@@ -180,12 +202,21 @@ def adjoint_roofline(problem, t_launch, S, L, n_launch):
     R = problem.sampler.n_elements
     cells = ny * nx
     shape = native.adjoint_launch_shape(ny, nx, S)
-    # algorithmic bytes of one launch: per shot 2 state fields and L forward
-    # fields read, 2 fields written; c^2 and the gradient (read + write) once;
-    # L residual rows
-    alg_bytes = 4.0 * (S * cells * (L + 4) + 3 * cells + L * S * R)
+    recon = problem._engine(S).reconstruct
+    if recon:
+        # per shot the adjoint pair and the forward pair read and written (8
+        # fields), the boundary cells of the L rebuilt fields; c^2 and the
+        # gradient (read + write) once; L residual rows
+        alg_bytes = 4.0 * (8 * S * cells + 3 * cells + L * S * R +
+                           L * S * (2 * nx + 2 * (ny - 2)))
+    else:
+        # per shot 2 state fields and L forward fields read, 2 fields written;
+        # c^2 and the gradient (read + write) once; L residual rows
+        alg_bytes = 4.0 * (S * cells * (L + 4) + 3 * cells + L * S * R)
     achieved = alg_bytes / t_launch / 1e9
-    if shape["full_width"]:
+    if recon:
+        kernel = f"adjr_kernel<{L}, every level imaged>"
+    elif shape["full_width"]:
         kernel = f"adjm_kernel<{L}, imaging>"
     elif L == 2:
         kernel = "adj2_kernel<residual+imaging>"
@@ -196,14 +227,16 @@ def adjoint_roofline(problem, t_launch, S, L, n_launch):
             "kernel": kernel, "launch_ms": round(t_launch * 1e3, 4),
             "launches_timed": n_launch, "algorithmic_bytes": alg_bytes,
             "launch_shape": {"grid": [ny, nx], "shots": S, **shape}}
-    tpath = os.path.join(HERE, "profiles", f"traffic_adjoint_{ny}x{nx}.json")
+    tpath = os.path.join(HERE, "profiles",
+                         f"traffic_adjoint{'_recon' if recon else ''}_{ny}x{nx}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
         want = {"grid": [ny, nx], "steps_per_launch": L, "shots_per_launch": S,
                 "segment_rows": shape["segment_rows"], "shot_groups": shape["shot_groups"]}
         have = {k: tr.get(k) for k in want}
-        if have == want and tr.get("kernel", "").find(f"adjm_kernel<{L},") >= 0:
+        want_kernel = f"adjr_kernel<{L}," if recon else f"adjm_kernel<{L},"
+        if have == want and tr.get("kernel", "").find(want_kernel) >= 0:
             traffic = float(tr["bytes_per_launch"])
             roof["traffic"] = traffic
             roof["traffic_gbs"] = round(traffic / t_launch / 1e9, 1)

@@ -2264,6 +2264,7 @@ __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wav
                                plane_bytes);
         }
         const size_t res_off = (size_t)s * p.n_rec;
+        const size_t res_step = (size_t)p.n_shots * p.n_rec;
         const int sr = p.src_rows[s], sc = p.src_cols[s];
         const int src_extra = transmitter_rows(p);
         // lam and U of each level's newest three rows (level 0: the inputs,
@@ -2354,18 +2355,21 @@ __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wav
             for (int j = 1; j <= KL; ++j) {
                 const int rho = y - j + 1;
                 const int m = n_top - j + 1;  // adjoint step of the level
-                const AdjEdge *hw = &ex[y & 1][w_slot][w_side];
-                const AdjEdge *he = &ex[y & 1][e_slot][e_side];
+                // neighbour (P, U) pairs of level j - 1 (one 64-bit LDS read each)
+                const float2 ew = ex[y & 1][w_slot][w_side].v[j - 1];
+                const float2 ee = ex[y & 1][e_slot][e_side].v[j - 1];
                 // adjoint step m (adjm_kernel's arithmetic)
                 const float u = lv[j - 1][1];
                 const float old = j == 1 ? a0 : lv[j - 2][2];
                 const float Ps = kh[j + 1] * lv[j - 1][2];
                 const float P0 = kh[j] * u;
                 const float Pn = kh[j - 1] * lv[j - 1][0];
-                const float pw = from_west(P0, hw->v[j - 1].x), pe = from_east(P0, he->v[j - 1].x);
+                const float pw = from_west(P0, ew.x), pe = from_east(P0, ee.x);
                 float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, p.idx2, p.idy2, p.cen),
                                     leap(u, old));
-                v = inject_lanes(p, p.lv_resid[j - 1], res_off, eh[j - 1], lane, v);
+                // the levels' residual rows lie res_step floats apart
+                v = inject_lanes(p, p.lv_resid[0] - (size_t)(j - 1) * res_step, res_off, eh[j - 1],
+                                 lane, v);
                 if constexpr (XE) {
                     if (!YE || (rho >= 1 && rho <= H - 2)) v = mur_sides(v, u, x, W, mh[j - 1]);
                 }
@@ -2387,7 +2391,7 @@ __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wav
                 // the forward field U(m - 2) from U(m - 1) (uv[j - 1]) and U(m)
                 const float U0 = uv[j - 1][1];
                 const float Uold = j == 1 ? ua : uv[j - 2][2];
-                const float Uw = from_west(U0, hw->v[j - 1].y), Ue = from_east(U0, he->v[j - 1].y);
+                const float Uw = from_west(U0, ew.y), Ue = from_east(U0, ee.y);
                 const float Lr = lap5(Uw, Ue, uv[j - 1][2], uv[j - 1][0], U0, p.idx2, p.idy2, p.cen);
                 float w = __fmaf_rn(dt2 * kh[j], Lr, leap(U0, Uold));
                 if (__builtin_expect((unsigned)(rho - sr) <= (unsigned)src_extra, 0))
