@@ -135,6 +135,10 @@ struct StepParams {
     const float *r_next, *r_cur;
     float *lv_rout[4];
     int img_from, bcol_rows;
+    // reconstructing adjoint launches: segments away from rows 0 / H - 1 take
+    // the lean marches (every level images and rebuilds a field of step >= 0,
+    // segments of at least two rows, rows of at least two wavefronts)
+    int adjr_lean;
 };
 
 // Floats per shot in a slot of the boundary ring: row 0, row H - 1, then
@@ -2156,6 +2160,29 @@ __global__ void __launch_bounds__(1024) adjm_batch_kernel(const StepParams *tabl
 // field U(n - j - 1)
 __device__ __forceinline__ int recon_field(int n, int j) { return n - j - 1; }
 
+// Boundary columns of shot s's rebuilt levels for a segment whose march
+// starts at row yb + KL - 1, staged in LDS: bcol[(j * 2 + side) * nb + row -
+// yb] for level j + 1, side 0 = column 0, 1 = column W - 1. Every thread of the
+// workgroup takes part; the barrier before the march orders the writes.
+template <int KL>
+__device__ __forceinline__ void adjr_stage_columns(const StepParams &p, int s, int yb,
+                                                   float *bcol)
+{
+    const int H = p.ny, W = p.nx, nb = p.bcol_rows;
+    const size_t bnd_n = bnd_floats(H, W);
+    for (int i = (int)threadIdx.x; i < KL * 2 * nb; i += (int)blockDim.x) {
+        const int r = i % nb, side = (i / nb) & 1, j = i / (2 * nb);
+        const int row = yb + r;
+        const int q = recon_field(p.src_index, j + 1);
+        const int k = q - p.bnd_first;
+        float v = 0.0f;
+        if ((row >= 1) & (row <= H - 2) & (q >= 0) & (k >= 0) & (k < p.bnd_slots))
+            v = p.bnd[((size_t)k * p.n_shots + s) * bnd_n + 2 * W + (size_t)side * (H - 2) +
+                      row - 1];
+        bcol[i] = v;
+    }
+}
+
 template <int KL, bool XE, bool YE, bool EVERY>
 __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wave, int grp,
                                           int seg, AdjEdgeRow *ex, float *bcol, float *brow)
@@ -2218,19 +2245,7 @@ __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wav
     float *acc = adjm_acc;
     for (int s = s_begin; s < s_end; ++s) {
         const size_t soff = (size_t)s * plane;
-        // boundary columns of this shot's levels (every thread of the
-        // workgroup; the barrier before the march orders the writes)
-        for (int i = (int)threadIdx.x; i < KL * 2 * nb; i += (int)blockDim.x) {
-            const int r = i % nb, side = (i / nb) & 1, j = i / (2 * nb);
-            const int row = yb + r;
-            const int q = recon_field(n_top, j + 1);
-            const int k = q - p.bnd_first;
-            float v = 0.0f;
-            if ((row >= 1) & (row <= H - 2) & (q >= 0) & (k >= 0) & (k < p.bnd_slots))
-                v = p.bnd[((size_t)k * p.n_shots + s) * bnd_n + 2 * W + (size_t)side * (H - 2) +
-                          row - 1];
-            bcol[i] = v;
-        }
+        adjr_stage_columns<KL>(p, s, yb, bcol);
         // rows 0 and H - 1 of the levels, for segments that reach them (staged
         // too: a conditional vector load in the march would make the compiler
         // wait for every outstanding load after it)
@@ -2434,10 +2449,273 @@ __device__ __forceinline__ void adjr_wave(const StepParams &p, int lane, int wav
             }
             __syncthreads();
         };
-        for (int y = y_begin; y <= y_end; y += kPfA) {
-#pragma unroll
-            for (int i = 0; i < kPfA; ++i) iteration(y + i, i);
+        // exactly y_end - y_begin + 1 iterations (one barrier each): every
+        // wavefront of the workgroup, whichever march it runs, passes the
+        // same number of barriers
+        static_assert(kPfA == 2, "the march below takes its prefetch slots in pairs");
+        for (int y = y_begin;; y += kPfA) {
+            iteration(y, 0);
+            if (y + 1 > y_end) break;
+            iteration(y + 1, 1);
+            if (y + 2 > y_end) break;
         }
+    }
+}
+
+// Lean march of a reconstructing adjoint launch, for a segment whose march
+// reaches neither row 1 nor row H - 1 in a launch whose every level images
+// and rebuilds a field of step >= 0: SIDE 0 = a wavefront holding neither
+// column 0 nor W - 1, SIDE 1 = the one holding column 0, SIDE 2 = the one
+// holding column W - 1 (the row spans at least two wavefronts). Per cell it
+// does adjr_wave's arithmetic in the same order (the same bits); the
+// bookkeeping around it is taken off the march:
+// * rings of registers indexed by the iteration count c, made static by
+//   unrolling six iterations (a multiple of every ring's period), so no row
+//   moves between registers: each level's rows and their P = c^2 lam (three
+//   live of six), the level-0 inputs, which the prefetch loads straight into
+//   the ring (slot c + 2 holds a row no longer needed), the scatter entries,
+//   Mur side coefficients and imaging sums of the rows in flight;
+// * P = c^2 lam is formed once, when its row is made, instead of per use;
+// * the first 2 KL - 1 iterations (they make rows below the segment, whose
+//   values are incomplete) are unrolled with their store tests resolved at
+//   compile time. After them every row a level stores or images is the
+//   segment's own or -- in the last two iterations -- row r1 or r1 + 1 of
+//   levels KL - 1 and KL, whose cells the next segment makes from the same
+//   inputs and stores with the same bits, so no test remains;
+// * one residual-injection test per iteration for the KL rows in flight;
+// * a side wavefront updates only its own Mur column (two wavefront shifts
+//   per level instead of four).
+template <int KL, int SIDE>
+__device__ __forceinline__ void adjr_lean(const StepParams &p, int lane, int wave, int grp,
+                                          int seg, AdjEdgeRow *ex, float *bcol)
+{
+    constexpr int kR = 6;             // ring length = iterations per unrolled group
+    constexpr int kPro = 2 * KL - 1;  // iterations before every stored row is owned
+    constexpr int kPf = kPfA;         // rows loaded ahead
+    static_assert(kR % kPf == 0 && kPf + KL <= kR, "prefetched rows must fit the rings");
+    const int W = p.nx, H = p.ny;
+    const int x = wave * kLanes - p.wide_off + lane;
+    const bool owned = SIDE == 0 || ((x >= 0) & (x < W));
+    const int xo = (SIDE == 0 ? x : min(max(x, 0), W - 1)) * 4;
+    const int so = owned ? x * 4 : kDropOffset;
+    // the Mur column of a side wavefront, its side coefficients (left ones
+    // for column 0, right ones for W - 1) and its staged boundary column
+    const bool side_lane = SIDE == 1 ? x == 0 : (SIDE == 2 ? x == W - 1 : false);
+    const int mo = SIDE == 2 ? H * 4 : 0;
+    const float *mur = p.mur;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_begin = r0 - (KL - 1);
+    const int y_end = r1 + KL - 1;
+    const int n_it = y_end - y_begin + 1;
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int row_bytes = W * 4;
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2, plane_bytes);
+    const float *emap = (const float *)p.cell_map;
+    const float dt2 = p.dt2, idx2 = p.idx2, idy2 = p.idy2, cen = p.cen;
+    const int w_slot = lane == 0 ? wave : wave + 1;
+    const int w_side = lane == 0 ? 1 : 0;
+    const int e_slot = lane == kLanes - 1 ? wave + 2 : wave + 1;
+    const int e_side = lane == kLanes - 1 ? 0 : 1;
+    const bool edge_lane = (lane == 0) | (lane == kLanes - 1);
+    const int my_side = lane == kLanes - 1 ? 1 : 0;
+    const int n_top = p.src_index;
+    // staged boundary column of level j at row rho: bcol[((j - 1) * 2 + side) *
+    // nb + rho - yb] with yb = y_begin - KL + 1, i.e. rho - yb = c + KL - j
+    const int nb = p.bcol_rows;
+    const float *bside = bcol + (SIDE == 2 ? nb : 0) + KL;
+    // edge rows by iteration parity: iteration c reads ex_c[c & 1] and writes
+    // ex_c[(c + 1) & 1] (the rings' length is even, so c & 1 is static)
+    AdjEdgeRow *const ex_c[2] = {ex + (y_begin & 1), ex + ((y_begin + 1) & 1)};
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    const int src_extra = transmitter_rows(p);
+    const size_t res_step = (size_t)p.n_shots * p.n_rec;
+    for (int s = s_begin; s < s_end; ++s) {
+        adjr_stage_columns<KL>(p, s, y_begin - KL + 1, bcol);
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rua = make_rsrc(p.r_next + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rub = make_rsrc(p.r_cur + soff, plane_bytes);
+        __amdgpu_buffer_rsrc_t ro[KL], rou[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) {
+            ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
+                              plane_bytes);
+            rou[j] = make_rsrc((const float *)((uintptr_t)p.lv_rout[j] + soff * sizeof(float)),
+                               plane_bytes);
+        }
+        const size_t res_off = (size_t)s * p.n_rec;
+        const int sr = p.src_rows[s], sc = p.src_cols[s];
+        // ring slot of iteration c = c mod kR. Level 0 (the inputs): lam[0],
+        // U[0] = row y + 1 at iteration c; lamo, uno = the older adjoint and
+        // the newer forward field at row y; c2 = c^2 of row y + 1; ent, mh =
+        // the scatter entry and Mur side coefficient at row y. Level j >= 1:
+        // lam[j], U[j] = the row made at iteration c (row y - j + 1); P[j] =
+        // c^2 lam of the same rows. racc: imaging sum of row y_begin + slot
+        // (mod kR). The rows in use plus those in flight fit every ring but
+        // that of c^2 (in use for KL + 1 iterations), which the prefetch
+        // fills through slots qk.
+        float lam[KL + 1][kR], P[KL][kR], U[KL][kR];
+        float c2[kR], lamo[kR], uno[kR], racc[kR], mh[kR], lsave[KL], qk[kPf];
+        int ent[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+#pragma unroll
+            for (int j = 0; j <= KL; ++j) lam[j][k] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < KL; ++j) P[j][k] = U[j][k] = 0.0f;
+            c2[k] = lamo[k] = uno[k] = racc[k] = mh[k] = 0.0f;
+            ent[k] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j < KL; ++j) lsave[j] = 0.0f;
+        // iterations -1 and -2: rows y_begin and y_begin - 1
+        lam[0][kR - 1] = bload(rb, xo, y_begin * row_bytes);
+        lam[0][kR - 2] = bload(rb, xo, (y_begin - 1) * row_bytes);
+        U[0][kR - 1] = bload(rub, xo, y_begin * row_bytes);
+        U[0][kR - 2] = bload(rub, xo, (y_begin - 1) * row_bytes);
+        c2[kR - 1] = bload(rk, xo, y_begin * row_bytes);
+        c2[kR - 2] = bload(rk, xo, (y_begin - 1) * row_bytes);
+#pragma unroll
+        for (int i = 0; i < kPf; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            const int yy = min(y_begin + i, y_end);
+            const int o0 = yy * row_bytes, o1 = o0 + row_bytes;
+            lam[0][i] = bload(rb, xo, o1);
+            U[0][i] = bload(rub, xo, o1);
+            qk[i] = bload(rk, xo, o1);
+            lamo[i] = bload(ra, xo, o0);
+            uno[i] = bload(rua, xo, o0);
+            ent[i] = __float_as_int(gload_off(emap, xo + o0));
+            if constexpr (SIDE != 0) mh[i] = gload_off(mur, mo + yy * 4);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (SIDE != 0) {
+            U[0][kR - 1] = owned ? U[0][kR - 1] : 0.0f;
+            U[0][kR - 2] = owned ? U[0][kR - 2] : 0.0f;
+        }
+        P[0][kR - 1] = c2[kR - 1] * lam[0][kR - 1];
+        P[0][kR - 2] = c2[kR - 2] * lam[0][kR - 2];
+        if (edge_lane) {
+            AdjEdge e;
+            e.v[0] = make_float2(P[0][kR - 1], U[0][kR - 1]);
+#pragma unroll
+            for (int j = 1; j < kMaxLevels; ++j) e.v[j] = make_float2(0.0f, 0.0f);
+            (*ex_c[0])[wave + 1][my_side] = e;
+        }
+        __syncthreads();
+        // iteration c with ring slot cm = c mod kR; pc = c in the first kPro
+        // iterations (row tests at compile time), -1 afterwards (no tests)
+        auto iteration = [&](int cm, int pc, int c) __attribute__((always_inline)) {
+            const int y = y_begin + c;
+            const int cn = cm, cc = (cm + kR - 1) % kR, cs = (cm + kR - 2) % kR;
+            const int q = cm % kPf;
+            const float k_new = take(qk[q]);
+            __builtin_amdgcn_sched_barrier(0);
+            {
+                const int f = (cm + kPf) % kR;
+                const int yp = min(y + kPf, y_end);
+                const int o0 = yp * row_bytes, o1 = o0 + row_bytes;
+                lam[0][f] = bload(rb, xo, o1);
+                U[0][f] = bload(rub, xo, o1);
+                qk[q] = bload(rk, xo, o1);
+                lamo[f] = bload(ra, xo, o0);
+                uno[f] = bload(rua, xo, o0);
+                ent[f] = __float_as_int(gload_off(emap, xo + o0));
+                if constexpr (SIDE != 0) mh[f] = gload_off(mur, mo + yp * 4);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            c2[cm] = k_new;
+            P[0][cm] = k_new * lam[0][cm];
+            if constexpr (SIDE != 0) U[0][cm] = owned ? U[0][cm] : 0.0f;
+            // a lane's entries of rows y .. y - KL + 1 have an AND >= 0 iff
+            // one of them is >= 0
+            int e_and = ent[cm];
+#pragma unroll
+            for (int j = 2; j <= KL; ++j) e_and &= ent[(cm + kR - (j - 1)) % kR];
+            const bool inject = __ballot(owned & (e_and >= 0)) != 0;
+            const bool near_src = (unsigned)(y - sr) <= (unsigned)(KL - 1 + src_extra);
+            const AdjEdgeRow &er = *ex_c[cm & 1];
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                const int rho = y - j + 1;
+                const int kj = (cm + kR - j) % kR;         // c^2 of row rho
+                const int ke = (cm + kR - (j - 1)) % kR;   // entry, Mur coefficient of row rho
+                const float2 ew = er[w_slot][w_side].v[j - 1];
+                const float2 ee = er[e_slot][e_side].v[j - 1];
+                // adjoint step n_top - j + 1
+                const float Pn = P[j - 1][cn], P0 = P[j - 1][cc], Ps = P[j - 1][cs];
+                const float u = lam[j - 1][cc];
+                const float old = j == 1 ? lamo[cm] : lam[j - 2][cs];
+                const float pw = from_west(P0, ew.x), pe = from_east(P0, ee.x);
+                float v = __fmaf_rn(dt2, lap5(pw, pe, Ps, Pn, P0, idx2, idy2, cen),
+                                    leap(u, old));
+                if (inject)
+                    v = inject_lanes(p, p.lv_resid[0] - (size_t)(j - 1) * res_step, res_off,
+                                     owned ? ent[ke] : -1, lane, v);
+                if constexpr (SIDE == 1) {  // column 0: from column 1's new and current values
+                    const float ve = from_east(v, 0.0f), ce = from_east(u, 0.0f);
+                    v = side_lane ? ce + mh[ke] * (ve - u) : v;
+                } else if constexpr (SIDE == 2) {  // column W - 1: from column W - 2's
+                    const float vw = from_west(v, 0.0f), cw = from_west(u, 0.0f);
+                    v = side_lane ? cw + mh[ke] * (vw - u) : v;
+                }
+                lam[j][cm] = v;
+                if (j < KL) P[j][cm] = c2[kj] * v;
+                const bool store = j >= KL - 1 && (pc < 0 || pc >= KL + j - 2);
+                if (store) bstore(ro[j - 1], v, so, rho * row_bytes);
+                // the forward field U(n_top - j - 1) from U(n_top - j) and U(n_top - j + 1)
+                const float U0 = U[j - 1][cc];
+                const float Uold = j == 1 ? uno[cm] : U[j - 2][cs];
+                const float Uw = from_west(U0, ew.y), Ue = from_east(U0, ee.y);
+                const float Lr = lap5(Uw, Ue, U[j - 1][cs], U[j - 1][cn], U0, idx2, idy2, cen);
+                float w = __fmaf_rn(dt2 * c2[kj], Lr, leap(U0, Uold));
+                if (__builtin_expect(near_src, 0) &&
+                    (unsigned)(rho - sr) <= (unsigned)src_extra)
+                    w = add_transmitter(p, s, w, rho, x, sr, sc,
+                                        ((const kcf *)p.src_values)[n_top - j + 1]);
+                if constexpr (SIDE != 0) {
+                    // the Mur column from the boundary ring; no field outside the grid
+                    const float bc = bside[(j - 1) * 2 * nb + c - j];
+                    w = side_lane ? bc : w;
+                    w = owned ? w : 0.0f;
+                }
+                if (j < KL) U[j][cm] = w;
+                if (store) bstore(rou[j - 1], w, so, rho * row_bytes);
+                // imaging of row rho - 1: lam' of the step against Lap(U) of the next
+                racc[kj] += lam[j][cc] * lsave[j - 1];
+                lsave[j - 1] = Lr;
+            }
+            // row y - KL has the terms of every level (rows below the
+            // segment, in the first kPro iterations, are dropped)
+            const int kf = (cm + kR - KL) % kR;
+            if (pc < 0 && owned) adjm_acc[(c - kPro) * W + x] += racc[kf];
+            racc[kf] = 0.0f;
+            if (edge_lane) {
+                AdjEdge e;
+#pragma unroll
+                for (int j = 0; j < kMaxLevels; ++j)
+                    e.v[j] = j < KL ? make_float2(P[j][cm], U[j][cm]) : make_float2(0.0f, 0.0f);
+                (*ex_c[(cm + 1) & 1])[wave + 1][my_side] = e;
+            }
+            __syncthreads();
+        };
+#pragma unroll
+        for (int c = 0; c < kPro; ++c) iteration(c % kR, c, c);
+        // the rest in groups of kR from ring slot kPro mod kR; exactly n_it
+        // iterations in all (one barrier each, as in adjr_wave)
+        int c = kPro;
+        for (;;) {
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                iteration((kPro + k) % kR, -1, c);
+                if (++c == n_it) goto march_done;
+            }
+        }
+    march_done:;
     }
 }
 
@@ -2467,7 +2745,16 @@ __device__ __forceinline__ void adjr_block(const StepParams &p, int b, AdjEdgeRo
     // (the first barrier of the march orders these writes before any use)
     const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
     const bool ye = (r0 < 2 * KL) || (r1 + KL >= p.ny);
-    if (xe) {
+    // lean marches (segments away from rows 0 / H - 1 of launches whose
+    // every level images and rebuilds a field; rows of two or more
+    // wavefronts) and the general one, with the same bits
+    bool lean = false;
+    if constexpr (EVERY) lean = !ye && p.adjr_lean;
+    if (lean) {
+        if (wave == 0) adjr_lean<KL, 1>(p, lane, wave, grp, seg, ex, bcol);
+        else if (wave == p.wide_waves - 1) adjr_lean<KL, 2>(p, lane, wave, grp, seg, ex, bcol);
+        else adjr_lean<KL, 0>(p, lane, wave, grp, seg, ex, bcol);
+    } else if (xe) {
         if (ye) adjr_wave<KL, true, true, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
         else adjr_wave<KL, true, false, EVERY>(p, lane, wave, grp, seg, ex, bcol, brow);
     } else {
@@ -3587,6 +3874,14 @@ int adjr_launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StepPar
     return SWFD_OK;
 }
 
+// Whether reconstructing launches may use the lean marches
+// (SWFD_ADJR_LEAN=0 turns them off; read at every launch).
+bool adjr_lean_enabled()
+{
+    const char *e = std::getenv("SWFD_ADJR_LEAN");
+    return e == nullptr || std::strcmp(e, "0") != 0;
+}
+
 // p.levels (1..4) adjoint steps with the forward field rebuilt in one
 // full-width launch; every level whose step is at least p.img_from images.
 int launch_adj_recon(StepParams &p, hipStream_t st)
@@ -3597,6 +3892,12 @@ int launch_adj_recon(StepParams &p, hipStream_t st)
     if (blocks < 0) return -blocks;
     const dim3 grid((unsigned)blocks), block((unsigned)threads);
     const bool every = p.src_index - p.levels + 1 >= p.img_from;
+    // the lean marches: every level images and rebuilds a field of step >= 0,
+    // segments hold at least two rows and rows at least two wavefronts;
+    // SWFD_ADJR_LEAN=0 sends every wavefront through the general march (the
+    // same bits)
+    p.adjr_lean = (every && p.src_index - p.levels - 1 >= 0 && p.ny / p.n_seg >= 2 &&
+                   p.wide_waves >= 2 && adjr_lean_enabled()) ? 1 : 0;
     switch (p.levels * 2 + (every ? 1 : 0)) {
         case 2: return adjr_launch<1, false>(grid, block, lds, st, p);
         case 3: return adjr_launch<1, true>(grid, block, lds, st, p);

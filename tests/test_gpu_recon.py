@@ -161,6 +161,59 @@ def test_reconstructed_gradient_matches_the_recomputing_gradient(ny, nx, n_steps
     assert e_all <= 2e-5
 
 
+@pytest.mark.parametrize("ny,nx,n_steps,bilinear", [
+    (320, 384, 700, False),  # six wavefronts per row (two side ones); ten segments
+    (400, 257, 611, True),   # nx = 64 k + 1 (rows start one column left); bilinear sources
+])
+def test_lean_march_equals_the_general_march(ny, nx, n_steps, bilinear, monkeypatch):
+    """Segments away from rows 0 and ny - 1 take the lean marches (static
+    register rings, no row tests after the pipeline fills, the next segment's
+    first rows stored with the same bits; the wavefronts holding columns 0 and
+    nx - 1 with their one-sided Mur update); SWFD_ADJR_LEAN=0 sends them
+    through the general march. Misfit, final adjoint states and gradient must
+    be the same bits, with transmitters (point and bilinear) inside such
+    segments and receiver footprints crossing interior and side wavefronts."""
+    print(f"lean vs general march {ny}x{nx}, {n_steps} steps, bilinear {bilinear}",
+          flush=True)
+    pb = ring_problem(ny, nx, 16, 7)
+    shots = np.arange(16)
+    arr = pb["array"]
+    solver = TorchFDTD2D(TorchScalarWave2D(np.full(pb["c_true"].shape, 1500.0), pb["h"],
+                                           device=DEV), pb["dt"])
+    truth = TorchFDTD2D(TorchScalarWave2D(pb["c_true"], pb["h"], device=DEV), pb["dt"])
+    sampler = TorchRingSampler(arr, device=DEV)
+    if bilinear:
+        rows, cols, w = arr.bilinear_sources(shots)
+        src_w = torch.as_tensor(w.astype(np.float32)).to(DEV)
+    else:
+        rows, cols, w = arr.rows[shots], arr.cols[shots], None
+        src_w = None
+    pulse_t = torch.as_tensor(pb["pulse"], dtype=torch.float32)
+    observed, _ = simulate_shots_batch(truth, sampler, pulse_t, rows, cols, n_steps,
+                                       source_weights=w)
+    values = torch.zeros(n_steps, device=DEV)
+    values[:min(n_steps, pulse_t.numel())] = pulse_t[:n_steps].to(DEV)
+    r = torch.as_tensor(np.asarray(rows, np.int32)).to(DEV)
+    c = torch.as_tensor(np.asarray(cols, np.int32)).to(DEV)
+    idx = torch.as_tensor(shots.astype(np.int32)).to(DEV)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SWFD_ADJR_LEAN", flag)
+        eng = CheckpointedGradient(solver, sampler, len(shots), n_steps, reconstruct=True)
+        samples = eng.forward(r, c, values, src_w)
+        residual, misfit = residual_and_misfit(samples, observed, idx, idx)
+        g = torch.zeros(pb["c_true"].shape, device=DEV)
+        eng.backward(residual, g, solver.dt2)
+        torch.cuda.synchronize()
+        out[flag] = (float(misfit.item()), g.clone(), [t.clone() for t in eng.lam[:2]])
+        del eng
+    (J0, g0, lam0), (J1, g1, lam1) = out["0"], out["1"]
+    assert J0 == J1
+    assert torch.equal(lam0[0], lam1[0]) and torch.equal(lam0[1], lam1[1])
+    assert torch.equal(g0, g1)
+    assert float(g1.abs().max()) > 0.0
+
+
 def test_reference_problem_gradient_with_reconstruction():
     g64 = np.load(os.path.join(GOLDEN, "ring41_fp64.npz"))
     sp, dt = float(g64["spacing"]), float(g64["dt"])
