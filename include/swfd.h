@@ -147,8 +147,8 @@ typedef struct swfd_forward_args {
      * cells of the field after step j, for j in [bnd_first, bnd_first +
      * bnd_slots), go to slot j - bnd_first, each slot (S, swfd_boundary_floats)
      * floats per shot: row 0, row ny-1, column 0 (rows 1..ny-2), column nx-1
-     * (rows 1..ny-2). Lets swfd_adjoint reconstruct the forward fields. Not in
-     * batched runs. */
+     * (rows 1..ny-2). Lets swfd_adjoint reconstruct the forward fields. In
+     * batched runs one batched copy per tick takes the ring of every run. */
     float *bnd;
     int32_t bnd_first, bnd_slots;
 } swfd_forward_args;
@@ -185,7 +185,9 @@ typedef struct swfd_adjoint_args {
      * step_hi-1 / step_hi-2 (read only); step n >= 1 images against the
      * reconstructed field after step n-1. recon_work0..3: four fields for the
      * reconstructed pairs; src_*: the forward run's sources. Needs full-width
-     * launches (see swfd_steps_per_launch) and work3; not in batched runs. */
+     * launches (see swfd_steps_per_launch) and work3; in batched runs the
+     * reconstructing launches of a tick go out as one batched launch per step
+     * count and row width. */
     const float *recon_next, *recon_cur;
     float *recon_work0, *recon_work1, *recon_work2, *recon_work3;
     const float *bnd;

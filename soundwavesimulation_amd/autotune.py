@@ -46,8 +46,8 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "swfd_tune.json"))
 
 
-def _key(device, ny, nx, n_shots):
-    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}:v2"
+def _key(device, ny, nx, n_shots, adjoint=True):
+    return f"{torch.cuda.get_device_name(device)}:{ny}x{nx}:{n_shots}:v2" + ("" if adjoint else ":fwd")
 
 
 def apply(cfg, ny, nx):
@@ -55,12 +55,14 @@ def apply(cfg, ny, nx):
     lib = _native.lib()
     f, a = cfg["forward"], cfg["adjoint"]
     _native.check(lib.swfd_tune(0, ny, nx, f["seg"], f.get("wpc", 0), 0))
-    _native.check(lib.swfd_tune(1, ny, nx, a["seg"], a.get("groups", 0), 0))
+    if a is not None:
+        _native.check(lib.swfd_tune(1, ny, nx, a["seg"], a.get("groups", 0), 0))
     if cfg["depth1"]:
         _native.check(lib.swfd_tune(2, ny, nx, 0, 0, 1))
         return
     _native.check(lib.swfd_tune(2, ny, nx, 0, f["wide"], f["levels"]))
-    _native.check(lib.swfd_tune(3, ny, nx, 0, a["wide"], a["levels"]))
+    if a is not None:
+        _native.check(lib.swfd_tune(3, ny, nx, 0, a["wide"], a["levels"]))
 
 
 def _time(fn, reps=2, label=None, log=None):
@@ -79,11 +81,14 @@ def _time(fn, reps=2, label=None, log=None):
 
 
 def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, log=None,
-         verbose=None):
+         verbose=None, adjoint=True):
     """Time candidates for (ny, nx, n_shots); install and return the fastest
-    (``verbose``: a log function for every candidate's time)."""
+    (``verbose``: a log function for every candidate's time). ``adjoint=False``
+    tunes the forward launches only and leaves the adjoint's shape to the
+    library's model (the shapes here are timed on the adjoint over a stored
+    field history, not on the adjoint that rebuilds the forward fields)."""
     path = _cache_path()
-    key = _key(device, ny, nx, n_shots)
+    key = _key(device, ny, nx, n_shots, adjoint)
     cache = {}
     if use_cache and os.path.exists(path):
         with open(path) as f:
@@ -150,7 +155,7 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
     adj_best = None
     lib.swfd_tune(2, ny, nx, 0, 0, 2)
     lib.swfd_tune(3, ny, nx, 0, 0, 2)
-    for seg in ADJOINT_SEG_ROWS:
+    for seg in (ADJOINT_SEG_ROWS if adjoint else ()):
         if seg > ny:
             continue
         for groups in ADJOINT_GROUPS:
@@ -158,7 +163,7 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
             t = _time(adj, label=f"adjoint blocks seg={seg} groups={groups}", log=verbose)
             if adj_best is None or t < adj_best[0]:
                 adj_best = (t, {"wide": 0, "levels": 2, "seg": seg, "groups": groups})
-    if wide:
+    if wide and adjoint:
         for levels in WIDE_LEVELS:
             lib.swfd_tune(3, ny, nx, 0, 1, levels)
             for seg in WIDE_ADJOINT_SEG_ROWS:
@@ -171,6 +176,12 @@ def tune(ny, nx, n_shots, device, n_steps=16, spacing=0.5e-3, use_cache=True, lo
     if log is not None:
         log(f"  {ny}x{nx}: single {t1 / n_steps * 1e3:.4f}, forward {fwd_best}, adjoint {adj_best}"
             " (ms per step for the whole run)")
+    if adj_best is None:  # forward only: the adjoint keeps the library's shape
+        for kind in (1, 3):
+            lib.swfd_tune(kind, ny, nx, 0, 0, 0)
+        lib.swfd_tune(2, ny, nx, 0, 0, 1)
+        t1 = _time(fwd)
+        adj_best = (0.0, None)
     t2 = fwd_best[0] + adj_best[0]
     cfg = {"depth1": t1 < t2, "forward": fwd_best[1], "adjoint": adj_best[1],
            "ms_per_step_single": round(t1 / n_steps * 1e3, 4),

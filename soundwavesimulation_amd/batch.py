@@ -29,8 +29,9 @@ import numpy as np
 import torch
 
 from . import _native
-from .inversion.fwi import default_checkpoint_every, segment_plan
-from .torch_backend import _source_values_tensor
+from .inversion.fwi import (RECON_EVERY, default_checkpoint_every, reconstruction_available,
+                            segment_plan)
+from .torch_backend import _source_values_tensor, adjoint_run, boundary_floats
 
 _F32 = torch.float32
 
@@ -41,7 +42,7 @@ def _ptr(t):
 
 def _forward_args(solver, n_shots, rows, cols, values, n_steps, *, init, work, samples=None,
                   hist=None, hist_first=0, hist_every=1, ckpt=None, ckpt_every=0, step_begin=0,
-                  final=None):
+                  final=None, bnd=None):
     return _native.ForwardArgs(
         n_shots=n_shots, step_begin=step_begin, n_steps=n_steps,
         init_old=_ptr(init[0]), init_u=_ptr(init[1]),
@@ -54,11 +55,21 @@ def _forward_args(solver, n_shots, rows, cols, values, n_steps, *, init, work, s
         src_rows=_ptr(rows), src_cols=_ptr(cols), src_values=_ptr(values),
         samples=_ptr(samples), hist=_ptr(hist), hist_first=hist_first, hist_every=hist_every,
         hist_slots=0 if hist is None else hist.shape[0], ckpt=_ptr(ckpt), ckpt_every=ckpt_every,
-        ckpt_slots=0 if ckpt is None else ckpt.shape[0])
+        ckpt_slots=0 if ckpt is None else ckpt.shape[0],
+        bnd=_ptr(bnd), bnd_first=0, bnd_slots=0 if bnd is None else bnd.shape[0])
 
 
 def _adjoint_args(n_shots, residual, step_hi, step_lo, *, state, work, hist=None, hist_first=0,
-                  hist_every=1, image_scale=0.0, grad=None, workspace=None):
+                  hist_every=1, image_scale=0.0, grad=None, workspace=None, recon=None,
+                  recon_work=None, bnd=None, sources=None):
+    rk = {}
+    if recon is not None:  # the forward fields rebuilt in the launches (see adjoint_run)
+        rows, cols, values = sources
+        rk = dict(recon_next=_ptr(recon[0]), recon_cur=_ptr(recon[1]),
+                  recon_work0=_ptr(recon_work[0]), recon_work1=_ptr(recon_work[1]),
+                  recon_work2=_ptr(recon_work[2]), recon_work3=_ptr(recon_work[3]),
+                  bnd=_ptr(bnd), bnd_first=0, bnd_slots=bnd.shape[0], src_rows=_ptr(rows),
+                  src_cols=_ptr(cols), src_values=_ptr(values))
     return _native.AdjointArgs(
         n_shots=n_shots, step_hi=step_hi, step_lo=step_lo,
         init_old=_ptr(state[0]), init_u=_ptr(state[1]),
@@ -67,7 +78,7 @@ def _adjoint_args(n_shots, residual, step_hi, step_lo, *, state, work, hist=None
         final_old=_ptr(state[0]), final_u=_ptr(state[1]), residual=_ptr(residual),
         hist=_ptr(hist), hist_first=hist_first, hist_every=hist_every,
         hist_slots=0 if hist is None else hist.shape[0], image_scale=float(image_scale),
-        grad=_ptr(grad), workspace=_ptr(workspace))
+        grad=_ptr(grad), workspace=_ptr(workspace), **rk)
 
 
 class _Prepared:
@@ -146,10 +157,16 @@ class GradientProblem:
     elements (sorted, as the reference's shot sampling keeps them,
     ``invert_ring.py:579-590``); ``observed`` holds one gather per element
     ``(n_elements, R, N)``, and each shot's own receiver channel is muted.
+    ``reconstruct`` (default: wherever rows fit a full-width launch and
+    SWFD_RECON is not 0): the adjoint rebuilds the forward fields backwards
+    in time from checkpoints every ``RECON_EVERY`` steps and a ring of the
+    boundary cells of every step, as ``CheckpointedGradient(...,
+    reconstruct=True)`` does; otherwise segments are recomputed into a field
+    history.
     """
 
     def __init__(self, solver, sampler, array, pulse, n_steps, observed, checkpoint_every=None,
-                 sources=None):
+                 sources=None, reconstruct=None):
         self.solver, self.sampler, self.array = solver, sampler, array
         dev = solver.model.device
         self.sources = (np.arange(array.n_elements) if sources is None
@@ -157,19 +174,36 @@ class GradientProblem:
         if self.sources.ndim != 1 or self.sources.size < 1 or \
                 self.sources.min() < 0 or self.sources.max() >= array.n_elements:
             raise ValueError("sources must name ring elements.")
+        if reconstruct is None:
+            reconstruct = reconstruction_available(solver.model.shape)
+        elif reconstruct and not reconstruction_available(solver.model.shape):
+            raise ValueError("this grid has no reconstructing adjoint (full-width launches).")
+        self.reconstruct = bool(reconstruct)
         self.S = int(self.sources.size)
         self.N = int(n_steps)
-        self.K = int(checkpoint_every or default_checkpoint_every(self.N))
+        default = RECON_EVERY if self.reconstruct else default_checkpoint_every(self.N)
+        self.K = int(checkpoint_every or default)
         self.n_seg, n_ckpt, n_hist = segment_plan(self.N, self.K)
         shape = (self.S, *solver.model.shape)
         e = lambda *s: torch.empty(s, device=dev, dtype=_F32)  # noqa: E731
         self.work = [e(*shape) for _ in range(4)]
-        # The adjoint state shares the forward work fields: the forward is
-        # over before the first adjoint step, and the segment recomputation
-        # writes every output into the history (its work fields stay idle).
-        self.lam = self.work
         self.ckpt = e(max(n_ckpt, 2), *shape) if n_ckpt > 0 else None
-        self.hist = e(n_hist, *shape)
+        if self.reconstruct:
+            # forward: work0..3 and two adjoint fields for the intermediate
+            # steps (the final pair stays in work0 / work1); adjoint: four
+            # fields for its state, four for the rebuilt forward pairs
+            self.lam = [e(*shape) for _ in range(4)]
+            self.rwork = [e(*shape) for _ in range(4)]
+            self.ring = e(self.N, self.S, boundary_floats(solver.model.shape))
+            self.hist = None
+        else:
+            # The adjoint state shares the forward work fields: the forward is
+            # over before the first adjoint step, and the segment recomputation
+            # writes every output into the history (its work fields stay idle).
+            self.lam = self.work
+            self.rwork = []
+            self.ring = None
+            self.hist = e(n_hist, *shape)
         ny, nx = solver.model.shape
         self.workspace = e(int(_native.lib().swfd_adjoint_workspace_floats(ny, nx, self.S)))
         R = sampler.n_elements
@@ -185,9 +219,10 @@ class GradientProblem:
         self.values = _source_values_tensor(torch.as_tensor(pulse), self.N, dev)
         self.observed = observed.contiguous()
         self.cells = 2.0 * ny * nx * self.N * self.S
-        self.memory_bytes = sum(t.numel() * 4 for t in
-                                [*self.work, self.hist, self.workspace, self.samples,
-                                 self.residual] + ([self.ckpt] if self.ckpt is not None else []))
+        held = [*self.work, self.workspace, self.samples, self.residual]
+        held += [self.hist] if self.hist is not None else [*self.lam, *self.rwork, self.ring]
+        held += [self.ckpt] if self.ckpt is not None else []
+        self.memory_bytes = sum(t.numel() * 4 for t in held)
 
     def segment(self, j):
         n0 = j * self.K
@@ -208,26 +243,53 @@ class UnevenGradients:
     """FWI gradients of many uneven problems, batched per tick.
 
     Schedule: one batched forward over all problems (each for its own step
-    count, with samples and checkpoints); the residual of each problem; then
-    phase i = 0, 1, ... runs, for every problem with a segment left, its
-    segment ``n_seg - 1 - i`` (checkpoint copy, batched recomputation, batched
-    adjoint with imaging). Short problems finish after fewer phases.
+    count, with samples and checkpoints, and the boundary ring where the
+    adjoint rebuilds the forward fields); the residual of each problem; then
+    the adjoints:
+
+    * recomputing problems: phase i = 0, 1, ... runs, for every problem with a
+      segment left, its segment ``n_seg - 1 - i`` (a checkpoint copy, a
+      batched recomputation and a batched adjoint over the recomputed
+      fields); short problems finish after fewer phases;
+    * reconstructing problems: each problem's segments back to front, the
+      problems spread over ``streams`` HIP streams (longest first, each to
+      the stream with the least work so far) that run side by side. Their
+      launches take their parameters as kernel arguments: the batched
+      reconstructing launch, which reads them from a table, holds them in
+      registers and spills (433 vector registers to scratch against 66), and
+      ran the 24-problem workload at 174 Gcell/s.
+
+    Every problem of a batch reconstructs, or none does.
     """
 
-    def __init__(self, problems):
+    def __init__(self, problems, streams=4):
         self.problems = list(problems)
+        modes = {p.reconstruct for p in self.problems}
+        if len(modes) != 1:
+            raise ValueError("the problems of a batch must all reconstruct or all recompute.")
+        self.reconstruct = modes.pop()
         self.phases = max(p.n_seg for p in self.problems)
-        # the history is idle during the forward pass: two of its fields keep the
-        # intermediate steps of four-step launches until their samples are gathered
-        self.forward = BatchedForward([
-            (p.solver, p.sampler, dict(n_shots=p.S, rows=p.rows, cols=p.cols, values=p.values,
-                                       n_steps=p.N, init=(p.work[0], p.work[1]),
-                                       work=p.work + ([p.hist[0], p.hist[1]]
-                                                      if p.hist.shape[0] >= 2 else []),
-                                       samples=p.samples, ckpt=p.ckpt,
-                                       ckpt_every=p.K if p.ckpt is not None else 0))
-            for p in self.problems]).prepare()
+        fwd = []
+        for p in self.problems:
+            kw = dict(n_shots=p.S, rows=p.rows, cols=p.cols, values=p.values, n_steps=p.N,
+                      init=(p.work[0], p.work[1]), samples=p.samples, ckpt=p.ckpt,
+                      ckpt_every=p.K if p.ckpt is not None else 0)
+            if self.reconstruct:
+                # the adjoint's fields keep the intermediate steps of four-step
+                # launches; the final pair stays in work0 / work1
+                kw.update(work=p.work + [p.lam[0], p.lam[1]], final=(p.work[0], p.work[1]),
+                          bnd=p.ring)
+            else:
+                # the history is idle during the forward pass: two of its fields keep
+                # the intermediate steps of four-step launches until their samples are
+                # gathered
+                kw.update(work=p.work + ([p.hist[0], p.hist[1]] if p.hist.shape[0] >= 2 else []))
+            fwd.append((p.solver, p.sampler, kw))
+        self.forward = BatchedForward(fwd).prepare()
         self.recompute, self.adjoint, self.members = [], [], []
+        if self.reconstruct:
+            self._plan_streams(streams)
+            return
         for i in range(self.phases):
             mem = [p for p in self.problems if p.n_seg - 1 - i >= 0]
             fwd_runs, adj_runs = [], []
@@ -249,6 +311,34 @@ class UnevenGradients:
             self.recompute.append(BatchedForward(fwd_runs).prepare() if fwd_runs else None)
             self.adjoint.append(BatchedAdjoint(adj_runs).prepare())
 
+    def _plan_streams(self, n_streams):
+        dev = self.problems[0].solver.model.device
+        n = max(1, min(int(n_streams), len(self.problems)))
+        self.streams = [torch.cuda.Stream(dev) for _ in range(n)]
+        load = [0.0] * n
+        self.lanes = [[] for _ in range(n)]
+        for k in sorted(range(len(self.problems)), key=lambda k: -self.problems[k].cells):
+            s = load.index(min(load))
+            self.lanes[s].append(k)
+            load[s] += self.problems[k].cells
+
+    @staticmethod
+    def _reconstructing_adjoint(p):
+        """Segments back to front; each rebuilds the forward fields from the
+        pair after its last two steps: the final state, else the next
+        segment's checkpoint."""
+        for j in range(p.n_seg - 1, -1, -1):
+            n0, n1 = p.segment(j)
+            if n1 == p.N:
+                pair = (p.work[1], p.work[0])
+            else:
+                c = n1 // p.K - 1
+                pair = (p.ckpt[2 * c + 1], p.ckpt[2 * c])
+            adjoint_run(p.solver, p.sampler, p.S, p.residual, n1, n0, state=(p.lam[0], p.lam[1]),
+                        work=p.lam, image_scale=p.solver.dt2, grad=p.grad,
+                        workspace=p.workspace, recon=pair, recon_work=p.rwork, bnd=p.ring,
+                        sources=(p.rows, p.cols, p.values, None))
+
     def run(self, events=None):
         """Enqueue every gradient. ``events`` (dict problem index -> torch
         event) are recorded right after each problem's last launch."""
@@ -261,15 +351,28 @@ class UnevenGradients:
             p.grad.zero_()
             p.lam[0].zero_()
             p.lam[1].zero_()
+        if self.reconstruct:
+            cur = torch.cuda.current_stream(self.problems[0].solver.model.device)
+            for st, lane in zip(self.streams, self.lanes):
+                st.wait_stream(cur)
+                with torch.cuda.stream(st):
+                    for k in lane:
+                        self._reconstructing_adjoint(self.problems[k])
+                        if events is not None:
+                            events[k].record()
+            for st in self.streams:
+                cur.wait_stream(st)
+            return
         for i in range(self.phases):
-            for p in self.members[i]:
-                j = p.n_seg - 1 - i
-                if j == 0:
-                    p.hist[0].zero_()
-                    p.hist[1].zero_()
-                else:
-                    p.hist[0].copy_(p.ckpt[2 * (j - 1)])
-                    p.hist[1].copy_(p.ckpt[2 * (j - 1) + 1])
+            if not self.reconstruct:
+                for p in self.members[i]:
+                    j = p.n_seg - 1 - i
+                    if j == 0:
+                        p.hist[0].zero_()
+                        p.hist[1].zero_()
+                    else:
+                        p.hist[0].copy_(p.ckpt[2 * (j - 1)])
+                        p.hist[1].copy_(p.ckpt[2 * (j - 1) + 1])
             if self.recompute[i] is not None:
                 self.recompute[i].launch()
             self.adjoint[i].launch()
@@ -283,15 +386,23 @@ class UnevenGradients:
         return sum(p.cells for p in self.problems)
 
 
-def gradient_problem_bytes(ny, nx, n_shots, n_receivers, n_steps, checkpoint_every=None):
+def gradient_problem_bytes(ny, nx, n_shots, n_receivers, n_steps, checkpoint_every=None,
+                           reconstruct=None):
     """Device bytes a ``GradientProblem`` allocates (work fields, checkpoints,
-    segment history, shot-group images, samples, residual)."""
-    K = int(checkpoint_every or default_checkpoint_every(n_steps))
-    _, n_ckpt, n_hist = segment_plan(n_steps, K)
+    segment history or rebuilt pairs and boundary ring, shot-group images,
+    samples, residual)."""
+    if reconstruct is None:
+        reconstruct = reconstruction_available((ny, nx))
+    K = int(checkpoint_every or (RECON_EVERY if reconstruct else
+                                 default_checkpoint_every(n_steps)))
+    n_seg, n_ckpt, n_hist = segment_plan(n_steps, K)
     plane = ny * nx
-    fields = 4 + (max(n_ckpt, 2) if n_ckpt > 0 else 0) + n_hist
+    ckpt = max(n_ckpt, 2) if n_ckpt > 0 else 0
+    fields = 4 + ckpt + (8 if reconstruct else n_hist)
+    ring = n_steps * boundary_floats((ny, nx)) if reconstruct else 0
     images = max(8, min(32, n_shots))
-    return 4 * (fields * n_shots * plane + images * plane + 2 * n_steps * n_shots * n_receivers)
+    return 4 * (fields * n_shots * plane + n_shots * ring + images * plane +
+                2 * n_steps * n_shots * n_receivers)
 
 
 def plan_waves(sizes, budget_bytes, wave_cells):

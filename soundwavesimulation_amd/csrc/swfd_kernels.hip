@@ -2780,6 +2780,18 @@ __global__ void __launch_bounds__(1024) adjr_kernel(StepParams p)
     adjr_block<KL, EVERY>(p, (int)blockIdx.x, ex);
 }
 
+// Batched reconstructing adjoint launch (entries of one row width; the LDS is
+// sized for the largest entry).
+template <int KL, bool EVERY>
+__global__ void __launch_bounds__(1024) adjr_batch_kernel(const StepParams *table,
+                                                          const int *first, int n)
+{
+    __shared__ AdjEdgeRow ex[2];
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    adjr_block<KL, EVERY>(batch_params(table, e), b - first[e], ex);
+}
+
 // g[i] += sum over groups of g_part[k][i], in group order.
 __global__ void __launch_bounds__(256) reduce_groups_kernel(size_t plane, int n_groups,
                                                             const float *g_part, float *g)
@@ -2917,10 +2929,10 @@ __global__ void fill_kernel(float *p, size_t n, float v)
 
 // Boundary cells of `field` (S, H, W) into a slot of the boundary ring (S,
 // bnd_floats): after launches that store their fields in full.
-__global__ void bnd_extract_kernel(const float *field, int S, int H, int W, float *slot)
+__device__ __forceinline__ void bnd_extract_cell(const float *field, int S, int H, int W,
+                                                 float *slot, size_t i)
 {
     const size_t P = bnd_floats(H, W);
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)S * P) return;
     const size_t s = i / P;
     const int k = (int)(i % P);
@@ -2939,6 +2951,23 @@ __global__ void bnd_extract_kernel(const float *field, int S, int H, int W, floa
         col = W - 1;
     }
     slot[i] = field[s * H * W + (size_t)row * W + col];
+}
+
+__global__ void bnd_extract_kernel(const float *field, int S, int H, int W, float *slot)
+{
+    bnd_extract_cell(field, S, H, W, slot, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched ring extraction: entry e copies the boundary cells of its n_shots
+// fields at p.f_cur into the ring slot at p.bnd.
+__global__ void __launch_bounds__(256) bnd_extract_batch_kernel(const StepParams *table,
+                                                                const int *first, int n)
+{
+    const int b = (int)blockIdx.x;
+    const int e = __builtin_amdgcn_readfirstlane(batch_entry(first, n, b));
+    const StepParams &p = batch_params(table, e);
+    bnd_extract_cell(p.f_cur, p.n_shots, p.ny, p.nx, p.bnd,
+                     (size_t)(b - first[e]) * blockDim.x + threadIdx.x);
 }
 
 // Reduced-precision history storage: float32 <-> 16-bit codes, both rounded
@@ -3859,19 +3888,38 @@ int adjr_shape(StepParams &p, int &threads, size_t &lds)
     return p.n_stencil_blocks;
 }
 
-template <int KL, bool EVERY>
-int adjr_launch(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StepParams &p)
+// fn(single, batched) with the reconstructing adjoint kernels of KL steps
+// (after raising their dynamic-LDS limit once).
+template <int KL, bool EVERY, typename Fn>
+int adjr_variant_kl(Fn &&fn)
 {
     static bool big = false;
     if (!big) {
         SWFD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&adjr_kernel<KL, EVERY>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)kMaxAdjrLds));
+        SWFD_HIP_CHECK(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&adjr_batch_kernel<KL, EVERY>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxAdjrLds));
         big = true;
     }
-    hipLaunchKernelGGL((adjr_kernel<KL, EVERY>), grid, block, lds, st, p);
-    SWFD_HIP_CHECK(hipGetLastError());
+    fn(adjr_kernel<KL, EVERY>, adjr_batch_kernel<KL, EVERY>);
     return SWFD_OK;
+}
+
+template <typename Fn>
+int adjr_variant(int levels, bool every, Fn &&fn)
+{
+    switch (levels * 2 + (every ? 1 : 0)) {
+        case 2: return adjr_variant_kl<1, false>(fn);
+        case 3: return adjr_variant_kl<1, true>(fn);
+        case 4: return adjr_variant_kl<2, false>(fn);
+        case 5: return adjr_variant_kl<2, true>(fn);
+        case 6: return adjr_variant_kl<3, false>(fn);
+        case 7: return adjr_variant_kl<3, true>(fn);
+        case 8: return adjr_variant_kl<4, false>(fn);
+        default: return adjr_variant_kl<4, true>(fn);
+    }
 }
 
 // Whether reconstructing launches may use the lean marches
@@ -3880,6 +3928,19 @@ bool adjr_lean_enabled()
 {
     const char *e = std::getenv("SWFD_ADJR_LEAN");
     return e == nullptr || std::strcmp(e, "0") != 0;
+}
+
+// Whether every level of a reconstructing launch images (its steps are all at
+// least p.img_from), and the lean-march flag (also: every rebuilt field of
+// step >= 0, segments of at least two rows and rows of at least two
+// wavefronts; SWFD_ADJR_LEAN=0 sends every wavefront through the general
+// march, with the same bits). Call after adjr_shape.
+bool adjr_every(const StepParams &p) { return p.src_index - p.levels + 1 >= p.img_from; }
+
+void adjr_set_lean(StepParams &p)
+{
+    p.adjr_lean = (adjr_every(p) && p.src_index - p.levels - 1 >= 0 && p.ny / p.n_seg >= 2 &&
+                   p.wide_waves >= 2 && adjr_lean_enabled()) ? 1 : 0;
 }
 
 // p.levels (1..4) adjoint steps with the forward field rebuilt in one
@@ -3891,23 +3952,13 @@ int launch_adj_recon(StepParams &p, hipStream_t st)
     const int blocks = adjr_shape(p, threads, lds);
     if (blocks < 0) return -blocks;
     const dim3 grid((unsigned)blocks), block((unsigned)threads);
-    const bool every = p.src_index - p.levels + 1 >= p.img_from;
-    // the lean marches: every level images and rebuilds a field of step >= 0,
-    // segments hold at least two rows and rows at least two wavefronts;
-    // SWFD_ADJR_LEAN=0 sends every wavefront through the general march (the
-    // same bits)
-    p.adjr_lean = (every && p.src_index - p.levels - 1 >= 0 && p.ny / p.n_seg >= 2 &&
-                   p.wide_waves >= 2 && adjr_lean_enabled()) ? 1 : 0;
-    switch (p.levels * 2 + (every ? 1 : 0)) {
-        case 2: return adjr_launch<1, false>(grid, block, lds, st, p);
-        case 3: return adjr_launch<1, true>(grid, block, lds, st, p);
-        case 4: return adjr_launch<2, false>(grid, block, lds, st, p);
-        case 5: return adjr_launch<2, true>(grid, block, lds, st, p);
-        case 6: return adjr_launch<3, false>(grid, block, lds, st, p);
-        case 7: return adjr_launch<3, true>(grid, block, lds, st, p);
-        case 8: return adjr_launch<4, false>(grid, block, lds, st, p);
-        default: return adjr_launch<4, true>(grid, block, lds, st, p);
-    }
+    adjr_set_lean(p);
+    const int rc = adjr_variant(p.levels, adjr_every(p), [&](auto single, auto) {
+        hipLaunchKernelGGL(single, grid, block, lds, st, p);
+    });
+    if (rc) return rc;
+    SWFD_HIP_CHECK(hipGetLastError());
+    return SWFD_OK;
 }
 
 // Number of imaging partial slices a step of this shape uses.
@@ -4045,6 +4096,8 @@ bool two_step_allowed(int ny, int nx, const float *work3)
     return env && work3 != nullptr && (t == nullptr || t->depth != 1);
 }
 
+struct LaunchGroup;
+
 struct ForwardRun {
     const swfd_grid *grid = nullptr;
     const swfd_medium *medium = nullptr;
@@ -4096,9 +4149,7 @@ struct ForwardRun {
         if (lap && batched)
             return fail(SWFD_EINVAL, "Laplacian history: not in batched runs");
         if (a->bnd != nullptr) {
-            if (batched || lap)
-                return fail(SWFD_EINVAL, "boundary ring: not in batched runs or with Laplacian "
-                                         "histories");
+            if (lap) return fail(SWFD_EINVAL, "boundary ring: not with Laplacian histories");
             if (a->bnd_first > a->step_begin ||
                 (long)a->bnd_first + a->bnd_slots < (long)a->step_begin + a->n_steps)
                 return fail(SWFD_EINVAL, "boundary ring: its slots must cover the run's steps");
@@ -4218,33 +4269,47 @@ struct ForwardRun {
     // boundary ring of the steps of a launch, from its stored fields (every
     // level of a full-width launch has an output; footprint stores keep the
     // boundary cells)
-    int launch_boundary(const StepParams &p, hipStream_t st) const
+    // (field, ring slot) of each level of the next launch (call before
+    // advance); returns the count, or a negative error code
+    int boundary_copies(const StepParams &p, const float **fields, float **slots) const
     {
-        if (a->bnd == nullptr) return SWFD_OK;
+        if (a->bnd == nullptr) return 0;
         const int n = a->step_begin + k;
-        const float *outs[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
         int L = 1;
         if (next_is_wide()) {
             L = p.levels;
-            for (int j = 0; j < L; ++j) outs[j] = p.lv_out[j];
+            for (int j = 0; j < L; ++j) fields[j] = p.lv_out[j];
         } else {
-            outs[0] = p.f_out;
+            fields[0] = p.f_out;
             if (next_is_pair()) {
-                outs[1] = p.f_out_b;
+                fields[1] = p.f_out_b;
                 L = 2;
             }
         }
         const size_t total = (size_t)a->n_shots * bnd_floats(grid->ny, grid->nx);
         for (int j = 0; j < L; ++j) {
-            const int slot = n + j - a->bnd_first;
-            if (outs[j] == nullptr) return fail(SWFD_EINVAL, "boundary ring: a level has no output");
+            if (fields[j] == nullptr)
+                return -fail(SWFD_EINVAL, "boundary ring: a level has no output");
+            slots[j] = a->bnd + (size_t)(n + j - a->bnd_first) * total;
+        }
+        return L;
+    }
+    int launch_boundary(const StepParams &p, hipStream_t st) const
+    {
+        const float *fields[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
+        float *slots[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
+        const int L = boundary_copies(p, fields, slots);
+        if (L < 0) return -L;
+        const size_t total = (size_t)a->n_shots * bnd_floats(grid->ny, grid->nx);
+        for (int j = 0; j < L; ++j) {
             hipLaunchKernelGGL(bnd_extract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                               0, st, outs[j], a->n_shots, grid->ny, grid->nx,
-                               a->bnd + (size_t)slot * total);
+                               0, st, fields[j], a->n_shots, grid->ny, grid->nx, slots[j]);
             SWFD_HIP_CHECK(hipGetLastError());
         }
         return SWFD_OK;
     }
+    // the ring copies of the next launch as batched-extraction entries
+    int boundary_entries(const StepParams &p, LaunchGroup &g) const;
     // the gather of a launch's levels when they are gathered after it
     int launch_gather(const StepParams &p, hipStream_t st) const
     {
@@ -4404,7 +4469,6 @@ struct AdjointRun {
                                      "launches (see swfd_lap_history) and work3");
         recon = a->recon_next != nullptr || a->recon_cur != nullptr;
         if (recon) {
-            if (batched) return fail(SWFD_EINVAL, "reconstruction: not in batched runs");
             if (a->recon_next == nullptr || a->recon_cur == nullptr || a->recon_work0 == nullptr ||
                 a->recon_work1 == nullptr || a->recon_work2 == nullptr ||
                 a->recon_work3 == nullptr || a->bnd == nullptr || a->bnd_slots < 1 ||
@@ -4605,7 +4669,9 @@ size_t align_up(size_t v) { return (v + kTableAlign - 1) / kTableAlign * kTableA
 
 // One launch of a tick: entries of one kernel variant (and workgroup size).
 struct LaunchGroup {
-    int kind = 0;  // 0: two-step blocks, 1: full width
+    // 0: two-step blocks, 1: full width, 2: reconstructing adjoint (adjoint
+    // batches) or receiver gathers (forward batches), 3: ring copies
+    int kind = 0;
     unsigned flags = 0;
     int levels = 0, sm = 0, threads = 0;
     size_t lds = 0;
@@ -4619,6 +4685,27 @@ struct LaunchGroup {
                align_up((entries.size() + 1) * sizeof(int));
     }
 };
+
+// The ring copies of a run's next launch, appended to the tick's batched
+// extraction group (entry: f_cur = field, bnd = ring slot, one thread per
+// boundary cell of every shot).
+int ForwardRun::boundary_entries(const StepParams &p, LaunchGroup &g) const
+{
+    const float *fields[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
+    float *slots[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
+    const int L = boundary_copies(p, fields, slots);
+    if (L < 0) return -L;
+    const size_t total = (size_t)a->n_shots * bnd_floats(grid->ny, grid->nx);
+    for (int j = 0; j < L; ++j) {
+        StepParams q;
+        base_params(q, grid, nullptr, a->n_shots);
+        q.f_cur = fields[j];
+        q.bnd = slots[j];
+        g.entries.push_back(q);
+        g.first.push_back(g.first.back() + (int)((total + 255) / 256));
+    }
+    return SWFD_OK;
+}
 
 // The group of a tick for this variant (created in first-seen order, so the
 // same in every mode).
@@ -4698,6 +4785,11 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
     for (;;) {
         bool any = false;
         std::vector<LaunchGroup> groups;
+        // boundary-ring copies of this tick's batched launches: one batched
+        // extraction after them (the next tick's launches overwrite the fields)
+        LaunchGroup ring;
+        ring.kind = 3;
+        ring.threads = 256;
         for (int i = 0; i < n_runs; ++i) {
             ForwardRun &r = runs[i];
             if (r.done()) continue;
@@ -4713,6 +4805,7 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 g.entries.push_back(p);
                 g.first.push_back(g.first.back() + blocks);
                 g.runs.push_back(i);
+                if ((rc = r.boundary_entries(p, ring))) return rc;
             } else if (r.next_is_pair()) {
                 const int blocks = pair_geometry(p, gathers);
                 LaunchGroup &g = group_for(groups, 0, pair_flags(p, gathers > 0), 2, 0,
@@ -4720,9 +4813,11 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                 g.entries.push_back(p);
                 g.first.push_back(g.first.back() + blocks);
                 g.runs.push_back(i);
+                if ((rc = r.boundary_entries(p, ring))) return rc;
             } else {
                 if (mode == 2 && ((rc = launch_step(p, false, st, gathers)) ||
-                                  (rc = r.launch_gather(p, st))))
+                                  (rc = r.launch_gather(p, st)) ||
+                                  (rc = r.launch_boundary(p, st))))
                     return rc;
                 r.advance(p);
             }
@@ -4773,6 +4868,17 @@ int forward_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                             });
             if (rc) return rc;
         }
+        if (!ring.entries.empty()) {
+            rc = emit_group(ring, mode, host, off, table, table_bytes, "forward batch",
+                            [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
+                                int ne) {
+                                hipLaunchKernelGGL(bnd_extract_batch_kernel,
+                                                   dim3((unsigned)gg.first.back()), dim3(256), 0,
+                                                   st, tab, first, ne);
+                                return SWFD_OK;
+                            });
+            if (rc) return rc;
+        }
     }
     if ((rc = finish_tables(mode, host, off, table, table_bytes, bytes_out, "forward batch")))
         return rc;
@@ -4806,7 +4912,18 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
             bool fill = false;
             if ((rc = r.build(p, fill))) return rc;
             if (mode == 2 && fill && (rc = r.launch_fill(st))) return rc;
-            if (r.next_is_wide()) {
+            if (r.recon) {
+                int threads = 0;
+                size_t lds = 0;
+                const int blocks = adjr_shape(p, threads, lds);
+                if (blocks < 0) return -blocks;
+                adjr_set_lean(p);
+                LaunchGroup &g = group_for(groups, 2, adjr_every(p) ? 1u : 0u, p.levels, 0, threads);
+                g.lds = std::max(g.lds, lds);
+                g.entries.push_back(p);
+                g.first.push_back(g.first.back() + blocks);
+                g.runs.push_back(i);
+            } else if (r.next_is_wide()) {
                 int threads = 0;
                 size_t lds = 0;
                 const int blocks = adjm_shape(p, threads, lds);
@@ -4841,6 +4958,12 @@ int adjoint_batch(int n_runs, const swfd_grid *grids, const swfd_medium *media,
                             [&](const LaunchGroup &gg, const StepParams *tab, const int *first,
                                 int ne) {
                                 const dim3 grid((unsigned)gg.first.back()), block(gg.threads);
+                                if (gg.kind == 2)
+                                    return adjr_variant(gg.levels, gg.flags != 0,
+                                                        [&](auto, auto batched) {
+                                        hipLaunchKernelGGL(batched, grid, block, gg.lds, st, tab,
+                                                           first, ne);
+                                    });
                                 if (gg.kind == 0) {
                                     if (gg.image)
                                         hipLaunchKernelGGL((adj2_batch_kernel<kResid | kImage>), grid,

@@ -4,8 +4,10 @@
 grid from {256^2, 384^2, 512^2, 768^2} at 0.5 mm, a ring of 16-128 elements
 (every element fires), 1500-5000 steps, a seeded ellipse phantom each (c
 1400-1650 m/s) and a 300 kHz Ricker -- simulates their observed data, tunes
-the launch shapes of every grid size on the device (``autotune.tune``), and
-computes all 24 checkpointed gradients (from a homogeneous 1500 m/s start)
+the launch shapes of every grid size on the device (``autotune.tune``; the
+forward only when the adjoint rebuilds the forward fields), and computes all
+24 gradients (from a homogeneous 1500 m/s start; the adjoint rebuilds the
+forward fields, or with ``--recompute`` recomputes checkpointed segments)
 with ``batch.WaveSchedule``: problems in order of increasing work, grouped
 into waves whose device memory stays within the budget (``--budget-gb``, so
 any draw runs) and whose cells per step stay within ``--wave-cells``; each
@@ -34,6 +36,7 @@ from .array import RingArray
 from .autotune import tune
 from .batch import GradientProblem, WaveSchedule, gradient_problem_bytes
 from .experiment import axis_centers, stable_dt
+from .inversion.fwi import reconstruction_available
 from .phantoms import ellipse_phantom
 from .sources import ricker
 from .torch_backend import TorchFDTD2D, TorchRingSampler, TorchScalarWave2D, simulate_shots_batch
@@ -80,17 +83,18 @@ def observed_data(spec, index, seed, device, spacing=0.5e-3, f_peak=300e3):
                 spacing=spacing)
 
 
-def problem_from(spec, data, device):
+def problem_from(spec, data, device, reconstruct=None):
     """The gradient problem of ``spec`` from a homogeneous 1500 m/s start."""
     h, _, n_steps = spec
     solver = TorchFDTD2D(TorchScalarWave2D(np.full((h, h), 1500.0), data["spacing"],
                                            device=device), data["dt"])
     return GradientProblem(solver, data["sampler"], data["array"], data["pulse"], n_steps,
-                           data["observed"])
+                           data["observed"], reconstruct=reconstruct)
 
 
-def sizes_of(specs):
-    return [(work_of(sp), gradient_problem_bytes(sp[0], sp[0], sp[1], sp[1], sp[2]),
+def sizes_of(specs, reconstruct=None):
+    return [(work_of(sp), gradient_problem_bytes(sp[0], sp[0], sp[1], sp[1], sp[2],
+                                                 reconstruct=reconstruct),
              sp[1] * sp[0] * sp[0]) for sp in specs]
 
 
@@ -105,6 +109,9 @@ def main(argv=None):
     p.add_argument("--scale", type=float, default=1.0,
                    help="multiply step counts (a quick reduced run: e.g. 0.05)")
     p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--recompute", action="store_true",
+                   help="recompute checkpointed segments instead of rebuilding the forward "
+                        "fields in the adjoint")
     p.add_argument("--no-graph", action="store_true")
     p.add_argument("--out", default=None)
     a = p.parse_args(argv)
@@ -113,20 +120,21 @@ def main(argv=None):
     specs = draw(a.seed, a.problems)
     if a.scale != 1.0:
         specs = [(h, r, max(8, int(round(s * a.scale)))) for h, r, s in specs]
-    sizes = sizes_of(specs)
+    recon = not a.recompute and all(reconstruction_available((h, h)) for h, _, _ in specs)
+    sizes = sizes_of(specs, recon)
     log(f"seed {a.seed}: {len(specs)} problems, {sum(z[1] for z in sizes) / 1e9:.0f} GB of "
         "gradient buffers in all")
     tuned = {}
     if not a.no_tune:
         for h in sorted({s[0] for s in specs}):
             shots = max(r for hh, r, _ in specs if hh == h)
-            tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log)
+            tuned[h] = tune(h, h, shots, dev, use_cache=False, log=log, adjoint=not recon)
     t0 = time.perf_counter()
     data = [observed_data(s, i, a.seed, dev) for i, s in enumerate(specs)]
     torch.cuda.synchronize()
     log(f"observed data in {time.perf_counter() - t0:.1f} s")
     cells = sum(z[0] for z in sizes)
-    sched = WaveSchedule(lambda i: problem_from(specs[i], data[i], dev), sizes,
+    sched = WaveSchedule(lambda i: problem_from(specs[i], data[i], dev, recon), sizes,
                          a.budget_gb * 1e9, a.wave_cells)
     waves = [[int(i) for i in w] for w in sched.waves]
     log(f"{len(waves)} waves: " + ", ".join(str(len(w)) for w in waves))
@@ -162,6 +170,7 @@ def main(argv=None):
     report = {
         "workload": f"uneven_{len(specs)}_fwi_gradients",
         "seed": a.seed,
+        "adjoint": "rebuilds the forward fields" if recon else "recomputed segments",
         "problems": [{"grid": h, "shots": r, "steps": s} for h, r, s in specs],
         "waves": waves,
         "budget_gb": a.budget_gb, "wave_cells": a.wave_cells,

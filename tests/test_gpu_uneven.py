@@ -64,7 +64,8 @@ def test_workload_draw_in_waves_matches_single_problem_gradients():
         g_g, J_g = graphed[i]
         assert torch.equal(g_g, g_e) and J_g == J_e, i
         p = problem_from(spec, data[i], DEV)
-        eng = CheckpointedGradient(p.solver, p.sampler, p.S, p.N, checkpoint_every=p.K)
+        eng = CheckpointedGradient(p.solver, p.sampler, p.S, p.N, checkpoint_every=p.K,
+                                   reconstruct=p.reconstruct)
         samples = eng.forward(p.rows, p.cols, p.values)
         residual, misfit = residual_and_misfit(samples, p.observed, p.index, p.index)
         g = torch.zeros_like(p.grad)
