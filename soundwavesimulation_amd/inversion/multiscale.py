@@ -74,12 +74,21 @@ class StageResult:
     seconds: float
     rmse: list = field(default_factory=list)
     times: list = field(default_factory=list)
+    stopped: bool = False  # the run reached stop_rmse in this stage
 
 
-def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out, cached=None):
+class _TargetReached(Exception):
+    def __init__(self, m):
+        super().__init__("target RMSE reached")
+        self.m = m
+
+
+def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out, cached=None,
+                          stop_rmse=None, misfit_out=None):
     """Wrap misfit_and_grad to record interior RMS speed error and wall time.
 
-    ``cached = (m, value, grad)`` answers the first call at exactly that ``m``."""
+    ``cached = (m, value, grad)`` answers the first call at exactly that ``m``;
+    with ``stop_rmse`` the first evaluation at or below it ends the stage."""
     pending = [cached] if cached is not None else []
 
     def misfit_and_grad(m):
@@ -90,6 +99,10 @@ def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out, c
             value, grad = problem.misfit_and_grad(m)
         rmse_out.append(rms_c_error(c_from_m(problem.project(m)), c_true_grid, mask))
         times_out.append(time.perf_counter() - t0)
+        if misfit_out is not None:
+            misfit_out.append(float(value))
+        if stop_rmse is not None and rmse_out[-1] <= stop_rmse:
+            raise _TargetReached(m)
         return value, grad
 
     return misfit_and_grad
@@ -98,12 +111,14 @@ def _objective_with_trace(problem, c_true_grid, mask, t0, rmse_out, times_out, c
 def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_min=1400.0,
               c_max=1650.0, c_background=1500.0, cfl=0.3, parallel_shots="auto",
               optimizer="cg", learning_rate=None, margin_cells=2, verbose=False, t0=None,
-              start_hook=None):
+              start_hook=None, stop_rmse=None):
     """Run one stage; ``c_true_fn(x_m, y_m)`` gives the true speed on the stage grid.
 
     ``start_hook(problem, m_start, c_true, mask)`` may replace the start once the
     stage's objective exists (its time counts in the stage); it returns ``m`` or
-    ``(m, (value, grad))`` when it has already evaluated the objective at ``m``."""
+    ``(m, (value, grad))`` when it has already evaluated the objective at ``m``.
+    ``stop_rmse``: the stage ends at the first objective evaluation whose
+    interior speed RMSE is at or below it (``StageResult.stopped``)."""
     t_start = time.perf_counter() if t0 is None else t0
     x = axis_centers(stage.n, stage.spacing_m)
     c_true = c_true_fn(x, x)
@@ -131,9 +146,23 @@ def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_m
             m_start, (value, grad) = m_start
             m_start = problem.project(m_start)
             cached = (m_start, value, grad)
-    rmse, times = [], []
-    mg = _objective_with_trace(problem, c_true, mask, t_start, rmse, times, cached)
+    rmse, times, misfits = [], [], []
+    mg = _objective_with_trace(problem, c_true, mask, t_start, rmse, times, cached,
+                               stop_rmse=stop_rmse, misfit_out=misfits)
     t_stage = time.perf_counter()
+    try:
+        m, history = _optimise(optimizer, m_start, problem, mg, mask, stage, learning_rate,
+                               c_background, verbose)
+    except _TargetReached as hit:
+        return StageResult(m=problem.project(hit.m), history={"misfit": misfits},
+                           seconds=time.perf_counter() - t_stage, rmse=rmse, times=times,
+                           stopped=True), problem, c_true, mask
+    return StageResult(m=m, history=history, seconds=time.perf_counter() - t_stage, rmse=rmse,
+                       times=times), problem, c_true, mask
+
+
+def _optimise(optimizer, m_start, problem, mg, mask, stage, learning_rate, c_background,
+              verbose):
     if optimizer == "cg":
         m, history = polak_ribiere(m_start, problem.misfit, mg, problem.project, mask,
                                    max_iter=stage.iterations, verbose=verbose)
@@ -147,8 +176,7 @@ def run_stage(stage, c_true_fn, m_start, device, *, ring_radius, n_elements, c_m
                                                  verbose=verbose)
     else:
         raise ValueError(f"Unknown optimizer {optimizer!r}; use cg, adam or gd.")
-    return StageResult(m=m, history=history, seconds=time.perf_counter() - t_stage, rmse=rmse,
-                       times=times), problem, c_true, mask
+    return m, history
 
 
 def multiscale_fwi(stages, c_true_fn, m0_fn, device, start_hook=None, on_stage=None,
@@ -175,6 +203,8 @@ def multiscale_fwi(stages, c_true_fn, m0_fn, device, start_hook=None, on_stage=N
         results.append(result)
         if on_stage is not None:
             on_stage(results)
+        if result.stopped:
+            break
         m = result.m
         axes = (x, x)
     return results

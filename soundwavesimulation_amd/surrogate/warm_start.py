@@ -229,12 +229,15 @@ def surrogate_hook(op, net_grid, fov_m, info, batch=64):
 
 def run(kind, device, n=1024, spacing_m=0.25e-3, n_elements=256, radius_m=110e-3, n_steps=6000,
         freqs=(150e3, 300e3, 500e3), iterations=20, n_ellipses=12, seed=7, sigma_m=10e-3,
-        target_rmse=10.0, surrogate_kwargs=None, surrogate=None, verbose=True, partial=None):
+        target_rmse=10.0, surrogate_kwargs=None, surrogate=None, verbose=True, partial=None,
+        stop_at_target=False):
     """One run (``plain`` or ``warm``) of the workload; returns the report dict.
 
     ``surrogate``: a trained ``(operator, report)`` pair (``load_surrogate``);
     without it the warm run trains one first (``surrogate_kwargs``); ``partial(report)``
-    receives the report so far after every stage."""
+    receives the report so far after every stage; ``stop_at_target``: the run
+    ends at the first objective evaluation whose interior RMSE is at or below
+    ``target_rmse``."""
     if kind not in ("plain", "warm"):
         raise ValueError("kind must be 'plain' or 'warm'")
     fov = n * spacing_m
@@ -289,6 +292,7 @@ def run(kind, device, n=1024, spacing_m=0.25e-3, n_elements=256, radius_m=110e-3
                         "times": [round(t, 2) for t in res.times]}
                        for f, res in zip(freqs, results)],
             "misfit_final": results[-1].history["misfit"][-1],
+            "stopped_at_target": bool(results[-1].stopped),
         })
         if verbose:
             log(f"stage {len(results)} done at {report['seconds']:.1f} s: RMSE "
@@ -297,7 +301,8 @@ def run(kind, device, n=1024, spacing_m=0.25e-3, n_elements=256, radius_m=110e-3
             partial(report)
 
     multiscale_fwi(stages, c_true_fn, m0_fn, device, start_hook=hook, on_stage=summarise,
-                   ring_radius=radius_m, n_elements=n_elements, verbose=verbose)
+                   ring_radius=radius_m, n_elements=n_elements, verbose=verbose,
+                   stop_rmse=target_rmse if stop_at_target else None)
     return report
 
 
@@ -316,6 +321,8 @@ def main(argv=None):
     p.add_argument("--test", type=int, default=128)
     p.add_argument("--epochs", type=int, default=150)
     p.add_argument("--seed", type=int, default=7)
+    p.add_argument("--stop-at-target", action="store_true",
+                   help="end the run when the interior speed RMSE reaches 10 m/s")
     p.add_argument("--out", default=None)
     a = p.parse_args(argv)
     dev = torch.device("cuda", 0)
@@ -338,7 +345,7 @@ def main(argv=None):
 
         rep = run(a.run, dev, n=a.grid, spacing_m=spacing, n_elements=a.elements,
                   n_steps=a.steps, iterations=a.iterations, seed=a.seed, surrogate_kwargs=kw,
-                  surrogate=sur, partial=partial)
+                  surrogate=sur, partial=partial, stop_at_target=a.stop_at_target)
     line = json.dumps(rep)
     print(line, flush=True)
     if a.out:

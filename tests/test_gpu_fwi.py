@@ -31,3 +31,32 @@ def test_two_stage_inversion_reduces_misfit_and_rmse():
               f"rmse {r.rmse[0]:.2f} -> {r.rmse[-1]:.2f} m/s; {r.seconds:.2f} s")
         assert r.history["misfit"][-1] < 0.7 * r.history["misfit"][0]
     assert results[-1].rmse[-1] < results[0].rmse[0]
+
+
+def test_inversion_stops_at_the_target_rmse():
+    """With stop_rmse between the start's RMSE and the first stage's end, the
+    run ends inside the first stage at the first evaluation at or below it,
+    and the second stage never runs."""
+    n, h, radius = 96, 0.5e-3, 0.020
+
+    def c_true_fn(x_m, y_m):
+        return ellipse_phantom(len(x_m), len(y_m), h, 3, seed=1, ring_radius_m=radius,
+                               semi_axes_m=(3e-3, 7e-3))
+
+    def m0_fn(x_m, y_m):
+        mask = interior_mask(x_m, y_m, radius, 2 * h)
+        return blurred_start(c_true_fn(x_m, y_m), 3e-3, h, mask)
+
+    stages = [Stage(n, h, 200e3, 700, iterations=5), Stage(n, h, 400e3, 700, iterations=4)]
+    full = multiscale_fwi(stages[:1], c_true_fn, m0_fn, DEV, ring_radius=radius, n_elements=24,
+                          parallel_shots=24)[0]
+    target = 0.5 * (full.rmse[0] + min(full.rmse))
+    results = multiscale_fwi(stages, c_true_fn, m0_fn, DEV, ring_radius=radius, n_elements=24,
+                             parallel_shots=24, stop_rmse=target)
+    r = results[0]
+    print(f"rmse {r.rmse[0]:.2f} -> {min(full.rmse):.2f} m/s in full; target {target:.2f}: "
+          f"stopped after {len(r.rmse)} evaluations at {r.rmse[-1]:.2f}")
+    assert len(results) == 1 and r.stopped
+    assert r.rmse[-1] <= target and all(v > target for v in r.rmse[:-1])
+    assert r.rmse == full.rmse[:len(r.rmse)]
+    assert len(r.history["misfit"]) == len(r.rmse)
