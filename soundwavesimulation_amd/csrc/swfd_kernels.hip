@@ -139,6 +139,9 @@ struct StepParams {
     // the lean marches (every level images and rebuilds a field of step >= 0,
     // segments of at least two rows, rows of at least two wavefronts)
     int adjr_lean;
+    // full-width forward launches with footprint stores: segments away from
+    // rows 1 / H - 1 take the lean marches (rows of at least two wavefronts)
+    int fwdm_lean;
 };
 
 // Floats per shot in a slot of the boundary ring: row 0, row H - 1, then
@@ -1276,10 +1279,204 @@ __device__ __forceinline__ void fwdm_wave(const StepParams &p, int lane, int wav
             }
             __syncthreads();
         };
-        for (int y = y_begin; y <= y_end; y += kPfW) {
-#pragma unroll
-            for (int i = 0; i < kPfW; ++i) iteration(y + i, qb[i], qa[i], qk[i], qm[i]);
+        // exactly y_end - y_begin + 1 iterations (one barrier each), as the
+        // lean march runs
+        static_assert(kPfW == 3, "the march below takes its prefetch slots in threes");
+        for (int y = y_begin;; y += kPfW) {
+            iteration(y, qb[0], qa[0], qk[0], qm[0]);
+            if (y + 1 > y_end) break;
+            iteration(y + 1, qb[1], qa[1], qk[1], qm[1]);
+            if (y + 2 > y_end) break;
+            iteration(y + 2, qb[2], qa[2], qk[2], qm[2]);
+            if (y + 3 > y_end) break;
         }
+    }
+}
+
+// Lean forward march: footprint stores (the last two levels everywhere, the
+// others in the rows and wavefronts holding receiver footprint cells, and with
+// BND on the boundary columns too), in a segment whose march reaches neither
+// row 1 nor row H - 1, of a row spanning at least two wavefronts. SIDE 0 = a
+// wavefront holding neither column 0 nor W - 1, 1 = the one holding column 0,
+// 2 = the one holding W - 1. Per cell it does fwdm_wave's arithmetic in the
+// same order (the same bits); the bookkeeping is that of adjr_lean: rings of
+// registers made static by a six-iteration unroll (no latch copies), the
+// prefetch loading straight into the rings, a compile-time prologue for the
+// rows below the segment and no row tests after it (the last iterations'
+// rows beyond the segment are the next segment's, made from the same inputs:
+// the same bits), the footprint mask of the next row loaded one iteration
+// ahead into a scalar register (the barrier's wait covers it), every store
+// issued with a dropped lane offset where nothing is stored (exact wait
+// counts), and a one-sided Mur update on the side wavefronts.
+template <int KL, int SIDE, unsigned F>
+__device__ __forceinline__ void fwdm_lean(const StepParams &p, int lane, int wave, int grp,
+                                          int seg, EdgeRow *ex)
+{
+    constexpr bool BND = (F & kBnd) != 0;
+    constexpr int kR = 6;             // ring length = iterations per unrolled group
+    constexpr int kPro = 2 * KL - 2;  // iterations before every stored row is owned
+    constexpr int kPf = SIDE == 0 ? 3 : 2;  // rows loaded ahead (side marches hold more)
+    static_assert(kR % kPf == 0 && kPf + 3 <= kR, "prefetched rows must fit the rings");
+    const int W = p.nx, H = p.ny;
+    const int x = wave * kLanes - p.wide_off + lane;
+    const bool owned = SIDE == 0 || ((x >= 0) & (x < W));
+    const int xo = (SIDE == 0 ? x : min(max(x, 0), W - 1)) * 4;
+    const int so = owned ? x * 4 : kDropOffset;
+    const bool side_lane = SIDE == 1 ? x == 0 : (SIDE == 2 ? x == W - 1 : false);
+    const int mo = SIDE == 2 ? H * 4 : 0;
+    int r0, r1;
+    seg_rows(p, seg, r0, r1);
+    const int y_begin = r0 - (KL - 1);
+    const int y_end = r1 + KL - 2;  // level KL reaches row r1 - 1
+    const int n_it = y_end - y_begin + 1;
+    const size_t plane = (size_t)H * W;
+    const unsigned plane_bytes = (unsigned)(plane * sizeof(float));
+    const int row_bytes = W * 4;
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(p.c2dt2, plane_bytes);
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(p.mur, (unsigned)(2 * H * sizeof(float)));
+    const kci *fp = (const kci *)p.fp_rows;
+    const float idx2 = p.idx2, idy2 = p.idy2, cen = p.cen;
+    const int w_slot = lane == 0 ? wave : wave + 1;
+    const int w_side = lane == 0 ? 1 : 0;
+    const int e_slot = lane == kLanes - 1 ? wave + 2 : wave + 1;
+    const int e_side = lane == kLanes - 1 ? 0 : 1;
+    const bool edge_lane = (lane == 0) | (lane == kLanes - 1);
+    const int my_side = lane == kLanes - 1 ? 1 : 0;
+    // edge rows by iteration parity: iteration c reads ex_c[c & 1] and writes
+    // ex_c[(c + 1) & 1] (the rings' length is even, so c & 1 is static)
+    EdgeRow *const ex_c[2] = {ex + (y_begin & 1), ex + ((y_begin + 1) & 1)};
+    const int s_begin = grp * p.shots_per_group;
+    const int s_end = min(p.n_shots, s_begin + p.shots_per_group);
+    const int src_extra = transmitter_rows(p);
+    for (int s = s_begin; s < s_end; ++s) {
+        const size_t soff = (size_t)s * plane;
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.f_old + soff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(p.f_cur + soff, plane_bytes);
+        __amdgpu_buffer_rsrc_t ro[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j)
+            ro[j] = make_rsrc((const float *)((uintptr_t)p.lv_out[j] + soff * sizeof(float)),
+                              plane_bytes);
+        const int sr = p.src_rows[s], sc = p.src_cols[s];
+        float sv[KL];
+#pragma unroll
+        for (int j = 0; j < KL; ++j) sv[j] = p.src_values[p.src_index + j];
+        // ring slot of iteration c = c mod kR: u[0] = the input's row y + 1,
+        // ua = the older input's row y, kh / mh = dt^2 c^2 and the Mur side
+        // coefficient of row y, fm = the footprint mask of row y; u[j] (j >= 1)
+        // = level j's row made at iteration c (row y - j + 1). The prefetch
+        // loads u[0] and ua straight into their rings; kh and mh stay in use
+        // for KL iterations and go through slots qk, qm.
+        float u[KL + 1][kR], ua[kR], kh[kR], mh[kR], qk[kPf], qm[kPf];
+        int fm[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+#pragma unroll
+            for (int j = 0; j <= KL; ++j) u[j][k] = 0.0f;
+            ua[k] = kh[k] = mh[k] = 0.0f;
+            fm[k] = 0;
+        }
+        // iterations -1 and -2: rows y_begin and y_begin - 1
+        u[0][kR - 1] = bload(rb, xo, y_begin * row_bytes);
+        u[0][kR - 2] = bload(rb, xo, (y_begin - 1) * row_bytes);
+#pragma unroll
+        for (int i = 0; i < kPf; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            const int o0 = (y_begin + i) * row_bytes;
+            u[0][i] = bload(rb, xo, o0 + row_bytes);
+            ua[i] = bload(ra, xo, o0);
+            qk[i] = bload(rk, xo, o0);
+            qm[i] = 0.0f;
+            if constexpr (SIDE != 0) qm[i] = bload(rm, mo, (y_begin + i) * 4);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        fm[0] = fp[y_begin];
+        if (edge_lane) {
+            Edge4 e;
+#pragma unroll
+            for (int j = 0; j < 2 * kMaxLevels; ++j) e.v[j] = 0.0f;
+            e.v[0] = u[0][kR - 1];
+            e.v[kMaxLevels] = u[0][kR - 2];
+            (*ex_c[0])[wave + 1][my_side] = e;
+        }
+        __syncthreads();
+        // iteration c with ring slot cm = c mod kR; pc = c in the first kPro
+        // iterations (row tests at compile time), -1 afterwards (no tests)
+        auto iteration = [&](int cm, int pc, int c) __attribute__((always_inline)) {
+            const int y = y_begin + c;
+            const int cn = cm, cc = (cm + kR - 1) % kR, cs = (cm + kR - 2) % kR;
+            const int q = cm % kPf;
+            const float k_new = take(qk[q]);
+            const float m_new = SIDE != 0 ? take(qm[q]) : 0.0f;
+            __builtin_amdgcn_sched_barrier(0);
+            {
+                const int f = (cm + kPf) % kR;
+                const int yp = min(y + kPf, y_end);
+                const int o0 = yp * row_bytes;
+                u[0][f] = bload(rb, xo, o0 + row_bytes);
+                ua[f] = bload(ra, xo, o0);
+                qk[q] = bload(rk, xo, o0);
+                if constexpr (SIDE != 0) qm[q] = bload(rm, mo, yp * 4);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            kh[cm] = k_new;
+            mh[cm] = m_new;
+            // the next iteration's footprint row (scalar; the barrier at the
+            // end of this iteration waits for it)
+            fm[(cm + 1) % kR] = fp[min(y + 1, y_end)];
+            const bool near_src = (unsigned)(y - sr) <= (unsigned)(KL - 1 + src_extra);
+            const EdgeRow &er = *ex_c[cm & 1];
+#pragma unroll
+            for (int j = 1; j <= KL; ++j) {
+                const int rho = y - j + 1;
+                const int kj = (cm + kR - (j - 1)) % kR;  // row rho's coefficients and mask
+                const float uc = u[j - 1][cc], un = u[j - 1][cn], us = u[j - 1][cs];
+                const float old = j == 1 ? ua[cm] : u[j - 2][cs];
+                const float uw = from_west(uc, er[w_slot][w_side].v[j - 1]);
+                const float ue = from_east(uc, er[e_slot][e_side].v[j - 1]);
+                const float lp = lap5(uw, ue, us, un, uc, idx2, idy2, cen);
+                float v = __fmaf_rn(kh[kj], lp, leap(uc, old));
+                if (__builtin_expect(near_src, 0) && (unsigned)(rho - sr) <= (unsigned)src_extra)
+                    v = add_transmitter(p, s, v, rho, x, sr, sc, sv[j - 1]);
+                if constexpr (SIDE == 1) {  // column 0: from column 1's new and current values
+                    const float ve = from_east(v, 0.0f), ce = from_east(uc, 0.0f);
+                    v = side_lane ? ce + mh[kj] * (ve - uc) : v;
+                } else if constexpr (SIDE == 2) {  // column W - 1: from column W - 2's
+                    const float vw = from_west(v, 0.0f), cw = from_west(uc, 0.0f);
+                    v = side_lane ? cw + mh[kj] * (vw - uc) : v;
+                }
+                u[j][cm] = v;
+                if (pc < 0 || pc >= KL + j - 2) {  // row rho >= r0
+                    int off = so;
+                    if (j < KL - 1) {  // footprint rows (and the ring's columns)
+                        const bool fp_here = ((unsigned)fm[kj] >> wave) & 1u;
+                        off = (fp_here || (BND && side_lane)) ? so : kDropOffset;
+                    }
+                    bstore(ro[j - 1], v, off, rho * row_bytes);
+                }
+            }
+            if (edge_lane) {  // the newest row of each level (the rest is not read)
+                float e[kMaxLevels];
+#pragma unroll
+                for (int j = 0; j < kMaxLevels; ++j) e[j] = j < KL ? u[j][cm] : 0.0f;
+                *reinterpret_cast<float4 *>(&(*ex_c[(cm + 1) & 1])[wave + 1][my_side].v[0]) =
+                    make_float4(e[0], e[1], e[2], e[3]);
+            }
+            __syncthreads();
+        };
+#pragma unroll
+        for (int c = 0; c < kPro; ++c) iteration(c % kR, c, c);
+        // the rest in groups of kR from ring slot kPro mod kR; exactly n_it
+        // iterations in all (one barrier each, as in fwdm_wave)
+        int c = kPro;
+        for (;;) {
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                iteration((kPro + k) % kR, -1, c);
+                if (++c == n_it) goto march_done;
+            }
+        }
+    march_done:;
     }
 }
 
@@ -1311,6 +1508,15 @@ __device__ __forceinline__ void fwdm_block(const StepParams &p, int b, EdgeRow *
     seg_rows(p, seg, r0, r1);
     const bool xe = (wave == 0) || (wave == p.wide_waves - 1);
     const bool ye = (r0 <= KL) || (r1 + KL - 2 >= p.ny - 1);
+    if constexpr (SM == kStoreFootprint && (F & (kLap | kOut2)) == 0) {
+        // lean marches away from rows 1 / H - 1 (the same bits)
+        if (!ye && p.fwdm_lean) {
+            if (wave == 0) fwdm_lean<KL, 1, F>(p, lane, wave, grp, seg, ex);
+            else if (wave == p.wide_waves - 1) fwdm_lean<KL, 2, F>(p, lane, wave, grp, seg, ex);
+            else fwdm_lean<KL, 0, F>(p, lane, wave, grp, seg, ex);
+            return;
+        }
+    }
     if (xe) {
         if (ye) fwdm_wave<KL, SM, true, true, F>(p, lane, wave, grp, seg, ex);
         else fwdm_wave<KL, SM, true, false, F>(p, lane, wave, grp, seg, ex);
@@ -3529,6 +3735,14 @@ int fwdm_levels_variant(int levels, int sm, unsigned f, Fn &&fn)
 // levels stored by the caller's plan): fills the geometry into p and returns
 // the workgroup count, the threads per workgroup, the flags and the store
 // pattern.
+// Whether footprint-store forward launches may use the lean marches
+// (SWFD_FWD_LEAN=0 turns them off; read at every launch).
+bool fwdm_lean_enabled()
+{
+    const char *e = std::getenv("SWFD_FWD_LEAN");
+    return e == nullptr || std::strcmp(e, "0") != 0;
+}
+
 int multi_shape(StepParams &p, int gather_pairs_count, int &threads, unsigned &f, int &sm)
 {
     const int min_levels = p.hist_lap ? 1 : 2;
@@ -3556,6 +3770,10 @@ int multi_shape(StepParams &p, int gather_pairs_count, int &threads, unsigned &f
     // a ring is taken from the stored fields: levels kept only where receivers
     // sit keep their boundary cells too
     if (p.bnd != nullptr && sm == kStoreFootprint) f |= kBnd;
+    // lean marches for footprint-store launches (SWFD_FWD_LEAN=0: the general
+    // march everywhere, with the same bits)
+    p.fwdm_lean = (sm == kStoreFootprint && p.wide_waves >= 2 && p.ny / p.n_seg >= 2 &&
+                   fwdm_lean_enabled()) ? 1 : 0;
     return blocks;
 }
 

@@ -214,6 +214,50 @@ def test_lean_march_equals_the_general_march(ny, nx, n_steps, bilinear, monkeypa
     assert float(g1.abs().max()) > 0.0
 
 
+@pytest.mark.parametrize("ny,nx,n_steps,bilinear", [
+    (320, 384, 403, False),  # six wavefronts per row (two side ones), five forward segments
+    (400, 257, 389, True),   # nx = 64 k + 1; bilinear sources
+    (520, 1024, 301, False),  # sixteen wavefronts per row
+])
+def test_lean_forward_march_equals_the_general_march(ny, nx, n_steps, bilinear, monkeypatch):
+    """Footprint-store forward launches away from rows 1 and ny - 1 take the
+    lean forward marches; SWFD_FWD_LEAN=0 sends them through the general
+    march. Receiver samples, the boundary ring of every step and the final
+    pair must be the same bits (transmitters inside lean segments, receiver
+    footprints crossing interior and side wavefronts)."""
+    print(f"lean vs general forward {ny}x{nx}, {n_steps} steps, bilinear {bilinear}",
+          flush=True)
+    pb = ring_problem(ny, nx, 16, 9)
+    solver = TorchFDTD2D(TorchScalarWave2D(pb["c_true"], pb["h"], device=DEV), pb["dt"])
+    sampler = TorchRingSampler(pb["array"], device=DEV)
+    shots = np.arange(16)
+    S = len(shots)
+    if bilinear:
+        rows, cols, w = pb["array"].bilinear_sources(shots)
+        src_w = torch.as_tensor(w.astype(np.float32)).to(DEV)
+    else:
+        rows, cols = pb["array"].rows[shots], pb["array"].cols[shots]
+        src_w = None
+    r = torch.as_tensor(np.asarray(rows, np.int32)).to(DEV)
+    c = torch.as_tensor(np.asarray(cols, np.int32)).to(DEV)
+    values = torch.zeros(n_steps, device=DEV)
+    p = torch.as_tensor(pb["pulse"], dtype=torch.float32)[:n_steps]
+    values[:p.numel()] = p.to(DEV)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SWFD_FWD_LEAN", flag)
+        ring = torch.full((n_steps, S, boundary_floats((ny, nx))), float("nan"), device=DEV)
+        samples = torch.empty((n_steps, S, sampler.n_elements), device=DEV)
+        work = [torch.empty((S, ny, nx), device=DEV) for _ in range(6)]
+        forward_run(solver, S, r, c, values, n_steps, samples=samples, sampler=sampler,
+                    work=work, src_weights=src_w, bnd=ring)
+        torch.cuda.synchronize()
+        out[flag] = (samples, ring, work[0].clone(), work[1].clone())
+    for a, b in zip(out["0"], out["1"]):
+        assert torch.equal(a, b)
+    assert float(out["1"][0].abs().max()) > 0.0
+
+
 def test_reference_problem_gradient_with_reconstruction():
     g64 = np.load(os.path.join(GOLDEN, "ring41_fp64.npz"))
     sp, dt = float(g64["spacing"]), float(g64["dt"])
