@@ -201,8 +201,8 @@ def adjoint_roofline(problem, t_launch, S, L, n_launch):
     ny, nx = problem.model.shape
     R = problem.sampler.n_elements
     cells = ny * nx
-    shape = native.adjoint_launch_shape(ny, nx, S)
     recon = problem._engine(S).reconstruct
+    shape = native.adjoint_launch_shape(ny, nx, S, recon)
     if recon:
         # per shot the adjoint pair and the forward pair read and written (8
         # fields), the boundary cells of the L rebuilt fields; c^2 and the

@@ -217,12 +217,14 @@ int32_t swfd_lap_history(int32_t ny, int32_t nx);
 int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samples);
 
 /* Launch shape of the imaging adjoint runs of n_shots shots on an ny x nx
- * grid given work3 (no counterpart in the reference; for profiling): out[0]
- * steps per launch, out[1] 1 for full-width launches, and for those out[2]
- * rows per segment, out[3] segments, out[4] shot groups, out[5] shots per
- * group (zeros otherwise). Host only, but the shape depends on the current
- * device's compute-unit count. */
-int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *out);
+ * grid given work3 (no counterpart in the reference; for profiling), for
+ * runs that rebuild the forward field (recon = 1) or read a history (0):
+ * out[0] steps per launch, out[1] 1 for full-width launches, and for those
+ * out[2] rows per segment, out[3] segments, out[4] shot groups, out[5] shots
+ * per group (zeros otherwise). Host only, but the shape depends on the
+ * current device's compute-unit count. */
+int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t recon,
+                              int32_t *out);
 
 /* Rows and full-width wavefronts that hold receiver footprint cells
  * (host-only, no launch; host pointers; no counterpart in the reference):
@@ -297,7 +299,9 @@ int swfd_adjoint(const swfd_grid *grid, const swfd_medium *medium, const swfd_re
                  const swfd_adjoint_args *args, void *stream);
 
 /* Workspace floats needed by swfd_adjoint with imaging: one ny x nx partial
- * image per shot group (up to 32 groups, at most n_shots, at least 8). */
+ * image per shot group (at most n_shots groups and at least 8; up to 32 on
+ * grids of 1024^2 cells and more, up to 2^25 / (ny nx) -- 512 at 256^2 --
+ * on smaller ones). */
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots);
 
 /* Workspace floats needed by swfd_residual. */

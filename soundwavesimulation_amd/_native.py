@@ -149,7 +149,7 @@ def load_library(path=LIBRARY):
     lib.swfd_lap_history.argtypes = [_c_int, _c_int]
     lib.swfd_steps_per_launch.restype = _c_int
     lib.swfd_steps_per_launch.argtypes = [_c_int, _c_int, _c_int, _c_int]
-    lib.swfd_adjoint_launch_shape.argtypes = [_c_int, _c_int, _c_int, vp]
+    lib.swfd_adjoint_launch_shape.argtypes = [_c_int, _c_int, _c_int, _c_int, vp]
     lib.swfd_boundary_floats.restype = ctypes.c_size_t
     lib.swfd_boundary_floats.argtypes = [_c_int, _c_int]
     lib.swfd_adjoint_recon.restype = _c_int
@@ -207,11 +207,12 @@ def require_device(device):
                         else device.index)
 
 
-def adjoint_launch_shape(ny, nx, n_shots):
+def adjoint_launch_shape(ny, nx, n_shots, recon=True):
     """``{"steps_per_launch", "full_width", "segment_rows", "segments", "shot_groups",
-    "shots_per_group"}`` of the imaging adjoint runs (needs the device)."""
+    "shots_per_group"}`` of the imaging adjoint runs that rebuild the forward
+    field (``recon``) or read a history (needs the device)."""
     out = (ctypes.c_int32 * 6)()
-    check(lib().swfd_adjoint_launch_shape(ny, nx, n_shots, out))
+    check(lib().swfd_adjoint_launch_shape(ny, nx, n_shots, 1 if recon else 0, out))
     keys = ("steps_per_launch", "full_width", "segment_rows", "segments", "shot_groups",
             "shots_per_group")
     return dict(zip(keys, (int(v) for v in out)))

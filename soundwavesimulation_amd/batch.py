@@ -386,6 +386,13 @@ class UnevenGradients:
         return sum(p.cells for p in self.problems)
 
 
+def max_wide_groups(ny, nx):
+    """Shot groups (one partial image each) a full-width adjoint launch on an
+    ny x nx grid may use: 32 on grids of 1024^2 cells and more, up to
+    2^25 / (ny nx) (at most 512) on smaller ones -- the library's rule."""
+    return max(32, min(512, (32 << 20) // (ny * nx)))
+
+
 def gradient_problem_bytes(ny, nx, n_shots, n_receivers, n_steps, checkpoint_every=None,
                            reconstruct=None):
     """Device bytes a ``GradientProblem`` allocates (work fields, checkpoints,
@@ -400,7 +407,7 @@ def gradient_problem_bytes(ny, nx, n_shots, n_receivers, n_steps, checkpoint_eve
     ckpt = max(n_ckpt, 2) if n_ckpt > 0 else 0
     fields = 4 + ckpt + (8 if reconstruct else n_hist)
     ring = n_steps * boundary_floats((ny, nx)) if reconstruct else 0
-    images = max(8, min(32, n_shots))
+    images = max(8, min(max_wide_groups(ny, nx), n_shots))
     return 4 * (fields * n_shots * plane + n_shots * ring + images * plane +
                 2 * n_steps * n_shots * n_receivers)
 

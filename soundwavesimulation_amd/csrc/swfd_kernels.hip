@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -2007,10 +2008,32 @@ __global__ void __launch_bounds__(256) adj2_batch_kernel(const StepParams *table
 // other; a cell's terms of the KL levels are summed in registers and then
 // added into an LDS image of the segment (each lane owns its column, so the
 // order is fixed); at the end the workgroup adds the segment into the group's
-// partial image (at most kMaxWideGroups), which are summed in group order
+// partial image (at most max_wide_groups), which are summed in group order
 // after the run: the gradient is deterministic.
 
 constexpr int kPfA = 2;  // rows loaded ahead in the adjoint march
+
+// col[i W] += scale * acc[i acc_stride] for rows i < n_rows of a lane's
+// column of a partial image, one fused multiply-add per cell. The rows'
+// loads go out kImgBatch at a time before their stores: a row's load cannot
+// move above the previous row's store (the compiler cannot prove the rows
+// apart), so one row at a time would wait a memory latency per row.
+constexpr int kImgBatch = 16;
+__device__ __forceinline__ void add_image_column(gf *col, int n_rows, int W, float scale,
+                                                 const float *acc, int acc_stride)
+{
+    int i = 0;
+    for (; i + kImgBatch <= n_rows; i += kImgBatch) {
+        float v[kImgBatch];
+#pragma unroll
+        for (int k = 0; k < kImgBatch; ++k) v[k] = col[(size_t)(i + k) * W];
+#pragma unroll
+        for (int k = 0; k < kImgBatch; ++k)
+            col[(size_t)(i + k) * W] = __fmaf_rn(scale, acc[(i + k) * acc_stride], v[k]);
+    }
+    for (; i < n_rows; ++i)
+        col[(size_t)i * W] = __fmaf_rn(scale, acc[i * acc_stride], col[(size_t)i * W]);
+}
 
 // (P of level j - 1, imaging field of level j) for j = 1..KL
 struct alignas(16) AdjEdge {
@@ -2307,10 +2330,7 @@ __device__ __forceinline__ void adjm_block(const StepParams &p, int b, AdjEdgeRo
         const int x = wave * kLanes - p.wide_off + lane;
         if ((x >= 0) & (x < W)) {
             gf *gp = gstore_ptr(p.g_part) + (size_t)grp * ((size_t)p.ny * W);
-            for (int i = 0; i < r1 - r0; ++i) {
-                gf *cell = gp + (size_t)(r0 + i) * W + x;
-                *cell = __fmaf_rn(p.img_scale, adjm_acc[i * W + x], *cell);
-            }
+            add_image_column(gp + (size_t)r0 * W + x, r1 - r0, W, p.img_scale, adjm_acc + x, W);
         }
     }
 }
@@ -2972,10 +2992,7 @@ __device__ __forceinline__ void adjr_block(const StepParams &p, int b, AdjEdgeRo
     const int x = wave * kLanes - p.wide_off + lane;
     if ((x >= 0) & (x < W)) {
         gf *gp = gstore_ptr(p.g_part) + (size_t)grp * ((size_t)p.ny * W);
-        for (int i = 0; i < r1 - r0; ++i) {
-            gf *cell = gp + (size_t)(r0 + i) * W + x;
-            *cell = __fmaf_rn(p.img_scale, adjm_acc[i * W + x], *cell);
-        }
+        add_image_column(gp + (size_t)r0 * W + x, r1 - r0, W, p.img_scale, adjm_acc + x, W);
     }
 }
 
@@ -3884,30 +3901,58 @@ int launch_adj_pair(StepParams &p, hipStream_t stream)
 // (one such workgroup per CU).
 constexpr size_t kMaxAdjwLds = 128 * 1024;
 
-// Shape of a full-width multi-step adjoint launch. The kernel needs about
-// 120 VGPRs (four wavefronts per SIMD), so a CU holds 16 / wavefronts-per-row
-// workgroups, fewer if their LDS images of the segment do not fit. A
-// workgroup marches its segment (plus the 3 rows shared with each
-// neighbour) once per shot of its group, so the launch takes about
-// ceil(workgroups / (CUs x resident)) x (rows + 3) x shots-per-group; the
-// shape minimises that over 8-32 row segments and 8-kMaxWideGroups shot
-// groups (one partial image each), preferring longer segments and fewer
-// groups on ties. SWFD_ADJW_SEG_ROWS / SWFD_ADJW_GROUPS or a kind-1 tuning
-// entry fix either.
-constexpr int kMaxWideGroups = 32;
+// Shape of a full-width multi-step adjoint launch: rows per segment and shot
+// groups (one partial image each). SWFD_ADJW_SEG_ROWS / SWFD_ADJW_GROUPS or a
+// kind-1 tuning entry fix either.
+//
+// Recomputing kernel (adjm_kernel): about 120 VGPRs (four wavefronts per
+// SIMD), so a CU holds 16 / wavefronts-per-row workgroups, fewer if their LDS
+// images of the segment do not fit. A workgroup marches its segment (plus the
+// 3 rows shared with each neighbour) once per shot of its group, so the
+// launch takes about ceil(workgroups / (CUs x resident)) x (rows + 3) x
+// shots-per-group; the shape minimises that over 8-32 row segments and 8-32
+// groups, preferring longer segments and, on ties, more groups.
+//
+// Reconstructing kernel (adjr_kernel): a march covers the segment plus 2 KL -
+// 1 = 7 rows. Measured on 256^2 - 1024^2 grids (128 and 256 shots, every
+// combination of 8 / 16 / 32 rows and 8 - 128 groups), the fastest launch
+// always had 32-row segments and about 3 - 4 workgroups per CU (256^2: 128
+// groups of 1 shot; 384^2: 64 groups of 2; 512^2: 64 of 2; 768^2: 32 of 4;
+// 1024^2 x 256 shots: 32 of 8), with whole rounds of workgroups over the CUs
+// (768^2 with 43 groups, 4.03 per CU: 18 % slower than 32 groups). More
+// groups per segment share its c^2 and scatter-map rows in an XCD's L2; too
+// many pay a partial image each. The shape minimises |log(ceil(m) / 3.5)| +
+// (ceil(m) - m) / ceil(m) + log((rows + 7) / rows), m = workgroups per CU,
+// over 8-32 row segments and 8 - max_wide_groups groups (on ties, longer
+// segments, then more groups).
+constexpr int kMaxWideGroups = 512;
+// Floats of partial images a launch may use: 32 images at 1024^2.
+constexpr size_t kWideImageFloats = (size_t)32 << 20;
+
+// Shot groups (partial images) a full-width adjoint launch on ny x nx may
+// use: 32 at 1024^2 and above, up to kMaxWideGroups on small grids (512 at
+// 256^2), so that small grids can fill the CUs with short shot loops.
+int max_wide_groups(int ny, int nx)
+{
+    const size_t plane = (size_t)ny * nx;
+    return (int)std::max<size_t>(32, std::min<size_t>(kMaxWideGroups, kWideImageFloats / plane));
+}
 
 struct AdjwShape {
     int n_seg, seg, n_groups, shots_per_group, acc_rows;
 };
 
-AdjwShape adjw_shape(int ny, int nx, int n_shots)
+// recon: the shape of the reconstructing kernel (adjr_kernel), else of the
+// recomputing one (adjm_kernel).
+AdjwShape adjw_shape(int ny, int nx, int n_shots, bool recon)
 {
     static const int seg_env = env_int("SWFD_ADJW_SEG_ROWS", 0);
     static const int groups_env = env_int("SWFD_ADJW_GROUPS", 0);
     const TuneEntry *t = find_tune(1, ny, nx);
     const int seg_fixed = t != nullptr && t->seg_rows >= 2 ? t->seg_rows : seg_env;
     const int groups_fixed = t != nullptr && t->param > 0 ? t->param : groups_env;
-    const int max_groups = std::max(1, std::min(kMaxWideGroups, n_shots));
+    const int max_groups =
+        std::max(1, std::min(recon ? max_wide_groups(ny, nx) : 32, n_shots));
     const int off = (nx % kLanes == 1) ? 1 : 0;
     const int waves = (nx + off + kLanes - 1) / kLanes;
     const int max_rows = std::max(1, (int)(kMaxAdjwLds / (sizeof(float) * (size_t)nx)));
@@ -3915,30 +3960,39 @@ AdjwShape adjw_shape(int ny, int nx, int n_shots)
         const int n = std::max(1, ny / seg);
         return std::max(n, (ny + max_rows - 1) / max_rows);
     };
-    auto cost = [&](int seg, int groups) {
+    auto cost = [&](int seg, int groups) -> double {
         const int n_seg = n_seg_of(seg);
         const int rows = (ny + n_seg - 1) / n_seg;
+        const long spg = (n_shots + groups - 1) / groups;
+        if (recon) {
+            const long used = (n_shots + spg - 1) / spg;  // groups actually formed
+            const double m = (double)n_seg * used / cu_count();
+            const double cm = std::ceil(m);
+            return std::fabs(std::log(cm / 3.5)) + (cm - m) / cm +
+                   std::log((rows + 2.0 * kMaxLevels - 1.0) / rows);
+        }
         const size_t lds = (size_t)rows * nx * sizeof(float) + 4096;
         const long resident = std::max(1, std::min(16 / waves, (int)((160 * 1024) / lds)));
         const long capacity = (long)cu_count() * resident;
         const long rounds = ((long)n_seg * groups + capacity - 1) / capacity;
-        const long spg = (n_shots + groups - 1) / groups;
-        return rounds * (rows + 3) * spg;
+        return (double)(rounds * (rows + 3) * spg);
     };
     int best_seg = seg_fixed >= 2 ? seg_fixed : 32;
     int best_groups = groups_fixed > 0 ? std::min(groups_fixed, max_groups) : std::min(8, max_groups);
-    long best = cost(best_seg, best_groups);
-    static const int kGroupChoices[3] = {8, 16, kMaxWideGroups};
+    double best = cost(best_seg, best_groups);
+    static const int kGroupChoices[7] = {8, 16, 32, 64, 128, 256, kMaxWideGroups};
     for (int seg = 32; seg >= 8; seg /= 2) {
         if (seg_fixed >= 2 && seg != seg_fixed) continue;
-        for (int gi = 0; gi < 3; ++gi) {
+        for (int gi = 0; gi < 7; ++gi) {
+            if (!recon && kGroupChoices[gi] > 32) break;
             const int groups = groups_fixed > 0 ? std::min(groups_fixed, max_groups)
                                                 : std::min(kGroupChoices[gi], max_groups);
-            const long c = cost(seg, groups);
+            const double c = cost(seg, groups);
             // on a tie, more groups: shorter shot loops per workgroup, and the
             // rounds overlap one workgroup's start with another's end (1024^2,
             // 128 shots: 32 rows x 16 groups 1.35 ms per launch, x 8 groups 1.40)
-            if (c < best || (c == best && seg == best_seg && groups > best_groups)) {
+            if (c < best - 1e-12 ||
+                (c <= best + 1e-12 && seg == best_seg && groups > best_groups)) {
                 best = c;
                 best_seg = seg;
                 best_groups = groups;
@@ -3985,7 +4039,7 @@ int adjm_shape(StepParams &p, int &threads, size_t &lds)
     const bool image = adjm_image(p) != 0;
     if (image && p.g_part == nullptr)
         return -fail(SWFD_EINVAL, "multi-step adjoint imaging needs the partial images");
-    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots);
+    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots, false);
     p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
     p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
     p.seg = a.seg;
@@ -4084,7 +4138,7 @@ int adjr_shape(StepParams &p, int &threads, size_t &lds)
     if (p.g_part == nullptr || p.r_next == nullptr || p.r_cur == nullptr || p.bnd == nullptr ||
         p.src_rows == nullptr || p.src_cols == nullptr || p.src_values == nullptr)
         return -fail(SWFD_EINVAL, "reconstructing adjoint: fields, ring, sources or images missing");
-    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots);
+    const AdjwShape a = adjw_shape(p.ny, p.nx, p.n_shots, true);
     p.wide_off = (p.nx % kLanes == 1) ? 1 : 0;
     p.wide_waves = (p.nx + p.wide_off + kLanes - 1) / kLanes;
     p.seg = a.seg;
@@ -4676,8 +4730,9 @@ struct AdjointRun {
         if (batched && a->hist_lap != 0)
             return fail(SWFD_EINVAL, "Laplacian history: not in batched runs");
         if (!wide) max_levels = std::min(max_levels, 2);
+        recon = a->recon_next != nullptr || a->recon_cur != nullptr;
         if (max_levels >= 2)
-            n_part = wide ? adjw_shape(g->ny, g->nx, a->n_shots).n_groups
+            n_part = wide ? adjw_shape(g->ny, g->nx, a->n_shots, recon).n_groups
                           : adj2_shape(g->ny, g->nx, a->n_shots).n_groups;
         else
             n_part = image_groups(g->ny, g->nx, a->n_shots);
@@ -4685,7 +4740,6 @@ struct AdjointRun {
         if (lap && (!wide || max_levels < 2))
             return fail(SWFD_EINVAL, "Laplacian history needs full-width multi-step adjoint "
                                      "launches (see swfd_lap_history) and work3");
-        recon = a->recon_next != nullptr || a->recon_cur != nullptr;
         if (recon) {
             if (a->recon_next == nullptr || a->recon_cur == nullptr || a->recon_work0 == nullptr ||
                 a->recon_work1 == nullptr || a->recon_work2 == nullptr ||
@@ -5572,7 +5626,8 @@ int32_t swfd_steps_per_launch(int32_t kind, int32_t ny, int32_t nx, int32_t samp
     return levels;
 }
 
-int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *out)
+int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t recon,
+                              int32_t *out)
 {
     if (ny < 3 || nx < 3 || n_shots < 1 || out == nullptr)
         return fail(SWFD_EINVAL, "adjoint launch shape: bad arguments");
@@ -5586,7 +5641,7 @@ int swfd_adjoint_launch_shape(int32_t ny, int32_t nx, int32_t n_shots, int32_t *
     out[0] = plan.levels;
     out[1] = plan.wide ? 1 : 0;
     if (plan.wide && plan.levels >= 2) {
-        const AdjwShape a = adjw_shape(ny, nx, n_shots);
+        const AdjwShape a = adjw_shape(ny, nx, n_shots, recon != 0);
         out[2] = a.acc_rows;
         out[3] = a.n_seg;
         out[4] = a.n_groups;
@@ -5627,7 +5682,7 @@ int32_t swfd_lap_history(int32_t ny, int32_t nx)
 
 size_t swfd_adjoint_workspace_floats(int32_t ny, int32_t nx, int32_t n_shots)
 {
-    const int groups = std::max(kMaxImageGroups, std::min(kMaxWideGroups, (int)n_shots));
+    const int groups = std::max(kMaxImageGroups, std::min(max_wide_groups(ny, nx), (int)n_shots));
     return (size_t)groups * (size_t)ny * (size_t)nx;
 }
 
