@@ -451,15 +451,58 @@ class WaveSchedule:
     ``spans`` (a list) receives a (start, end) event pair around each wave's
     launches. ``graph=True`` runs each wave once eagerly on a side stream,
     captures it as a HIP graph and replays it (the span covers the replay
-    only; results equal the eager run).
+    only; results equal the eager run). ``graph="whole"`` allocates every
+    wave's problems at once (their bytes must fit the budget together),
+    captures all the waves in order into one HIP graph and replays that
+    graph once: one launch from the host for the whole schedule (``spans``
+    gets one pair around the replay; no per-problem events).
     """
 
     def __init__(self, make_problem, sizes, budget_bytes, wave_cells):
         self.make_problem = make_problem
         self.sizes = list(sizes)
+        self.budget = float(budget_bytes)
         self.waves = plan_waves(self.sizes, budget_bytes, wave_cells)
 
+    def total_bytes(self):
+        return sum(self.sizes[i][1] for wave in self.waves for i in wave)
+
+    def _run_whole(self, spans=None):
+        if self.total_bytes() > self.budget:
+            raise ValueError(f"the whole schedule needs {self.total_bytes() / 1e9:.1f} GB, "
+                             f"over the {self.budget / 1e9:.1f} GB budget: run it in waves")
+        batches = []
+        for wave in self.waves:
+            problems = [self.make_problem(i) for i in wave]
+            batches.append((wave, problems, UnevenGradients(problems)))
+        dev = batches[0][1][0].solver.model.device
+        cur = torch.cuda.current_stream(dev)
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):  # warm-up on the capture stream
+            for _, _, ug in batches:
+                ug.run()
+        cur.wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _, _, ug in batches:
+                ug.run()
+        span = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        span[0].record()
+        g.replay()
+        span[1].record()
+        if spans is not None:
+            spans.append(span)
+        results = {}
+        for wave, problems, _ in batches:
+            for k, i in enumerate(wave):
+                results[i] = (problems[k].grad.cpu(), float(problems[k].misfit.cpu()))
+        del g, batches
+        return results
+
     def run(self, events=None, graph=False, spans=None):
+        if graph == "whole":
+            return self._run_whole(spans)
         results = {}
         for wave in self.waves:
             problems = [self.make_problem(i) for i in wave]

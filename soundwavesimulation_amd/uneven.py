@@ -12,7 +12,9 @@ with ``batch.WaveSchedule``: problems in order of increasing work, grouped
 into waves whose device memory stays within the budget (``--budget-gb``, so
 any draw runs) and whose cells per step stay within ``--wave-cells``; each
 wave is one ``UnevenGradients`` batch (one batched launch per kernel variant
-and tick), waves back to back.
+and tick), waves back to back. When every wave's buffers fit the budget
+together, the whole schedule is also captured into one HIP graph and timed
+as one replay.
 
 Reported (one JSON line): throughput in cell-updates per second (2 ny nx N S
 per problem, summed) over the eager run and over per-wave graph replays, the
@@ -194,6 +196,17 @@ def main(argv=None):
         report["graph_replay_seconds"] = round(sum(busy_g), 3)
         report["gcell_s_graph"] = round(cells / sum(busy_g) / 1e9, 2)
         log(f"graphs: {sum(busy_g):.2f} s of replays, {cells / sum(busy_g) / 1e9:.1f} Gcell/s")
+        if sched.total_bytes() <= sched.budget:
+            # every wave captured into one graph, replayed once
+            spans = []
+            torch.cuda.synchronize()
+            sched.run(graph="whole", spans=spans)
+            torch.cuda.synchronize()
+            t_whole = spans[0][0].elapsed_time(spans[0][1]) / 1e3
+            report["whole_graph_seconds"] = round(t_whole, 3)
+            report["gcell_s_whole_graph"] = round(cells / t_whole / 1e9, 2)
+            log(f"one graph of all {len(waves)} waves: {t_whole:.2f} s, "
+                f"{cells / t_whole / 1e9:.1f} Gcell/s")
     line = json.dumps(report)
     print(line, flush=True)
     if a.out:

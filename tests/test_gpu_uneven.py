@@ -7,7 +7,8 @@
   schedule (shortest work first, device memory per wave bounded); every
   problem's misfit and gradient equal its own single-problem checkpointed
   gradient (rel <= 1e-6: only the order of the per-segment image sums
-  differs), and the per-wave graph replays equal the eager run bit for bit;
+  differs), and the per-wave graph replays (and one graph of every wave) equal the
+  eager run bit for bit;
 * two shots of a 384^2 and a 768^2 problem (6- and 12-wavefront rows) run in
   one batch against the float64 CPU oracle: traces rel-L2 <= 1e-5, misfit rel
   <= 1e-5 or the bound the trace error implies (2 |dtraces| / |residual|, the
@@ -58,11 +59,15 @@ def test_workload_draw_in_waves_matches_single_problem_gradients():
         assert sum(sizes[i][1] for i in w) <= budget
     eager = sched.run()
     graphed = sched.run(graph=True)
+    assert sched.total_bytes() <= budget
+    whole = sched.run(graph="whole")  # every wave in one graph
     worst = 0.0
     for i, spec in enumerate(specs):
         g_e, J_e = eager[i]
         g_g, J_g = graphed[i]
         assert torch.equal(g_g, g_e) and J_g == J_e, i
+        g_w, J_w = whole[i]
+        assert torch.equal(g_w, g_e) and J_w == J_e, i
         p = problem_from(spec, data[i], DEV)
         eng = CheckpointedGradient(p.solver, p.sampler, p.S, p.N, checkpoint_every=p.K,
                                    reconstruct=p.reconstruct)
