@@ -74,20 +74,25 @@ int swno_gelu_backward(int64_t n, const float *pre, const float *dy, float *dx, 
  * columns needs its width and row stride to be multiples of 4 and a 16-byte
  * aligned base; column sums need N (N / 4 for such an operand) to divide 256.
  * Replaces, for the operator's backward pass, a split-K GEMM plus a column
- * sum over the same B (one pass over A and B instead of two over B). */
+ * sum over the same B (one pass over A and B instead of two over B).
+ * a_bf16 != 0: A holds bf16 codes (M a multiple of 8 above 16, lda a multiple
+ * of 8, 16-byte aligned); the products are the same, since A is rounded to
+ * bf16 for them anyway. */
 size_t swno_weight_grad_workspace_floats(int64_t rows, int32_t M, int32_t N);
-int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t lda,
+int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const void *A, int64_t lda,
                      const float *B, int64_t ldb, float *dW, float *db, float *workspace,
-                     void *stream);
+                     int32_t a_bf16, void *stream);
 
 /* Truncated transform along x over `slices` slices (one sample row each:
  * n pixels x ch channels, channel-last; ch = 32, n <= 128):
  * out[s] (rows_out x ch) = A X[s], with A (rows_out x n, rows_out <= 32)
  * given by element strides a_rs, a_cs (the forward transform Fx, or Gx^T in
  * the backward pass). bf16 MFMA with fp32 accumulation; replaces the batched
- * GEMM for this shape. */
+ * GEMM for this shape. X is float32, or bf16 codes when x_bf16 != 0 (the
+ * same products: X is rounded to bf16 for them anyway). */
 int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, const float *A,
-                     int64_t a_rs, int64_t a_cs, const float *X, float *out, void *stream);
+                     int64_t a_rs, int64_t a_cs, const void *X, float *out, int32_t x_bf16,
+                     void *stream);
 
 /* Tail of a spectral layer, fused, over `slices` slices (one sample row
  * each: n pixels x ch channels, channel-last; ch = 32, n a multiple of 16 up
@@ -96,10 +101,14 @@ int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, co
  * with Gx (n x 2 modes) the inverse x transform, Z (slices, 2 modes, ch) the
  * spectra after the y inverse transform, v the layer input, W (ch x ch).
  * Replaces the inverse-x GEMM, the pointwise GEMM (beta = 1) and bias + GELU.
- * Gx and v must be 16-byte aligned. */
+ * Gx and v must be 16-byte aligned. bf16_flags: bit 0, v holds bf16 codes
+ * (rounded to bf16 for the product anyway: the same results); bit 1, vnext
+ * is written as bf16 codes (16-byte aligned) -- activations stored at the
+ * precision their consumers use, half the bytes. pre stays float32. */
 int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *Gx,
-                       const float *Z, const float *v, const float *W, const float *bias,
-                       float *pre, float *vnext, int32_t gelu_act, void *stream);
+                       const float *Z, const void *v, const float *W, const float *bias,
+                       float *pre, void *vnext, int32_t gelu_act, int32_t bf16_flags,
+                       void *stream);
 
 /* Backward counterpart: dv = dpre W^T + Fx^T dT1, with Fx (2 modes x n) the
  * forward x transform and dT1 (slices, 2 modes, ch); out = dv * GELU'(pre_prev)
@@ -114,22 +123,24 @@ int swno_layer_backward(int32_t slices, int32_t n, int32_t modes, int32_t ch, co
  * out[r] = GELU(hpre[r]) . Q2 + bq2, for v (rows x ch, 16-byte aligned),
  * Q1 (ch x hidden), Q2 (hidden). v Q1 is a bf16 MFMA product (fp32
  * accumulation); bias, GELU and the dot with Q2 are fp32. Replaces a GEMM,
- * bias + GELU and a second GEMM over the hidden units. */
-int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const float *v, const float *Q1,
+ * bias + GELU and a second GEMM over the hidden units. v holds bf16 codes
+ * when v_bf16 != 0 (the same results). */
+int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const void *v, const float *Q1,
                       const float *bq1, const float *Q2, const float *bq2, float *hpre,
-                      float *out, void *stream);
+                      float *out, int32_t v_bf16, void *stream);
 
 /* Backward pass of swno_head_forward for d loss / d out = dout (rows):
  * dv = dh Q1^T with dh = dout Q2^T * GELU'(hpre), dQ1 = v^T dh,
  * dbq1 = sum_r dh, dQ2 = GELU(hpre)^T dout, dbq2 = sum_r dout. dv and dQ1
  * are bf16 MFMA products (fp32 accumulation); the parameter gradients are
  * summed in a fixed order through `workspace`
- * (swno_head_backward_workspace_floats floats). */
+ * (swno_head_backward_workspace_floats floats). v holds bf16 codes when
+ * v_bf16 != 0 (the same results). */
 size_t swno_head_backward_workspace_floats(int64_t rows);
 int swno_head_backward(int64_t rows, int32_t ch, int32_t hidden, const float *dout,
-                       const float *hpre, const float *v, const float *Q1, const float *Q2,
+                       const float *hpre, const void *v, const float *Q1, const float *Q2,
                        float *dv, float *dQ1, float *dbq1, float *dQ2, float *dbq2,
-                       float *workspace, void *stream);
+                       float *workspace, int32_t v_bf16, void *stream);
 
 /* out[c] = beta * out[c] + sum_r x[r, c], summed in a fixed order through
  * `workspace` (swno_col_sum_workspace_floats floats). */

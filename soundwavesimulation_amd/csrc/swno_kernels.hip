@@ -195,7 +195,7 @@ constexpr int kWgMaxParts = 1024;
 struct WgradP {
     int64_t rows, rows_per_part;
     int M, N;
-    const float *A;
+    const void *A;  // float, or bf16 codes (a_bf16)
     int64_t lda;
     const float *B;
     int64_t ldb;
@@ -203,13 +203,18 @@ struct WgradP {
     float *ws;         // [part][M N + N]
 };
 
-// One operand's share of a slab: item i of thread t is float4 (V) or scalar
-// number e = t + 256 i of the slab in row-major order. Its row and its offset
-// in the operand do not change from slab to slab.
+// One operand's share of a slab: item i of thread t is a float4 (V = 1), a
+// scalar (V = 0) or eight bf16 codes (V = 2) number e = t + 256 i of the slab
+// in row-major order. Its row and its offset in the operand do not change
+// from slab to slab.
 template <int CP, int V>
 struct WgOperand {
-    static constexpr int kItems = V ? CP / 16 : CP / 4;  // 64 rows x CP columns / 256 threads
-    typedef typename std::conditional<V, f4, float>::type item_t;
+    static constexpr int kVec = V == 2 ? 8 : (V ? 4 : 1);  // values per item
+    static constexpr int kItems = CP * kWgRows / 256 / kVec;
+    typedef typename std::conditional<V == 2, __bf16, float>::type elem_t;
+    typedef typename std::conditional<V == 2, bf16x8,
+                                      typename std::conditional<V == 1, f4, float>::type>::type
+        item_t;
     item_t reg[kItems];
     int row[kItems];  // slab row, or kWgRows when the item is past the slab
     int64_t off[kItems];
@@ -217,19 +222,19 @@ struct WgOperand {
 
     __device__ void init(int cols, int64_t ld, int tid)
     {
-        const int per_row = V ? cols >> 2 : cols;
+        const int per_row = cols / kVec;
 #pragma unroll
         for (int i = 0; i < kItems; ++i) {
             const int e = tid + 256 * i;
-            const int r = e / per_row, c = (e - r * per_row) * (V ? 4 : 1);
+            const int r = e / per_row, c = (e - r * per_row) * kVec;
             row[i] = e < kWgRows * per_row ? r : kWgRows;
             col[i] = c;
             off[i] = (int64_t)r * ld + c;
         }
     }
-    __device__ void load(const float *X, int64_t ld, int64_t r0, int64_t r_end)
+    __device__ void load(const void *Xv, int64_t ld, int64_t r0, int64_t r_end)
     {
-        const float *base = X + r0 * ld;
+        const elem_t *base = (const elem_t *)Xv + r0 * ld;
         // rows of this slab inside the range (items past the slab have row == kWgRows)
         const int64_t lim = min(r_end - r0, (int64_t)kWgRows);
 #pragma unroll
@@ -247,7 +252,7 @@ struct WgOperand {
             if (row[i] < kWgRows) {
                 if constexpr (V) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) T[col[i] + j][row[i]] = (__bf16)reg[i][j];
+                    for (int j = 0; j < kVec; ++j) T[col[i] + j][row[i]] = (__bf16)reg[i][j];
                 } else {
                     T[col[i]][row[i]] = (__bf16)reg[i];
                 }
@@ -420,6 +425,18 @@ __device__ __forceinline__ bf16x8 bf16_row8(const float *p, int valid)
     return r;
 }
 
+// 8 consecutive activations stored as bf16 (the rounding every MFMA consumer
+// applies to an fp32 activation, so storing them rounded changes no result),
+// or zeros past `valid` of them; p 16-byte aligned when valid >= 8
+__device__ __forceinline__ bf16x8 bf16_row8(const __bf16 *p, int valid)
+{
+    if (valid >= 8) return *(const bf16x8 *)p;
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = j < valid ? p[j] : (__bf16)0.0f;
+    return r;
+}
+
 // 8 floats p[0], p[stride], ..., or zeros from index `valid` on
 __device__ __forceinline__ bf16x8 bf16_col8(const float *p, int64_t stride, int valid)
 {
@@ -437,13 +454,17 @@ __device__ __forceinline__ bf16x8 bf16_col8(const float *p, int64_t stride, int 
 // is multiplied), rounded to bf16 into LDS transposed (channel-major, the row
 // index contiguous) in one of two buffers, so one barrier per slice suffices;
 // wavefront w then computes the output tile (w / 2, w % 2).
-constexpr int kXtItems = kLayerMaxN * kLayerC / 4 / 256;  // float4 per thread and slice
-
+// X is fp32 or bf16 (T); a thread loads 16 bytes per item (4 or 8 values).
+template <typename T>
 __global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, int n,
                                                           const float *A, int64_t a_rs,
-                                                          int64_t a_cs, const float *X,
+                                                          int64_t a_cs, const T *X,
                                                           float *out)
 {
+    constexpr bool kF32 = std::is_same<T, float>::value;
+    constexpr int kVec = kF32 ? 4 : 8;  // values per 16-byte item
+    constexpr int kXtItems = kLayerMaxN * kLayerC / kVec / 256;  // items per thread and slice
+    typedef typename std::conditional<kF32, f4, bf16x8>::type item_t;
     __shared__ __attribute__((aligned(16))) __bf16 Bs[2][kLayerC][kLayerMaxN + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, g = lane >> 4;
@@ -462,14 +483,14 @@ __global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, i
         }
     // rows k >= n of both buffers stay zero (they pad the last 32-row step)
     for (int i = tid; i < 2 * kLayerC * (kLayerMaxN + 8); i += 256) (&Bs[0][0][0])[i] = (__bf16)0.0f;
-    const int count = n * (kLayerC / 4);
-    f4 rx[kXtItems];
+    const int count = n * (kLayerC / kVec);
+    item_t rx[kXtItems];
     auto load = [&](int sl) {
-        const float *xs = X + (int64_t)sl * n * kLayerC;
+        const T *xs = X + (int64_t)sl * n * kLayerC;
 #pragma unroll
         for (int i = 0; i < kXtItems; ++i) {
             const int e4 = tid + 256 * i;
-            rx[i] = e4 < count ? *(const f4 *)(xs + 4 * e4) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+            rx[i] = e4 < count ? *(const item_t *)(xs + kVec * e4) : item_t{};
         }
     };
     int sl = blockIdx.x;
@@ -481,9 +502,9 @@ __global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, i
         for (int i = 0; i < kXtItems; ++i) {
             const int e4 = tid + 256 * i;
             if (e4 < count) {
-                const int k = e4 >> 3, c4 = (e4 & 7) * 4;
+                const int k = e4 / (kLayerC / kVec), c4 = (e4 % (kLayerC / kVec)) * kVec;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) Bs[buf][c4 + j][k] = (__bf16)rx[i][j];
+                for (int j = 0; j < kVec; ++j) Bs[buf][c4 + j][k] = (__bf16)rx[i][j];
             }
         }
         __syncthreads();
@@ -509,10 +530,12 @@ __global__ void __launch_bounds__(256) x_transform_kernel(int n_slices, int M, i
 // Forward tail of a layer: each wavefront's outputs (its 32 rows x 32
 // channels of pre and vnext) are staged in LDS and written with float4
 // stores, consecutive lanes at consecutive 16 bytes (measured faster than
-// stores from the MFMA output layout, which leaves 64-byte pieces).
+// stores from the MFMA output layout, which leaves 64-byte pieces). The
+// layer input v (TV) and output vnext (TN) are fp32 or bf16; pre is fp32.
+template <typename TV, typename TN>
 __global__ void __launch_bounds__(256) layer_forward_kernel(
-    int n_slices, int n, int k1, const float *Gx, const float *Z, const float *v,
-    const float *W, const float *bias, float *pre, float *vnext, int act)
+    int n_slices, int n, int k1, const float *Gx, const float *Z, const TV *v,
+    const float *W, const float *bias, float *pre, TN *vnext, int act)
 {
     constexpr int kStride = kLayerC + 4;  // floats per staged row (16-byte aligned rows)
     __shared__ __attribute__((aligned(16))) float stage[4][2][32][kStride];
@@ -536,7 +559,7 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(
     float (*sv)[kStride] = stage[wave][1];
     for (int sl = blockIdx.x; sl < n_slices; sl += gridDim.x) {
         const float *zs = Z + (int64_t)sl * k1 * kLayerC;
-        const float *vs = v + (int64_t)sl * n * kLayerC;
+        const TV *vs = v + (int64_t)sl * n * kLayerC;
         bf16x8 zb[2], va[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) zb[j] = bf16_col8(zs + (8 * g) * kLayerC + 16 * j + r16,
@@ -569,7 +592,7 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         float *ps = pre + (int64_t)sl * n * kLayerC;
-        float *ns = vnext + (int64_t)sl * n * kLayerC;
+        TN *ns = vnext + (int64_t)sl * n * kLayerC;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int rt = wave + 4 * i;
@@ -582,7 +605,21 @@ __global__ void __launch_bounds__(256) layer_forward_kernel(
                 const int rl = 16 * i + rr;
                 const int64_t o = (int64_t)(16 * rt + rr) * kLayerC + c4;
                 *(f4 *)(ps + o) = *(const f4 *)&sp[rl][c4];
-                *(f4 *)(ns + o) = *(const f4 *)&sv[rl][c4];
+                if constexpr (std::is_same<TN, float>::value)
+                    *(f4 *)(ns + o) = *(const f4 *)&sv[rl][c4];
+            }
+            if constexpr (!std::is_same<TN, float>::value) {
+                // 16 rows x 32 values in bf16 = 64 pieces of 16 bytes, one per lane
+                const int rr = lane >> 2, c8 = (lane & 3) * 8;
+                const int rl = 16 * i + rr;
+                const f4 x0 = *(const f4 *)&sv[rl][c8], x1 = *(const f4 *)&sv[rl][c8 + 4];
+                bf16x8 b;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    b[j] = (__bf16)x0[j];
+                    b[4 + j] = (__bf16)x1[j];
+                }
+                *(bf16x8 *)(ns + (int64_t)(16 * rt + rr) * kLayerC + c8) = b;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -681,7 +718,8 @@ __global__ void __launch_bounds__(256) layer_backward_kernel(int n_slices, int n
 constexpr int kHeadC = 32, kHeadH = 128;
 constexpr int kHeadTiles = kHeadH / 16;
 
-__global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const float *v,
+template <typename T>  // v: fp32 or bf16
+__global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const T *v,
                                                            const float *Q1, const float *bq1,
                                                            const float *Q2, const float *bq2,
                                                            float *hpre, float *out)
@@ -705,19 +743,7 @@ __global__ void __launch_bounds__(256) head_forward_kernel(int64_t rows, const f
     for (int64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
         const int64_t row0 = blk * 64 + wave * 16;
         const int64_t ra = row0 + r16;
-        bf16x8 a;
-        if (ra < rows) {
-            const f4 x0 = *(const f4 *)(v + ra * kHeadC + 8 * g);
-            const f4 x1 = *(const f4 *)(v + ra * kHeadC + 8 * g + 4);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                a[j] = (__bf16)x0[j];
-                a[4 + j] = (__bf16)x1[j];
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.0f;
-        }
+        const bf16x8 a = bf16_row8(v + (ra < rows ? ra : 0) * kHeadC + 8 * g, ra < rows ? 8 : 0);
         float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int t = 0; t < kHeadTiles; ++t) {
@@ -768,14 +794,16 @@ constexpr int kHeadJ = kHeadC * kHeadH + 2 * kHeadH + 1;
 // One 64-row block's inputs of head_backward_kernel, as one wavefront's lane
 // holds them: hpre at rows 16 w + 4 g + e and columns 16 t + r16, dout at
 // rows 16 w + 4 g + e, and two float4 of the block's v (for v^T in LDS).
+template <typename T>  // v: fp32 (two float4 per thread) or bf16 (eight values)
 struct HeadBlock {
     float hp[kHeadTiles][4];
     float d[4];
-    f4 vx[2];
+    typename std::conditional<std::is_same<T, float>::value, f4[2], bf16x8[1]>::type vx;
 };
 
-__device__ __forceinline__ void head_load(HeadBlock &b, int64_t blk0, int64_t r_end,
-                                          const float *dout, const float *hpre, const float *v,
+template <typename T>
+__device__ __forceinline__ void head_load(HeadBlock<T> &b, int64_t blk0, int64_t r_end,
+                                          const float *dout, const float *hpre, const T *v,
                                           int tid)
 {
     const int lane = tid & 63, wave = tid >> 6, r16 = lane & 15, g = lane >> 4;
@@ -788,19 +816,26 @@ __device__ __forceinline__ void head_load(HeadBlock &b, int64_t blk0, int64_t r_
         for (int t = 0; t < kHeadTiles; ++t)
             b.hp[t][e] = in ? hpre[(row0 + e) * kHeadH + 16 * t + r16] : 0.0f;
     }
+    if constexpr (std::is_same<T, float>::value) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int e4 = tid + 256 * i;
-        const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
-        b.vx[i] = blk0 + rr < r_end ? *(const f4 *)(v + (blk0 + rr) * kHeadC + c4)
-                                    : f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 2; ++i) {
+            const int e4 = tid + 256 * i;
+            const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
+            b.vx[i] = blk0 + rr < r_end ? *(const f4 *)(v + (blk0 + rr) * kHeadC + c4)
+                                        : f4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    } else {
+        const int rr = tid >> 2, c8 = (tid & 3) * 8;
+        b.vx[0] = blk0 + rr < r_end ? *(const bf16x8 *)(v + (blk0 + rr) * kHeadC + c8)
+                                    : bf16x8{};
     }
 }
 
 // two workgroups per CU: at most 256 VGPRs, LDS below half of the CU's
+template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 head_backward_kernel(int64_t rows, int64_t rows_per_part, const float *dout, const float *hpre,
-                     const float *v, const float *Q1, const float *Q2, float *dv, float *ws)
+                     const T *v, const float *Q1, const float *Q2, float *dv, float *ws)
 {
     constexpr int kDH = 64 * (kHeadH + 8) * 2, kDHT = kHeadH * (64 + 8) * 2;
     constexpr int kVT = kHeadC * (64 + 8) * 2, kQS = kHeadC * (kHeadH + 8) * 2;
@@ -829,7 +864,7 @@ head_backward_kernel(int64_t rows, int64_t rows_per_part, const float *dout, con
     float gb2 = 0.0f;
     const int64_t r_begin = (int64_t)blockIdx.x * rows_per_part;
     const int64_t r_end = min(rows, r_begin + rows_per_part);
-    HeadBlock cur, nxt;
+    HeadBlock<T> cur, nxt;
     head_load(cur, r_begin, r_end, dout, hpre, v, tid);
     for (int64_t blk0 = r_begin; blk0 < r_end; blk0 += 64) {
         // the next block's loads are in flight while this block is computed
@@ -851,12 +886,18 @@ head_backward_kernel(int64_t rows, int64_t rows_per_part, const float *dout, con
                 DHT[col][rl] = (__bf16)dh;
             }
         }
+        if constexpr (std::is_same<T, float>::value) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int e4 = tid + 256 * i;
-            const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
+            for (int i = 0; i < 2; ++i) {
+                const int e4 = tid + 256 * i;
+                const int rr = e4 >> 3, c4 = (e4 & 7) * 4;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) VT[c4 + j][rr] = (__bf16)cur.vx[i][j];
+                for (int j = 0; j < 4; ++j) VT[c4 + j][rr] = (__bf16)cur.vx[i][j];
+            }
+        } else {
+            const int rr = tid >> 2, c8 = (tid & 3) * 8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) VT[c8 + j][rr] = cur.vx[0][j];
         }
         __syncthreads();
         const int64_t row0 = blk0 + 16 * wave;
@@ -1178,10 +1219,19 @@ void launch_wgrad(dim3 grid, hipStream_t st, const WgradP &p)
     hipLaunchKernelGGL((wgrad_kernel<MP, NP, VA, VB>), grid, dim3(256), 0, st, p);
 }
 
+// va: 0 scalar, 1 float4, 2 bf16 codes (8 per load) for A
 template <int MP, int NP>
-void launch_wgrad_v(bool va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
+void launch_wgrad_v(int va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
 {
-    if (va) {
+    if (va == 2) {
+        if constexpr (MP >= 32) {
+            if (vb) {
+                launch_wgrad<MP, NP, 2, 1>(grid, st, p);
+            } else if constexpr (NP <= 32) {
+                launch_wgrad<MP, NP, 2, 0>(grid, st, p);
+            }
+        }
+    } else if (va) {
         if (vb) {
             launch_wgrad<MP, NP, 1, 1>(grid, st, p);
         } else if constexpr (NP <= 32) {
@@ -1197,7 +1247,7 @@ void launch_wgrad_v(bool va, bool vb, dim3 grid, hipStream_t st, const WgradP &p
 }
 
 template <int MP>
-void launch_wgrad_n(int np, bool va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
+void launch_wgrad_n(int np, int va, bool vb, dim3 grid, hipStream_t st, const WgradP &p)
 {
     switch (np) {
     case 16: launch_wgrad_v<MP, 16>(va, vb, grid, st, p); break;
@@ -1207,7 +1257,7 @@ void launch_wgrad_n(int np, bool va, bool vb, dim3 grid, hipStream_t st, const W
     }
 }
 
-void launch_wgrad_m(int mp, int np, bool va, bool vb, dim3 grid, hipStream_t st,
+void launch_wgrad_m(int mp, int np, int va, bool vb, dim3 grid, hipStream_t st,
                     const WgradP &p)
 {
     switch (mp) {
@@ -1315,9 +1365,9 @@ size_t swno_weight_grad_workspace_floats(int64_t rows, int32_t M, int32_t N)
     return (size_t)parts * (size_t)(M * N + N);
 }
 
-int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t lda,
+int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const void *A, int64_t lda,
                      const float *B, int64_t ldb, float *dW, float *db, float *workspace,
-                     void *stream)
+                     int32_t a_bf16, void *stream)
 {
     if (rows < 1 || M < 1 || N < 1 || M > 128 || N > 128)
         return fail(SWNO_EINVAL, "weight_grad: rows >= 1 and 1 <= M, N <= 128 required");
@@ -1334,6 +1384,10 @@ int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t
     p.B = B;
     p.ldb = ldb;
     const bool va = M % 4 == 0 && lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    if (a_bf16 && !(M % 8 == 0 && M > 16 && lda % 8 == 0 &&
+                    reinterpret_cast<uintptr_t>(A) % 16 == 0))
+        return fail(SWNO_EINVAL, "weight_grad: a bf16 A needs M a multiple of 8 above 16, lda a "
+                                 "multiple of 8 and a 16-byte aligned base");
     const bool vb = N % 4 == 0 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0;
     // scalar loads keep up to 8 values per thread: 64 rows x 32 columns
     if ((!va && M > 32) || (!vb && N > 32))
@@ -1352,7 +1406,7 @@ int swno_weight_grad(int64_t rows, int32_t M, int32_t N, const float *A, int64_t
     hipStream_t st = as_stream(stream);
     const int mp = M <= 16 ? 16 : M <= 32 ? 32 : M <= 64 ? 64 : 128;
     const int np = N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 128;
-    launch_wgrad_m(mp, np, va, vb, dim3((unsigned)n_part), st, p);
+    launch_wgrad_m(mp, np, a_bf16 ? 2 : (va ? 1 : 0), vb, dim3((unsigned)n_part), st, p);
     SWNO_HIP_CHECK(hipGetLastError());
     const int MN = M * N, J = MN + (p.bias ? N : 0);
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((J + 15) / 16)), dim3(256), 0, st,
@@ -1368,7 +1422,8 @@ bool layer_shape_ok(int32_t slices, int32_t n, int32_t modes, int32_t ch)
 }
 
 int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, const float *A,
-                     int64_t a_rs, int64_t a_cs, const float *X, float *out, void *stream)
+                     int64_t a_rs, int64_t a_cs, const void *X, float *out, int32_t x_bf16,
+                     void *stream)
 {
     if (slices < 1 || ch != kLayerC || rows_out < 1 || rows_out > 32 || n < 1 ||
         n > kLayerMaxN)
@@ -1380,15 +1435,22 @@ int swno_x_transform(int32_t slices, int32_t rows_out, int32_t n, int32_t ch, co
         return fail(SWNO_EINVAL, "x_transform: X must be 16-byte aligned");
     // a few slices per workgroup, so that the next slice's loads overlap the current one
     const unsigned grid = (unsigned)std::min(slices, 2048);
-    hipLaunchKernelGGL(x_transform_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
-                       rows_out, n, A, a_rs, a_cs, X, out);
+    if (x_bf16)
+        hipLaunchKernelGGL(x_transform_kernel<__bf16>, dim3(grid), dim3(256), 0,
+                           as_stream(stream), slices, rows_out, n, A, a_rs, a_cs,
+                           (const __bf16 *)X, out);
+    else
+        hipLaunchKernelGGL(x_transform_kernel<float>, dim3(grid), dim3(256), 0,
+                           as_stream(stream), slices, rows_out, n, A, a_rs, a_cs,
+                           (const float *)X, out);
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }
 
 int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, const float *Gx,
-                       const float *Z, const float *v, const float *W, const float *bias,
-                       float *pre, float *vnext, int32_t gelu_act, void *stream)
+                       const float *Z, const void *v, const float *W, const float *bias,
+                       float *pre, void *vnext, int32_t gelu_act, int32_t bf16_flags,
+                       void *stream)
 {
     if (!layer_shape_ok(slices, n, modes, ch))
         return fail(SWNO_EINVAL, "layer_forward: needs 32 channels, n a multiple of 16 up to "
@@ -1398,9 +1460,20 @@ int swno_layer_forward(int32_t slices, int32_t n, int32_t modes, int32_t ch, con
         return fail(SWNO_EINVAL, "layer_forward: operand is NULL");
     if (reinterpret_cast<uintptr_t>(v) % 16 != 0 || reinterpret_cast<uintptr_t>(Gx) % 16 != 0)
         return fail(SWNO_EINVAL, "layer_forward: v and Gx must be 16-byte aligned");
+    if ((bf16_flags & 2) != 0 && reinterpret_cast<uintptr_t>(vnext) % 16 != 0)
+        return fail(SWNO_EINVAL, "layer_forward: a bf16 vnext must be 16-byte aligned");
     const unsigned grid = (unsigned)std::min(slices, 8192);
-    hipLaunchKernelGGL(layer_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), slices,
-                       n, 2 * modes, Gx, Z, v, W, bias, pre, vnext, gelu_act);
+    hipStream_t st = as_stream(stream);
+#define SWNO_LAYER_FWD(TV, TN)                                                              \
+    hipLaunchKernelGGL((layer_forward_kernel<TV, TN>), dim3(grid), dim3(256), 0, st, slices, n, \
+                       2 * modes, Gx, Z, (const TV *)v, W, bias, pre, (TN *)vnext, gelu_act)
+    switch (bf16_flags & 3) {
+    case 0: SWNO_LAYER_FWD(float, float); break;
+    case 1: SWNO_LAYER_FWD(__bf16, float); break;
+    case 2: SWNO_LAYER_FWD(float, __bf16); break;
+    default: SWNO_LAYER_FWD(__bf16, __bf16); break;
+    }
+#undef SWNO_LAYER_FWD
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }
@@ -1435,9 +1508,9 @@ int head_parts(int64_t rows, int64_t *rows_per_part)
     return (int)((rows + *rows_per_part - 1) / *rows_per_part);
 }
 
-int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const float *v, const float *Q1,
+int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const void *v, const float *Q1,
                       const float *bq1, const float *Q2, const float *bq2, float *hpre,
-                      float *out, void *stream)
+                      float *out, int32_t v_bf16, void *stream)
 {
     if (ch != kHeadC || hidden != kHeadH)
         return fail(SWNO_EINVAL, "head_forward: built for 32 channels and 128 hidden units");
@@ -1448,8 +1521,14 @@ int swno_head_forward(int64_t rows, int32_t ch, int32_t hidden, const float *v, 
         return fail(SWNO_EINVAL, "head_forward: v must be 16-byte aligned");
     const int64_t blocks = (rows + 63) / 64;
     const unsigned grid = (unsigned)std::min<int64_t>(blocks, 4096);
-    hipLaunchKernelGGL(head_forward_kernel, dim3(grid), dim3(256), 0, as_stream(stream), rows,
-                       v, Q1, bq1, Q2, bq2, hpre, out);
+    if (v_bf16)
+        hipLaunchKernelGGL(head_forward_kernel<__bf16>, dim3(grid), dim3(256), 0,
+                           as_stream(stream), rows, (const __bf16 *)v, Q1, bq1, Q2, bq2, hpre,
+                           out);
+    else
+        hipLaunchKernelGGL(head_forward_kernel<float>, dim3(grid), dim3(256), 0,
+                           as_stream(stream), rows, (const float *)v, Q1, bq1, Q2, bq2, hpre,
+                           out);
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }
@@ -1462,9 +1541,9 @@ size_t swno_head_backward_workspace_floats(int64_t rows)
 }
 
 int swno_head_backward(int64_t rows, int32_t ch, int32_t hidden, const float *dout,
-                       const float *hpre, const float *v, const float *Q1, const float *Q2,
+                       const float *hpre, const void *v, const float *Q1, const float *Q2,
                        float *dv, float *dQ1, float *dbq1, float *dQ2, float *dbq2,
-                       float *workspace, void *stream)
+                       float *workspace, int32_t v_bf16, void *stream)
 {
     if (ch != kHeadC || hidden != kHeadH)
         return fail(SWNO_EINVAL, "head_backward: built for 32 channels and 128 hidden units");
@@ -1477,8 +1556,12 @@ int swno_head_backward(int64_t rows, int32_t ch, int32_t hidden, const float *do
     int64_t rpp = 0;
     const int n_part = head_parts(rows, &rpp);
     hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(head_backward_kernel, dim3((unsigned)n_part), dim3(256), 0, st, rows, rpp,
-                       dout, hpre, v, Q1, Q2, dv, workspace);
+    if (v_bf16)
+        hipLaunchKernelGGL(head_backward_kernel<__bf16>, dim3((unsigned)n_part), dim3(256), 0, st,
+                           rows, rpp, dout, hpre, (const __bf16 *)v, Q1, Q2, dv, workspace);
+    else
+        hipLaunchKernelGGL(head_backward_kernel<float>, dim3((unsigned)n_part), dim3(256), 0, st,
+                           rows, rpp, dout, hpre, (const float *)v, Q1, Q2, dv, workspace);
     SWNO_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(head_reduce_kernel, dim3((unsigned)((kHeadJ + 15) / 16)), dim3(256), 0,
                        st, n_part, (const float *)workspace, dQ1, dbq1, dQ2, dbq2);

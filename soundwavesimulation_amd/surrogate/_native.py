@@ -59,14 +59,16 @@ def load_library():
     lib.swno_col_sum.argtypes = [_i64, _i32, _vp, _f32, _vp, _vp, _vp]
     lib.swno_weight_grad_workspace_floats.restype = ctypes.c_size_t
     lib.swno_weight_grad_workspace_floats.argtypes = [_i64, _i32, _i32]
-    lib.swno_weight_grad.argtypes = [_i64, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp]
-    lib.swno_head_forward.argtypes = [_i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.swno_weight_grad.argtypes = [_i64, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                     _i32, _vp]
+    lib.swno_head_forward.argtypes = [_i64, _i32, _i32] + [_vp] * 7 + [_i32, _vp]
     lib.swno_head_backward_workspace_floats.restype = ctypes.c_size_t
     lib.swno_head_backward_workspace_floats.argtypes = [_i64]
-    lib.swno_head_backward.argtypes = [_i64, _i32, _i32] + [_vp] * 12
-    lib.swno_layer_forward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7 + [_i32, _vp]
+    lib.swno_head_backward.argtypes = [_i64, _i32, _i32] + [_vp] * 11 + [_i32, _vp]
+    lib.swno_layer_forward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7 + [_i32, _i32, _vp]
     lib.swno_layer_backward.argtypes = [_i32, _i32, _i32, _i32] + [_vp] * 7
-    lib.swno_x_transform.argtypes = [_i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp, _vp]
+    lib.swno_x_transform.argtypes = [_i32, _i32, _i32, _i32, _vp, _i64, _i64, _vp, _vp, _i32,
+                                     _vp]
     lib.swno_spectral_pack.argtypes = [_i32, _i32, _i32, _i32, _vp, _vp, _vp]
     lib.swno_mix_assemble.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp]
     lib.swno_mix_assemble_backward.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _vp]

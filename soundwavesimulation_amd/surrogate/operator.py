@@ -20,7 +20,10 @@ mixing a GEMM batched over modes with the real 2C x 2C form of each complex
 matrix, and the pointwise linears one GEMM over all pixels; the backward pass
 uses the same kernel on transposed (strided) views, with split-K for the
 weight gradients (reductions over every pixel of the batch). Activations are
-channel-last fp32 ``(B, N, N, C)``.
+channel-last ``(B, N, N, C)``: fp32, except the layer outputs v_1 .. v_L on the
+fused path, which are stored as bf16 (every consumer of a layer output rounds
+it to bf16 for an MFMA product, so the results are the same bits with half the
+bytes; ``SWNO_BF16_ACT=0`` keeps them fp32).
 
 ``SpectralOperator.forward`` keeps what ``backward`` needs; ``backward``
 returns the gradient of the loss for every parameter in one flat buffer laid
@@ -87,6 +90,10 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def _is_bf16(t):
+    return 1 if t.dtype == torch.bfloat16 else 0
+
+
 class SpectralOperator:
     """Device parameters, buffers and forward / backward of the operator."""
 
@@ -142,9 +149,15 @@ class SpectralOperator:
         n, C, m, H, L = self.n, self.C, self.m, self.H, self.L
         rows = B * n * n
         e = lambda *s: torch.empty(s, device=self.device, dtype=_F32)  # noqa: E731
+        # layer outputs in bf16 where only the fused kernels (which round them to bf16
+        # for their products anyway) read them
+        bf16_act = (self.fused_layers and self.fused_head
+                    and os.environ.get("SWNO_BF16_ACT", "1") != "0")
         b = {
             "inp": e(rows, self.Cin),
-            "v": [e(rows, C) for _ in range(L + 1)],
+            "v": [e(rows, C)] + [torch.empty((rows, C), device=self.device,
+                                             dtype=torch.bfloat16 if bf16_act else _F32)
+                                 for _ in range(L)],
             "pre": [e(rows, C) for _ in range(L)],
             "hpre": e(rows, H), "h": e(rows, H), "out": e(rows, 1), "outpre": e(rows, 1),
             "T1": e(B * n * 2 * m * C), "P": e(B * 4 * m * m * C),
@@ -173,7 +186,8 @@ class SpectralOperator:
     def weight_grad(self, rows, A, M, B, N, dW, db, ws):
         """dW (M x N) = A^T B and db = column sums of B over ``rows`` rows."""
         _native.check(self.lib.swno_weight_grad(rows, M, N, _ptr(A), M, _ptr(B), N, _ptr(dW),
-                                                _ptr(db), _ptr(ws), self._stream()))
+                                                _ptr(db), _ptr(ws), _is_bf16(A),
+                                                self._stream()))
 
     # ------------------------------------------------------------ spectral
     def _spectral(self, l, v, out, bufs, B):
@@ -184,7 +198,7 @@ class SpectralOperator:
         T1, P, X, Y, U, Z = bufs["T1"], bufs["P"], bufs["X"][l], bufs["Y"], bufs["U"], bufs["Z"]
         if self.fused_layers:
             _native.check(self.lib.swno_x_transform(B * n, 2 * m, n, C, _ptr(self.Fx), n, 1,
-                                                    _ptr(v), _ptr(T1), st))
+                                                    _ptr(v), _ptr(T1), _is_bf16(v), st))
         else:
             self.gemm(2 * m, C, n, B * n, self.Fx, (n, 1, 0), v, (C, 1, n * C), T1,
                       (C, 1, 2 * m * C))
@@ -210,7 +224,7 @@ class SpectralOperator:
         # dZ[b,y] = Gx^T dpre[b,y]
         if self.fused_layers:
             _native.check(self.lib.swno_x_transform(B * n, 2 * m, n, C, _ptr(self.Gx), 1, 2 * m,
-                                                    _ptr(dpre), _ptr(Z), st))
+                                                    _ptr(dpre), _ptr(Z), 0, st))
         else:
             self.gemm(2 * m, C, n, B * n, self.Gx, (1, 2 * m, 0), dpre, (C, 1, n * C), Z,
                       (C, 1, 2 * m * C))
@@ -264,7 +278,7 @@ class SpectralOperator:
                 _native.check(lib.swno_layer_forward(
                     B * n, n, self.m, C, _ptr(self.Gx), _ptr(bufs["Z"]), _ptr(v[l]),
                     _ptr(self.p[f"W{l}"]), _ptr(self.p[f"b{l}"]), _ptr(pre), _ptr(v[l + 1]),
-                    int(l < self.L - 1), st))
+                    int(l < self.L - 1), _is_bf16(v[l]) | (_is_bf16(v[l + 1]) << 1), st))
                 continue
             self._spectral(l, v[l], pre, bufs, B)
             self.gemm(rows, C, C, 1, v[l], (C, 1, 0), self.p[f"W{l}"], (C, 1, 0), pre, (C, 1, 0),
@@ -275,7 +289,7 @@ class SpectralOperator:
             _native.check(lib.swno_head_forward(
                 rows, C, H, _ptr(v[self.L]), _ptr(self.p["Q1"]), _ptr(self.p["bq1"]),
                 _ptr(self.p["Q2"]), _ptr(self.p["bq2"]), _ptr(bufs["hpre"]), _ptr(bufs["out"]),
-                st))
+                _is_bf16(v[self.L]), st))
         else:
             self.gemm(rows, H, C, 1, v[self.L], (C, 1, 0), self.p["Q1"], (H, 1, 0),
                       bufs["hpre"], (H, 1, 0))
@@ -320,7 +334,8 @@ class SpectralOperator:
             _native.check(lib.swno_head_backward(
                 rows, C, H, _ptr(dout), _ptr(bufs["hpre"]), _ptr(bufs["v"][self.L]),
                 _ptr(self.p["Q1"]), _ptr(self.p["Q2"]), _ptr(dv), _ptr(g["Q1"]), _ptr(g["bq1"]),
-                _ptr(g["Q2"]), _ptr(g["bq2"]), _ptr(bufs["headws"]), st))
+                _ptr(g["Q2"]), _ptr(g["bq2"]), _ptr(bufs["headws"]), _is_bf16(bufs["v"][self.L]),
+                st))
         else:
             self.weight_grad(rows, bufs["h"], H, dout, 1, g["Q2"], g["bq2"], wgws)
             self.gemm(rows, H, 1, 1, dout, (1, 1, 0), self.p["Q2"], (1, 1, 0), bufs["dh"],

@@ -85,7 +85,7 @@ def test_weight_grad_matches_bf16_rounded_product(rows, M, N, bias, offset):
     ws = torch.empty(int(lib.swno_weight_grad_workspace_floats(rows, M, N)), device=DEV)
     st = torch.cuda.current_stream().cuda_stream
     args = (rows, M, N, A.data_ptr(), M, B.data_ptr(), N, dW.data_ptr(),
-            db.data_ptr() if bias else None, ws.data_ptr(), st)
+            db.data_ptr() if bias else None, ws.data_ptr(), 0, st)
     _native.check(lib.swno_weight_grad(*args))
     first = (dW.clone(), db.clone())
     _native.check(lib.swno_weight_grad(*args))  # fixed summation order: bitwise repeatable
@@ -115,7 +115,7 @@ def test_x_transform_matches_bf16_rounded_product(slices, M, n, transposed):
     out = torch.full((slices, M, 32), float("nan"), device=DEV)
     a_rs, a_cs = (1, M) if transposed else (n, 1)
     _native.check(lib.swno_x_transform(slices, M, n, 32, A.data_ptr(), a_rs, a_cs, X.data_ptr(),
-                                       out.data_ptr(), torch.cuda.current_stream().cuda_stream))
+                                       out.data_ptr(), 0, torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     Am = A.T if transposed else A
     ref = torch.matmul(_bf16(Am.cpu()), _bf16(X.cpu()))
@@ -140,7 +140,7 @@ def test_head_kernels_match_float64(rows):
     out = torch.empty(rows, device=DEV)
     _native.check(lib.swno_head_forward(rows, C, H, v.data_ptr(), Q1.data_ptr(), bq1.data_ptr(),
                                         Q2.data_ptr(), bq2.data_ptr(), hpre.data_ptr(),
-                                        out.data_ptr(), st))
+                                        out.data_ptr(), 0, st))
     dv = torch.empty((rows, C), device=DEV)
     dQ1, dbq1 = torch.empty((C, H), device=DEV), torch.empty(H, device=DEV)
     dQ2, dbq2 = torch.empty(H, device=DEV), torch.empty(1, device=DEV)
@@ -148,7 +148,7 @@ def test_head_kernels_match_float64(rows):
     _native.check(lib.swno_head_backward(rows, C, H, dout.data_ptr(), hpre.data_ptr(),
                                          v.data_ptr(), Q1.data_ptr(), Q2.data_ptr(),
                                          dv.data_ptr(), dQ1.data_ptr(), dbq1.data_ptr(),
-                                         dQ2.data_ptr(), dbq2.data_ptr(), ws.data_ptr(), st))
+                                         dQ2.data_ptr(), dbq2.data_ptr(), ws.data_ptr(), 0, st))
     torch.cuda.synchronize()
     Q264 = Q2.cpu().double()
     hp64 = _bf16(v.cpu()) @ _bf16(Q1.cpu()) + bq1.cpu().double()
@@ -172,10 +172,10 @@ def test_weight_grad_rejects_bad_shapes():
     x = torch.zeros(1024, device=DEV)
     st = torch.cuda.current_stream().cuda_stream
     p = x.data_ptr()
-    assert lib.swno_weight_grad(16, 129, 4, p, 129, p, 4, p, None, p, st) != 0  # M > 128
-    assert lib.swno_weight_grad(16, 33, 4, p + 4, 36, p, 4, p, None, p, st) != 0  # wide scalar
-    assert lib.swno_weight_grad(16, 4, 12, p, 4, p, 12, p, p, p, st) != 0  # 12 / 4 ∤ 256
-    assert lib.swno_weight_grad(16, 4, 4, p, 2, p, 4, p, None, p, st) != 0  # lda < M
+    assert lib.swno_weight_grad(16, 129, 4, p, 129, p, 4, p, None, p, 0, st) != 0  # M > 128
+    assert lib.swno_weight_grad(16, 33, 4, p + 4, 36, p, 4, p, None, p, 0, st) != 0  # wide scalar
+    assert lib.swno_weight_grad(16, 4, 12, p, 4, p, 12, p, p, p, 0, st) != 0  # 12 / 4 ∤ 256
+    assert lib.swno_weight_grad(16, 4, 4, p, 2, p, 4, p, None, p, 0, st) != 0  # lda < M
 
 
 def test_elementwise_kernels():
@@ -354,3 +354,89 @@ def test_batched_first_gradients_equal_one_at_a_time():
         err = float((gb[k] - g1).norm() / g1.norm())
         print(f"  phantom {k}: rel {err:.1e}")
         assert err <= 1e-6
+
+
+@pytest.mark.parametrize("B", [5, 300])  # 300: 9600 slices, several per workgroup
+def test_bf16_layer_outputs_give_the_same_bits(B, monkeypatch):
+    """The fused path stores the layer outputs v_1 .. v_L as bf16: every kernel
+    that reads them rounds them to bf16 for an MFMA product, so the output and
+    every gradient must equal those of fp32 storage (SWNO_BF16_ACT=0) bit for
+    bit."""
+    n, m, C, H = 32, 8, 32, 128
+    print(f"bf16 vs fp32 layer outputs, batch {B}", flush=True)
+    rng = np.random.default_rng(5)
+    params = init_params(seed=6, width=C, modes=m, hidden=H)
+    for l in range(4):
+        params[f"b{l}"] = rng.standard_normal(C) * 0.1
+    inp = torch.as_tensor(rng.standard_normal((B, n, n, 3)), dtype=torch.float32, device=DEV)
+    target = torch.as_tensor(rng.standard_normal((B, n, n)), dtype=torch.float32, device=DEV)
+    res = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SWNO_BF16_ACT", flag)
+        op = SpectralOperator(params, n, DEV, width=C, modes=m, hidden=H)
+        out = op.forward(inp).clone()
+        loss = float(op.loss_and_backward(target))
+        torch.cuda.synchronize()
+        dt = op._buffers(B)["v"][1].dtype
+        assert dt == (torch.bfloat16 if flag == "1" else torch.float32)
+        res[flag] = (out, loss, op.gradients())
+        del op
+    assert torch.equal(res["0"][0], res["1"][0]) and res["0"][1] == res["1"][1]
+    for k in params:
+        assert np.array_equal(res["0"][2][k], res["1"][2][k]), k
+
+
+def test_kernels_read_bf16_activations_like_their_rounded_fp32():
+    """x_transform, the head kernels and weight_grad given bf16 activations
+    equal the same kernels given the fp32 values of those codes, bit for bit
+    (partial 64-row blocks and several slices per workgroup)."""
+    print("bf16 activation inputs of the fused kernels", flush=True)
+    lib = _native.load_library()
+    st = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator().manual_seed(17)
+    # x transform
+    slices, M, n = 5000, 32, 32
+    A = torch.randn((M, n), generator=g).to(DEV)
+    Xb = torch.randn((slices, n, 32), generator=g).to(DEV).to(torch.bfloat16)
+    Xf = Xb.float()
+    outs = []
+    for X, flag in ((Xf, 0), (Xb, 1)):
+        o = torch.full((slices, M, 32), float("nan"), device=DEV)
+        _native.check(lib.swno_x_transform(slices, M, n, 32, A.data_ptr(), n, 1, X.data_ptr(),
+                                           o.data_ptr(), flag, st))
+        outs.append(o)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    # head forward / backward and the weight gradient on v
+    rows, C, H = 70001, 32, 128
+    vb = torch.randn((rows, C), generator=g).to(DEV).to(torch.bfloat16)
+    Q1 = (torch.randn((C, H), generator=g) * 0.3).to(DEV)
+    bq1 = (torch.randn(H, generator=g) * 0.3).to(DEV)
+    Q2 = (torch.randn(H, generator=g) * 0.2).to(DEV)
+    bq2 = torch.randn(1, generator=g).to(DEV)
+    dout = torch.randn(rows, generator=g).to(DEV)
+    dpre = torch.randn((rows, C), generator=g).to(DEV)
+    res = []
+    for v, flag in ((vb.float(), 0), (vb, 1)):
+        hpre = torch.empty((rows, H), device=DEV)
+        out = torch.empty(rows, device=DEV)
+        _native.check(lib.swno_head_forward(rows, C, H, v.data_ptr(), Q1.data_ptr(),
+                                            bq1.data_ptr(), Q2.data_ptr(), bq2.data_ptr(),
+                                            hpre.data_ptr(), out.data_ptr(), flag, st))
+        dv = torch.empty((rows, C), device=DEV)
+        grads = [torch.empty((C, H), device=DEV), torch.empty(H, device=DEV),
+                 torch.empty(H, device=DEV), torch.empty(1, device=DEV)]
+        ws = torch.empty(int(lib.swno_head_backward_workspace_floats(rows)), device=DEV)
+        _native.check(lib.swno_head_backward(rows, C, H, dout.data_ptr(), hpre.data_ptr(),
+                                             v.data_ptr(), Q1.data_ptr(), Q2.data_ptr(),
+                                             dv.data_ptr(), *[t.data_ptr() for t in grads],
+                                             ws.data_ptr(), flag, st))
+        dW, db = torch.empty((C, C), device=DEV), torch.empty(C, device=DEV)
+        wws = torch.empty(int(lib.swno_weight_grad_workspace_floats(rows, C, C)), device=DEV)
+        _native.check(lib.swno_weight_grad(rows, C, C, v.data_ptr(), C, dpre.data_ptr(), C,
+                                           dW.data_ptr(), db.data_ptr(), wws.data_ptr(), flag,
+                                           st))
+        res.append([hpre, out, dv, *grads, dW, db])
+    torch.cuda.synchronize()
+    for a, b in zip(*res):
+        assert torch.equal(a, b)
