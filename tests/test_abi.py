@@ -30,6 +30,23 @@ def test_host_only_entry_points():
     assert lib.swfd_residual_workspace_floats(130, 3, 65) == 3 * 2 * 3
 
 
+def test_adjoint_workspace_matches_the_host_memory_model():
+    """The imaging workspace holds one partial image per shot group: at most
+    n_shots and at least 8, up to 32 at 1024^2 and up to 2^25 / (ny nx) on
+    smaller grids; batch.max_wide_groups restates the rule for the memory
+    model of the uneven batch."""
+    from soundwavesimulation_amd.batch import max_wide_groups
+
+    lib = _native.load_library()
+    for ny, nx in ((41, 41), (256, 256), (384, 384), (512, 512), (768, 768), (1024, 1024),
+                   (1023, 700), (2048, 2048)):
+        for S in (1, 8, 20, 37, 128, 256, 1000):
+            groups = max(8, min(max_wide_groups(ny, nx), S))
+            assert lib.swfd_adjoint_workspace_floats(ny, nx, S) == groups * ny * nx, (ny, nx, S)
+    assert max_wide_groups(1024, 1024) == 32 and max_wide_groups(256, 256) == 512
+    assert max_wide_groups(512, 512) == 128 and max_wide_groups(2048, 2048) == 32
+
+
 def test_operator_library_exports_every_declared_function():
     from soundwavesimulation_amd.surrogate import _native as no_native
 
