@@ -57,9 +57,10 @@ int swno_gemm(const swno_gemm_args *g, float *workspace, void *stream);
 
 /* pre[r, c] = x[r, c] + bias[c] (bias may be NULL); y = GELU(pre) when
  * gelu != 0, else y = pre. GELU is the erf form 0.5 x (1 + erf(x / sqrt 2)).
- * pre may equal x; y may equal pre only when gelu == 0. */
+ * y is float32, or bf16 codes (rounded to nearest even) when y_bf16 != 0.
+ * pre may equal x; y may equal pre only when gelu == 0 and y is float32. */
 int swno_bias_act(int64_t rows, int32_t cols, const float *x, const float *bias, float *pre,
-                  float *y, int32_t gelu, void *stream);
+                  void *y, int32_t gelu, int32_t y_bf16, void *stream);
 
 /* dx[i] = dy[i] * GELU'(pre[i]) (dx may equal dy). */
 int swno_gelu_backward(int64_t n, const float *pre, const float *dy, float *dx, void *stream);

@@ -991,8 +991,9 @@ __global__ void __launch_bounds__(256) head_reduce_kernel(int n_part, const floa
     }
 }
 
+template <typename TY>  // y: fp32 or bf16 codes
 __global__ void __launch_bounds__(256) bias_act_kernel(int64_t rows, int cols, const float *x,
-                                                       const float *bias, float *pre, float *y,
+                                                       const float *bias, float *pre, TY *y,
                                                        int use_gelu)
 {
     const int64_t n = rows * cols;
@@ -1001,7 +1002,7 @@ __global__ void __launch_bounds__(256) bias_act_kernel(int64_t rows, int cols, c
         const int c = (int)(i % cols);
         const float v = x[i] + (bias != nullptr ? bias[c] : 0.0f);
         pre[i] = v;
-        y[i] = use_gelu ? gelu(v) : v;
+        y[i] = (TY)(use_gelu ? gelu(v) : v);
     }
 }
 
@@ -1336,13 +1337,18 @@ int swno_gemm(const swno_gemm_args *g, float *workspace, void *stream)
 }
 
 int swno_bias_act(int64_t rows, int32_t cols, const float *x, const float *bias, float *pre,
-                  float *y, int32_t use_gelu, void *stream)
+                  void *y, int32_t use_gelu, int32_t y_bf16, void *stream)
 {
     if (rows < 1 || cols < 1 || x == nullptr || pre == nullptr || y == nullptr)
         return fail(SWNO_EINVAL, "bias_act: bad arguments");
-    if (use_gelu && y == pre) return fail(SWNO_EINVAL, "bias_act: y must differ from pre");
-    hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(rows * cols)), dim3(256), 0,
-                       as_stream(stream), rows, cols, x, bias, pre, y, use_gelu);
+    if ((use_gelu || y_bf16) && y == pre)
+        return fail(SWNO_EINVAL, "bias_act: y must differ from pre");
+    if (y_bf16)
+        hipLaunchKernelGGL(bias_act_kernel<__bf16>, dim3(blocks_for(rows * cols)), dim3(256), 0,
+                           as_stream(stream), rows, cols, x, bias, pre, (__bf16 *)y, use_gelu);
+    else
+        hipLaunchKernelGGL(bias_act_kernel<float>, dim3(blocks_for(rows * cols)), dim3(256), 0,
+                           as_stream(stream), rows, cols, x, bias, pre, (float *)y, use_gelu);
     SWNO_HIP_CHECK(hipGetLastError());
     return SWNO_OK;
 }

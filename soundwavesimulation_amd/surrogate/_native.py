@@ -52,7 +52,7 @@ def load_library():
     lib.swno_gemm_workspace_floats.restype = ctypes.c_size_t
     lib.swno_gemm_workspace_floats.argtypes = [ctypes.POINTER(GemmArgs)]
     lib.swno_gemm.argtypes = [ctypes.POINTER(GemmArgs), _vp, _vp]
-    lib.swno_bias_act.argtypes = [_i64, _i32, _vp, _vp, _vp, _vp, _i32, _vp]
+    lib.swno_bias_act.argtypes = [_i64, _i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp]
     lib.swno_gelu_backward.argtypes = [_i64, _vp, _vp, _vp, _vp]
     lib.swno_col_sum_workspace_floats.restype = ctypes.c_size_t
     lib.swno_col_sum_workspace_floats.argtypes = [_i64, _i32]

@@ -20,10 +20,10 @@ mixing a GEMM batched over modes with the real 2C x 2C form of each complex
 matrix, and the pointwise linears one GEMM over all pixels; the backward pass
 uses the same kernel on transposed (strided) views, with split-K for the
 weight gradients (reductions over every pixel of the batch). Activations are
-channel-last ``(B, N, N, C)``: fp32, except the layer outputs v_1 .. v_L on the
-fused path, which are stored as bf16 (every consumer of a layer output rounds
-it to bf16 for an MFMA product, so the results are the same bits with half the
-bytes; ``SWNO_BF16_ACT=0`` keeps them fp32).
+channel-last ``(B, N, N, C)``: fp32, except, on the fused path, the lifting's
+output v_0 and the layer outputs v_1 .. v_L, which are stored as bf16 (every
+consumer rounds them to bf16 for an MFMA product, so the results are the same
+bits with half the bytes; ``SWNO_BF16_ACT=0`` keeps them fp32).
 
 ``SpectralOperator.forward`` keeps what ``backward`` needs; ``backward``
 returns the gradient of the loss for every parameter in one flat buffer laid
@@ -155,9 +155,9 @@ class SpectralOperator:
                     and os.environ.get("SWNO_BF16_ACT", "1") != "0")
         b = {
             "inp": e(rows, self.Cin),
-            "v": [e(rows, C)] + [torch.empty((rows, C), device=self.device,
-                                             dtype=torch.bfloat16 if bf16_act else _F32)
-                                 for _ in range(L)],
+            "v": [torch.empty((rows, C), device=self.device,
+                              dtype=torch.bfloat16 if bf16_act else _F32)
+                  for _ in range(L + 1)],
             "pre": [e(rows, C) for _ in range(L)],
             "hpre": e(rows, H), "h": e(rows, H), "out": e(rows, 1), "outpre": e(rows, 1),
             "T1": e(B * n * 2 * m * C), "P": e(B * 4 * m * m * C),
@@ -267,10 +267,13 @@ class SpectralOperator:
             _native.check(lib.swno_mix_assemble(self.m * self.m, C, _ptr(self.p[f"R{l}_re"]),
                                                 _ptr(self.p[f"R{l}_im"]), _ptr(self.rm[l]), st))
         v = bufs["v"]
+        # lifting: into v[0] itself, or (bf16 layer outputs) through the fp32 buffer of
+        # layer 0's pre-activation, which layer 0 overwrites later
+        lift = bufs["pre"][0] if _is_bf16(v[0]) else v[0]
         self.gemm(rows, C, self.Cin, 1, bufs["inp"], (self.Cin, 1, 0), self.p["P"], (C, 1, 0),
-                  v[0], (C, 1, 0))
-        _native.check(lib.swno_bias_act(rows, C, _ptr(v[0]), _ptr(self.p["bp"]), _ptr(v[0]),
-                                        _ptr(v[0]), 0, st))
+                  lift, (C, 1, 0))
+        _native.check(lib.swno_bias_act(rows, C, _ptr(lift), _ptr(self.p["bp"]), _ptr(lift),
+                                        _ptr(v[0]), 0, _is_bf16(v[0]), st))
         for l in range(self.L):
             pre = bufs["pre"][l]
             if self.fused_layers:
@@ -284,7 +287,7 @@ class SpectralOperator:
             self.gemm(rows, C, C, 1, v[l], (C, 1, 0), self.p[f"W{l}"], (C, 1, 0), pre, (C, 1, 0),
                       beta=1.0)
             _native.check(lib.swno_bias_act(rows, C, _ptr(pre), _ptr(self.p[f"b{l}"]), _ptr(pre),
-                                            _ptr(v[l + 1]), int(l < self.L - 1), st))
+                                            _ptr(v[l + 1]), int(l < self.L - 1), 0, st))
         if self.fused_head:
             _native.check(lib.swno_head_forward(
                 rows, C, H, _ptr(v[self.L]), _ptr(self.p["Q1"]), _ptr(self.p["bq1"]),
@@ -294,11 +297,11 @@ class SpectralOperator:
             self.gemm(rows, H, C, 1, v[self.L], (C, 1, 0), self.p["Q1"], (H, 1, 0),
                       bufs["hpre"], (H, 1, 0))
             _native.check(lib.swno_bias_act(rows, H, _ptr(bufs["hpre"]), _ptr(self.p["bq1"]),
-                                            _ptr(bufs["hpre"]), _ptr(bufs["h"]), 1, st))
+                                            _ptr(bufs["hpre"]), _ptr(bufs["h"]), 1, 0, st))
             self.gemm(rows, 1, H, 1, bufs["h"], (H, 1, 0), self.p["Q2"], (1, 1, 0),
                       bufs["outpre"], (1, 1, 0))
             _native.check(lib.swno_bias_act(rows, 1, _ptr(bufs["outpre"]), _ptr(self.p["bq2"]),
-                                            _ptr(bufs["out"]), _ptr(bufs["out"]), 0, st))
+                                            _ptr(bufs["out"]), _ptr(bufs["out"]), 0, 0, st))
         self._last_B = B
         return bufs["out"].view(B, n, n)
 

@@ -188,7 +188,7 @@ def test_elementwise_kernels():
     pre = torch.empty_like(x)
     y = torch.empty_like(x)
     _native.check(lib.swno_bias_act(rows, cols, x.data_ptr(), bias.data_ptr(), pre.data_ptr(),
-                                    y.data_ptr(), 1, st))
+                                    y.data_ptr(), 1, 0, st))
     dy = torch.randn((rows, cols), generator=g).to(DEV)
     dx = torch.empty_like(dy)
     _native.check(lib.swno_gelu_backward(rows * cols, pre.data_ptr(), dy.data_ptr(),
