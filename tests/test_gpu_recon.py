@@ -28,7 +28,7 @@ import numpy as np
 import pytest
 import torch
 
-from soundwavesimulation_amd import RingArray, axis_centers, ricker, stable_dt
+from soundwavesimulation_amd import RingArray, _native, axis_centers, ricker, stable_dt
 from soundwavesimulation_amd.inversion import interior_mask
 from soundwavesimulation_amd.inversion.fwi import (CheckpointedGradient, TorchLeastSquaresFWI,
                                                    residual_and_misfit)
@@ -282,3 +282,21 @@ def test_reference_problem_gradient_with_reconstruction():
           f"float64 {e64:.2e}, vs recomputation {e_re:.2e}")
     assert J == out[False][0]
     assert eJ <= 1e-5 and e64 <= 5e-4 and e_re <= 2e-5
+
+
+@pytest.mark.parametrize("ny,S,rows,groups", [
+    (256, 128, 32, 128), (384, 128, 32, 64), (512, 128, 32, 64), (768, 128, 32, 32),
+    (1024, 256, 32, 32), (1024, 128, 32, 32),
+])
+def test_reconstructing_adjoint_shapes_of_the_measured_sweep(ny, S, rows, groups):
+    """The reconstructing adjoint's launch shape at the sizes of the shape sweep
+    (every combination of 8/16/32-row segments and 8-128 shot groups timed on
+    the device) is the fastest one found there; the recomputing adjoint keeps
+    at most 32 groups."""
+    if torch.cuda.get_device_properties(0).multi_processor_count != 256:
+        pytest.skip("the shape rule's figures are for a 256-CU MI355X")
+    shape = _native.adjoint_launch_shape(ny, ny, S, True)
+    print(f"adjoint shape {ny}^2 x {S}: {shape}", flush=True)
+    assert shape["full_width"] == 1 and shape["steps_per_launch"] == 4
+    assert (shape["segment_rows"], shape["shot_groups"]) == (rows, groups)
+    assert _native.adjoint_launch_shape(ny, ny, S, False)["shot_groups"] <= 32
