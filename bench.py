@@ -302,15 +302,23 @@ def cpu_baseline(a, c_true, dt, h):
 
 
 def cpu_model():
-    """The host CPU's model name (``/proc/cpuinfo``)."""
+    """The host CPU's model name, with its family / model / stepping numbers
+    (``/proc/cpuinfo``; a virtual machine may report only a generic name)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if info:
+                        break  # the first processor's block is enough
+                    continue
+                key, value = (t.strip() for t in line.split(":", 1))
+                info.setdefault(key, value)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 # The reference's batched op sequence (fdtd2d/torch_backend.py:477-598), restated
