@@ -3176,9 +3176,18 @@ __device__ __forceinline__ void bnd_extract_cell(const float *field, int S, int 
     slot[i] = field[s * H * W + (size_t)row * W + col];
 }
 
-__global__ void bnd_extract_kernel(const float *field, int S, int H, int W, float *slot)
+// The fields of a launch's levels and their ring slots, for one extraction
+// launch over all of them (blockIdx.y = level).
+struct BndLevels {
+    const float *field[kMaxLevels];
+    float *slot[kMaxLevels];
+};
+
+__global__ void __launch_bounds__(256) bnd_extract_kernel(BndLevels lv, int S, int H, int W)
 {
-    bnd_extract_cell(field, S, H, W, slot, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = (int)blockIdx.y;
+    bnd_extract_cell(lv.field[j], S, H, W, lv.slot[j],
+                     (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Batched ring extraction: entry e copies the boundary cells of its n_shots
@@ -4568,16 +4577,14 @@ struct ForwardRun {
     }
     int launch_boundary(const StepParams &p, hipStream_t st) const
     {
-        const float *fields[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
-        float *slots[kMaxLevels] = {nullptr, nullptr, nullptr, nullptr};
-        const int L = boundary_copies(p, fields, slots);
-        if (L < 0) return -L;
+        BndLevels lv = {};
+        const int L = boundary_copies(p, lv.field, lv.slot);
+        if (L <= 0) return -L;
         const size_t total = (size_t)a->n_shots * bnd_floats(grid->ny, grid->nx);
-        for (int j = 0; j < L; ++j) {
-            hipLaunchKernelGGL(bnd_extract_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                               0, st, fields[j], a->n_shots, grid->ny, grid->nx, slots[j]);
-            SWFD_HIP_CHECK(hipGetLastError());
-        }
+        // every level in one launch: the levels' strided column reads overlap
+        hipLaunchKernelGGL(bnd_extract_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)L),
+                           dim3(256), 0, st, lv, a->n_shots, grid->ny, grid->nx);
+        SWFD_HIP_CHECK(hipGetLastError());
         return SWFD_OK;
     }
     // the ring copies of the next launch as batched-extraction entries
